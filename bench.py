@@ -1,0 +1,226 @@
+"""Benchmark: atom-evals/s of the sparse-GP force evaluation (E, F, stress
+and local sigma per atom) on one MI355X, chosen form vs the FP64 drop-in.
+
+Workload (default): a 32,000-atom two-species random 50/50 FCC alloy
+(a = 4.0 A, 20^3 cells, displacements 0.08 A); B2 with 12 radial functions
+per species, l_max 6, cutoff 6.0 A, quadratic cutoff (d = 2100); 2,000
+sparse environments drawn from the structure (seed 3); alpha ~ N(0, 1e-2)
+(seed 4); NormalizedDotProduct sigma 2, power 2; L_inv from the FP64
+Cholesky of Kuu + 1e-8 I.
+
+One step = neighbour list + descriptors + kernel contraction + adjoint
+forces/stress + local variances, from positions already resident in HBM to
+results on the host. Prints one JSON line.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "atom-evals/s (E,F,σ) chosen vs FP64 drop-in at stated error; p50/p99 latency"
+PEAK = {"bf16": 2500.0, "fp64": 78.6}  # MI355X dense TFLOP/s
+
+
+class Hip:
+    """hipMalloc / hipMemcpy through ctypes, for inputs resident in HBM."""
+
+    def __init__(self):
+        self.L = C.CDLL("libamdhip64.so")
+        self.L.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        self.L.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        self.L.hipFree.argtypes = [C.c_void_p]
+        self.L.hipDeviceSynchronize.argtypes = []
+
+    def upload(self, a):
+        p = C.c_void_p()
+        assert self.L.hipMalloc(C.byref(p), a.nbytes) == 0
+        assert self.L.hipMemcpy(p, a.ctypes.data, a.nbytes, 1) == 0
+        return p
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_model(C_, cell, species, pos, n_sparse, seed_sparse=3, seed_alpha=4):
+    from flare_amd import _C_flare as CF
+    calc = CF.B2("chebyshev", "quadratic", [0.0, 6.0], [], [2, 12, 6], np.full((2, 2), 6.0))
+    s = CF.Structure(cell, species, pos, 6.0, [calc])
+    rng = np.random.default_rng(seed_sparse)
+    atoms = sorted(rng.choice(len(pos), n_sparse, replace=False).tolist())
+    sgp = CF.SparseGP([CF.NormalizedDotProduct(2.0, 2.0)], 0.01, 0.05, 0.005)
+    sgp.add_specific_environments(s, atoms)
+    sgp.update_L_inverse()
+    sgp.set_alpha(np.random.default_rng(seed_alpha).normal(scale=0.1, size=sgp.n_sparse))
+    return calc, sgp
+
+
+def run_form(lib, ctx, model_h, calc, cell, sp_dev, pos_dev, n, form, steps, warmup):
+    from flare_amd import _lib
+    settings, keep = calc._settings()
+    cellv = _lib.f64(cell.reshape(-1))
+    flags = _lib.PREDICT_MEAN | _lib.PREDICT_LOCAL
+    mean = np.zeros(1 + 3 * n + 6)
+    var = np.zeros(n)
+    lat = []
+    stage = {k: [0.0, 0] for k in _lib.Context.STAGES}
+
+    def step():
+        h = C.c_void_p()
+        _lib.check(lib.fm_structure_create_device(ctx.h, _lib.ptr(cellv), n, sp_dev, pos_dev,
+                                                  6.0, C.byref(h)))
+        _lib.check(lib.fm_structure_compute_b2(h, C.byref(settings), form))
+        _lib.check(lib.fm_predict(model_h, h, flags, form, _lib.ptr(mean), _lib.ptr(var), None))
+        lib.fm_structure_destroy(h)
+
+    for _ in range(warmup):
+        step()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        a = time.perf_counter()
+        step()
+        lat.append(time.perf_counter() - a)
+        for k, (ms, nl) in ctx.timings().items():
+            stage[k][0] += ms
+            stage[k][1] += nl
+    ctx.synchronize()
+    total = time.perf_counter() - t0
+    del keep
+    return dict(total=total, lat=np.array(lat), stage=stage, mean=mean.copy(), var=var.copy())
+
+
+def cpu_baseline(sgp, threads):
+    """The oracle's scalable mode (cell list + per-atom adjoint with K.alpha)
+    on a 4,000-atom sample of the same alloy and model, FP64, OpenMP."""
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    from flare_amd.workloads import random_alloy
+    from oracle import ref_cpu as R
+    cell, species, pos = random_alloy((10, 10, 10), a=4.0, nos=2, sigma=0.08, seed=12)
+    cd = sgp.sparse_descriptors[0]
+    m = R.Model(2.0, 2.0, cd.descriptors, cd.descriptor_norms, sgp.alpha, L_inv=sgp.L_inv,
+                fast=True)
+    t0 = time.perf_counter()
+    nb = R.Neighbors(cell, species, pos, 6.0, "cells", fast=True)
+    m.predict_scalable(nb, 2, 12, 6, np.full((2, 2), 6.0))
+    dt = time.perf_counter() - t0
+    return {"value": len(pos) / dt, "unit": "atom-evals/s", "cores": threads, "kind": "port",
+            "sample": f"{len(pos)}-atom 2-species FCC alloy (10^3 cells, a = 4.0 A), same model "
+                      f"(2000 sparse, d = 2100), one FP64 evaluation in {dt:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--sparse", type=int, default=2000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(the multi-GPU path is a design, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from flare_amd import _lib
+    from flare_amd.workloads import fcc_alloy_32k
+    lib = _lib.load()
+    ctx = _lib.default_context()
+    cell, species, pos = fcc_alloy_32k()
+    n = len(pos)
+    log(f"workload: {n} atoms, building model ...")
+    calc, sgp = build_model(None, cell, species, pos, args.sparse)
+    hip = Hip()
+    pos_dev = hip.upload(np.ascontiguousarray(pos.reshape(-1)))
+    sp_dev = hip.upload(np.ascontiguousarray(species.astype(np.int32)))
+    hip.L.hipDeviceSynchronize()
+    model_h = sgp._model(0, ctx).h
+
+    res = {}
+    for name, form in (("fp64", _lib.FORM_FP64), ("chosen", _lib.FORM_CHOSEN)):
+        log(f"timing form {name} ...")
+        res[name] = run_form(lib, ctx, model_h, calc, cell, sp_dev, pos_dev, n, form,
+                             args.steps, args.warmup)
+        r = res[name]
+        log(f"  {name}: {r['total'] / args.steps * 1e3:.2f} ms/step, "
+            + ", ".join(f"{k} {v[0] / args.steps:.3f} ms" for k, v in r['stage'].items()))
+
+    f64, ch = res["fp64"], res["chosen"]
+    dE = abs(ch["mean"][0] - f64["mean"][0]) / n * 1e3
+    dF = np.abs(ch["mean"][1:1 + 3 * n] - f64["mean"][1:1 + 3 * n]).max() * 1e3
+    s64 = np.sqrt(np.abs(f64["var"]))
+    s32 = np.sqrt(np.abs(ch["var"]))
+    srel = np.abs(s32 - s64) / np.maximum(s64, 1e-300)
+    d = 2100
+    M_by = [len(x) for x in sgp.sparse_descriptors[0].descriptor_norms]
+    n_by = [int((species == t).sum()) for t in range(2)]
+    kflops = sum(2.0 * a * b * d for a, b in zip(n_by, M_by))  # S contraction, all species
+
+    def roof(r, peak):
+        ms, nl = r["stage"]["kernel_contraction"]
+        per_launch_ms = ms / max(nl, 1)
+        per_launch_flops = kflops / len(M_by)
+        achieved = per_launch_flops / (per_launch_ms * 1e-3) / 1e12
+        return {"bound": "mfma", "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s",
+                "frac": round(achieved / peak, 4), "traffic": None,
+                "kernel": "kernel contraction S = B X_hat^T + epilogue",
+                "avg_launch_ms": round(per_launch_ms, 4), "flops_per_launch": per_launch_flops}
+
+    def summary(r, peak):
+        return {"atom_evals_per_s": n * args.steps / r["total"],
+                "ms_per_step": r["total"] / args.steps * 1e3,
+                "p50_ms": float(np.percentile(r["lat"], 50) * 1e3),
+                "p99_ms": float(np.percentile(r["lat"], 99) * 1e3),
+                "stage_ms": {k: round(v[0] / args.steps, 4) for k, v in r["stage"].items()},
+                "roofline": roof(r, peak)}
+
+    chosen = summary(ch, PEAK["bf16"])
+    fp64 = summary(f64, PEAK["fp64"])
+    out = {
+        "metric": METRIC,
+        "value": chosen["atom_evals_per_s"],
+        "unit": "atom-evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": chosen["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32+bf16x3",
+        "data": "synthetic",
+        "config": {"workload": "32,000-atom 2-species FCC alloy, B2 [2,12,6] rc 6.0 (d=2100), "
+                               "2,000 sparse envs, NDP sigma 2 p 2; E/F/stress/local sigma",
+                   "form": "chosen: FP32 descriptors + adjoint, split-BF16x3 MFMA kernel "
+                           "contraction (FP32 accumulate), FP64 variance contraction",
+                   "n_atoms": n, "n_sparse": int(sum(M_by)), "n_descriptors": d},
+        "p50_ms": chosen["p50_ms"],
+        "p99_ms": chosen["p99_ms"],
+        "roofline": chosen["roofline"],
+        "stage_ms": chosen["stage_ms"],
+        "fp64_dropin": fp64,
+        "speedup_vs_fp64": chosen["atom_evals_per_s"] / fp64["atom_evals_per_s"],
+        "error_vs_fp64": {"energy_meV_per_atom": dE, "force_max_meV_per_A": dF,
+                          "sigma_rel_max": float(srel.max()),
+                          "sigma_rel_median": float(np.median(srel))},
+    }
+    if not args.no_cpu_baseline:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+        log(f"cpu baseline on {threads} threads ...")
+        out["cpu_baseline"] = cpu_baseline(sgp, threads)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,564 @@
+// The C ABI of libflare_mi (include/flare_mi.h): contexts, structures,
+// models and predictions. Host-side setup (3x3 cell algebra, model upload)
+// lives here; every per-atom and per-bond stage runs on the device.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "fm_internal.hpp"
+
+namespace {
+thread_local std::string g_last_error;
+
+template <typename F>
+int guarded(F &&f) {
+  try {
+    f();
+    return FM_OK;
+  } catch (const fm::Error &e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::exception &e) {
+    g_last_error = e.what();
+    return FM_ERR_INVALID;
+  }
+}
+
+// Cofactor inverse and first-row determinant of a 3x3 matrix, the cell
+// algebra of structure.cpp:16-21 (cell_transpose_inverse, cell_dot_inverse,
+// volume = |det cell|). This translation unit is built with
+// -ffp-contract=off so these match the oracle's expressions bit for bit.
+double det3(const double *m) {
+  return m[0] * (m[4] * m[8] - m[5] * m[7]) - m[1] * (m[3] * m[8] - m[5] * m[6]) +
+         m[2] * (m[3] * m[7] - m[4] * m[6]);
+}
+
+void inverse3(const double *m, double *o) {
+  double c00 = m[4] * m[8] - m[5] * m[7];
+  double c01 = m[5] * m[6] - m[3] * m[8];
+  double c02 = m[3] * m[7] - m[4] * m[6];
+  double det = m[0] * c00 + m[1] * c01 + m[2] * c02;
+  double inv = 1.0 / det;
+  o[0] = c00 * inv;
+  o[1] = (m[2] * m[7] - m[1] * m[8]) * inv;
+  o[2] = (m[1] * m[5] - m[2] * m[4]) * inv;
+  o[3] = c01 * inv;
+  o[4] = (m[0] * m[8] - m[2] * m[6]) * inv;
+  o[5] = (m[2] * m[3] - m[0] * m[5]) * inv;
+  o[6] = c02 * inv;
+  o[7] = (m[1] * m[6] - m[0] * m[7]) * inv;
+  o[8] = (m[0] * m[4] - m[1] * m[3]) * inv;
+}
+
+// structure.cpp:135-165: the smallest in-plane height of the cell faces.
+double single_sweep_cutoff(const double *c) {
+  const double *v1 = c, *v2 = c + 3, *v3 = c + 6;
+  double ab = v1[0] * v2[0] + v1[1] * v2[1] + v1[2] * v2[2];
+  double ac = v1[0] * v3[0] + v1[1] * v3[1] + v1[2] * v3[2];
+  double bc = v2[0] * v3[0] + v2[1] * v3[1] + v2[2] * v3[2];
+  double a = std::sqrt(v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2]);
+  double b = std::sqrt(v2[0] * v2[0] + v2[1] * v2[1] + v2[2] * v2[2]);
+  double cc = std::sqrt(v3[0] * v3[0] + v3[1] * v3[1] + v3[2] * v3[2]);
+  double cand[6] = {a * std::sqrt(1 - std::pow(ab / (a * b), 2)),
+                    b * std::sqrt(1 - std::pow(ab / (a * b), 2)),
+                    a * std::sqrt(1 - std::pow(ac / (a * cc), 2)),
+                    cc * std::sqrt(1 - std::pow(ac / (a * cc), 2)),
+                    b * std::sqrt(1 - std::pow(bc / (b * cc), 2)),
+                    cc * std::sqrt(1 - std::pow(bc / (b * cc), 2))};
+  double best = cand[0];
+  for (int i = 0; i < 6; i++)
+    if (cand[i] < best) best = cand[i];
+  return best;
+}
+
+void set_cell(fm_structure s, const double *cell, double cutoff) {
+  for (int i = 0; i < 9; i++) s->cell[i] = cell[i];
+  double ct[9];
+  for (int r = 0; r < 3; r++)
+    for (int c = 0; c < 3; c++) ct[3 * r + c] = cell[3 * c + r];
+  inverse3(ct, s->cell_t_inv);
+  for (int r = 0; r < 3; r++)
+    for (int c = 0; c < 3; c++)
+      s->cell_dot[3 * r + c] = (cell[3 * r] * cell[3 * c] + cell[3 * r + 1] * cell[3 * c + 1]) +
+                               cell[3 * r + 2] * cell[3 * c + 2];
+  inverse3(s->cell_dot, s->cell_dot_inv);
+  s->volume = std::fabs(det3(cell));
+  s->single_sweep_cutoff = single_sweep_cutoff(cell);
+  s->cutoff = cutoff;
+  s->sweep = (int)std::ceil(cutoff / s->single_sweep_cutoff);
+  FM_CHECK(std::isfinite(s->volume) && s->volume > 0, FM_ERR_INVALID, "cell is singular");
+  FM_CHECK(cutoff > 0 && std::isfinite(cutoff), FM_ERR_INVALID, "cutoff must be positive");
+  FM_CHECK(s->sweep >= 1 && s->sweep <= 64, FM_ERR_CAPACITY,
+           "cutoff spans too many periodic images of the cell");
+}
+
+template <typename T>
+void d2h(T *dst, const fm::DevBuf &src, size_t count, hipStream_t st) {
+  if (dst && count) FM_HIP(hipMemcpyAsync(dst, src.as<T>(), count * sizeof(T), hipMemcpyDeviceToHost, st));
+}
+
+__global__ void k_check_positions(int n, const double *pos, const int *species, int *bad) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int k = 0; k < 3; k++)
+    if (!isfinite(pos[3 * i + k])) atomicOr(bad, 1);
+  if (species[i] < 0) atomicOr(bad, 2);
+}
+
+}  // namespace
+
+namespace fm {
+
+Pool::~Pool() {
+  for (auto &kv : free_)
+    for (void *p : kv.second) (void)hipFree(p);
+}
+
+void *Pool::get(size_t bytes, size_t *cls) {
+  size_t c = 256;
+  while (c < bytes) c <<= 1;
+  *cls = c;
+  auto it = free_.find(c);
+  if (it != free_.end() && !it->second.empty()) {
+    void *p = it->second.back();
+    it->second.pop_back();
+    cached_ -= c;
+    return p;
+  }
+  void *p = nullptr;
+  hipError_t e = hipMalloc(&p, c);
+  if (e != hipSuccess) {
+    // Drop the cache and retry once before giving up.
+    for (auto &kv : free_)
+      for (void *q : kv.second) (void)hipFree(q);
+    free_.clear();
+    cached_ = 0;
+    FM_HIP(hipMalloc(&p, c));
+  }
+  return p;
+}
+
+void Pool::put(void *p, size_t cls) {
+  free_[cls].push_back(p);
+  cached_ += cls;
+}
+
+}  // namespace fm
+
+void fm_context_s::mark_begin(int s) {
+  (void)hipEventRecord(ev_begin[s], stream);
+  launches[s] = 0;
+}
+void fm_context_s::mark_end(int s) {
+  (void)hipEventRecord(ev_end[s], stream);
+  recorded[s] = true;
+}
+
+extern "C" {
+
+const char *fm_last_error(void) { return g_last_error.c_str(); }
+
+int fm_context_create(int device, fm_context *out) {
+  return guarded([&] {
+    FM_CHECK(out, FM_ERR_INVALID, "null output handle");
+    FM_HIP(hipSetDevice(device));
+    auto *c = new fm_context_s();
+    c->device = device;
+    FM_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    for (int s = 0; s < fm::ST_COUNT; s++) {
+      FM_HIP(hipEventCreate(&c->ev_begin[s]));
+      FM_HIP(hipEventCreate(&c->ev_end[s]));
+    }
+    *out = c;
+  });
+}
+
+int fm_context_destroy(fm_context ctx) {
+  return guarded([&] {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (int s = 0; s < fm::ST_COUNT; s++) {
+      (void)hipEventDestroy(ctx->ev_begin[s]);
+      (void)hipEventDestroy(ctx->ev_end[s]);
+    }
+    hipStream_t st = ctx->stream;
+    delete ctx;
+    (void)hipStreamDestroy(st);
+  });
+}
+
+int fm_context_synchronize(fm_context ctx) {
+  return guarded([&] { FM_HIP(hipStreamSynchronize(ctx->stream)); });
+}
+
+int fm_context_stream(fm_context ctx, void **stream) {
+  return guarded([&] { *stream = (void *)ctx->stream; });
+}
+
+static int structure_create(fm_context ctx, const double *cell, int n, const int32_t *species,
+                            const double *positions, double cutoff, bool device_ptrs,
+                            fm_structure *out) {
+  return guarded([&] {
+    FM_CHECK(ctx && out && cell, FM_ERR_INVALID, "null argument");
+    FM_CHECK(n >= 0, FM_ERR_INVALID, "negative atom count");
+    FM_CHECK(n == 0 || (species && positions), FM_ERR_INVALID, "null species or positions");
+    FM_HIP(hipSetDevice(ctx->device));
+    auto *s = new fm_structure_s();
+    s->ctx = ctx;
+    s->n = n;
+    try {
+      set_cell(s, cell, cutoff);
+      fm::Pool *pool = &ctx->pool;
+      s->species.alloc(pool, std::max<size_t>(1, (size_t)n * sizeof(int)));
+      s->positions.alloc(pool, std::max<size_t>(1, 3 * (size_t)n * sizeof(double)));
+      hipMemcpyKind kind = device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+      if (n > 0) {
+        FM_HIP(hipMemcpyAsync(s->species.as<int>(), species, (size_t)n * sizeof(int), kind,
+                              ctx->stream));
+        FM_HIP(hipMemcpyAsync(s->positions.as<double>(), positions, 3 * (size_t)n * sizeof(double),
+                              kind, ctx->stream));
+        fm::DevBuf bad(pool, sizeof(int));
+        FM_HIP(hipMemsetAsync(bad.as<int>(), 0, sizeof(int), ctx->stream));
+        hipLaunchKernelGGL(k_check_positions, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n,
+                           s->positions.as<double>(), s->species.as<int>(), bad.as<int>());
+        int hb = 0;
+        FM_HIP(hipMemcpyAsync(&hb, bad.as<int>(), sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        FM_HIP(hipStreamSynchronize(ctx->stream));
+        FM_CHECK(!(hb & 1), FM_ERR_INVALID, "positions contain NaN or Inf");
+        FM_CHECK(!(hb & 2), FM_ERR_INVALID, "species must be non-negative");
+      }
+      ctx->mark_begin(fm::ST_NEIGH);
+      if (n > 0) {
+        fm::build_neighbors(s);
+      } else {
+        s->cumulative.alloc(pool, sizeof(int));
+        FM_HIP(hipMemsetAsync(s->cumulative.as<int>(), 0, sizeof(int), ctx->stream));
+      }
+      ctx->mark_end(fm::ST_NEIGH);
+    } catch (...) {
+      delete s;
+      throw;
+    }
+    *out = s;
+  });
+}
+
+int fm_structure_create(fm_context ctx, const double *cell, int n_atoms, const int32_t *species,
+                        const double *positions, double cutoff, fm_structure *out) {
+  return structure_create(ctx, cell, n_atoms, species, positions, cutoff, false, out);
+}
+
+int fm_structure_create_device(fm_context ctx, const double *cell, int n_atoms,
+                               const int32_t *species_dev, const double *positions_dev,
+                               double cutoff, fm_structure *out) {
+  return structure_create(ctx, cell, n_atoms, species_dev, positions_dev, cutoff, true, out);
+}
+
+int fm_structure_destroy(fm_structure s) {
+  return guarded([&] {
+    if (!s) return;
+    (void)hipStreamSynchronize(s->ctx->stream);
+    delete s;
+  });
+}
+
+int fm_structure_info(fm_structure s, int *sweep, int64_t *n_neighbors, double *volume,
+                      double *single_sweep_cutoff) {
+  return guarded([&] {
+    FM_CHECK(s, FM_ERR_INVALID, "null structure");
+    if (sweep) *sweep = s->sweep;
+    if (n_neighbors) *n_neighbors = s->n_nb;
+    if (volume) *volume = s->volume;
+    if (single_sweep_cutoff) *single_sweep_cutoff = s->single_sweep_cutoff;
+  });
+}
+
+int fm_structure_get_neighbors(fm_structure s, double *wrapped, int32_t *count, int32_t *cumulative,
+                               int32_t *structure_indices, int32_t *neighbor_species,
+                               double *relative) {
+  return guarded([&] {
+    FM_CHECK(s, FM_ERR_INVALID, "null structure");
+    hipStream_t st = s->ctx->stream;
+    if (s->n > 0) {
+      d2h(wrapped, s->wrapped, 3 * (size_t)s->n, st);
+      d2h(count, s->count, (size_t)s->n, st);
+    }
+    d2h(cumulative, s->cumulative, (size_t)s->n + 1, st);
+    d2h(structure_indices, s->sidx, (size_t)s->n_nb, st);
+    d2h(neighbor_species, s->nspecies, (size_t)s->n_nb, st);
+    d2h(relative, s->rel, 4 * (size_t)s->n_nb, st);
+    FM_HIP(hipStreamSynchronize(st));
+  });
+}
+
+int fm_structure_compute_b2(fm_structure s, const fm_b2_settings *st_in, int form) {
+  return guarded([&] {
+    FM_CHECK(s && st_in && st_in->cutoffs, FM_ERR_INVALID, "null argument");
+    FM_CHECK(form == FM_FORM_FP64 || form == FM_FORM_CHOSEN, FM_ERR_INVALID, "unknown form");
+    FM_CHECK(st_in->n_species >= 1 && st_in->n_max >= 1 && st_in->l_max >= 0, FM_ERR_INVALID,
+             "descriptor settings must be positive");
+    FM_CHECK(st_in->cutoff_function == FM_CUTOFF_QUADRATIC ||
+                 st_in->cutoff_function == FM_CUTOFF_COSINE,
+             FM_ERR_INVALID, "cutoff function must be quadratic or cosine");
+    fm_context ctx = s->ctx;
+    FM_HIP(hipSetDevice(ctx->device));
+    hipStream_t stream = ctx->stream;
+    fm::Pool *pool = &ctx->pool;
+    s->nos = st_in->n_species;
+    s->N = st_in->n_max;
+    s->lmax = st_in->l_max;
+    s->cutkind = st_in->cutoff_function;
+    s->r1 = st_in->radial_r1;
+    s->form = form;
+    s->cutoffs.assign(st_in->cutoffs, st_in->cutoffs + s->nos * s->nos);
+    for (double c : s->cutoffs)
+      FM_CHECK(c > s->r1 && std::isfinite(c), FM_ERR_INVALID, "pair cutoffs must exceed r1");
+    int nr = s->nos * s->N;
+    s->nd = nr * (nr + 1) / 2 * (s->lmax + 1);
+    if (s->n > 0) {
+      // Species must lie in [0, n_species) for this descriptor.
+      std::vector<int> sp(s->n);
+      FM_HIP(hipMemcpyAsync(sp.data(), s->species.as<int>(), (size_t)s->n * sizeof(int),
+                            hipMemcpyDeviceToHost, stream));
+      FM_HIP(hipStreamSynchronize(stream));
+      for (int v : sp) FM_CHECK(v >= 0 && v < s->nos, FM_ERR_INVALID, "species outside [0, n_species)");
+    }
+    std::vector<int> pairs;
+    for (int a = 0; a < nr; a++)
+      for (int b = a; b < nr; b++) {
+        pairs.push_back(a);
+        pairs.push_back(b);
+      }
+    s->pair_table.alloc(pool, pairs.size() * sizeof(int));
+    FM_HIP(hipMemcpyAsync(s->pair_table.as<int>(), pairs.data(), pairs.size() * sizeof(int),
+                          hipMemcpyHostToDevice, stream));
+    ctx->mark_begin(fm::ST_DESC);
+    if (s->n > 0) fm::build_bonds(s);
+    else {
+      s->n_by_type.assign(s->nos, 0);
+      s->type_off.assign(s->nos + 1, 0);
+      s->n_bonds = 0;
+    }
+    size_t esz = form == FM_FORM_FP64 ? sizeof(double) : sizeof(float);
+    s->B.alloc(pool, std::max<size_t>(1, (size_t)s->n * s->nd * esz));
+    s->norm.alloc(pool, std::max<size_t>(1, (size_t)s->n * sizeof(double)));
+    s->nn.alloc(pool, std::max<size_t>(1, (size_t)s->n * sizeof(double)));
+    if (s->n > 0) fm::launch_b2(s);
+    ctx->mark_end(fm::ST_DESC);
+    // Neighbour counts by species (b2.cpp:106-112).
+    s->nb_by_type.assign(s->nos, 0);
+    if (s->n > 0) {
+      std::vector<int> bo(s->n + 1), sp(s->n);
+      FM_HIP(hipMemcpyAsync(bo.data(), s->bond_off.as<int>(), (s->n + 1) * sizeof(int),
+                            hipMemcpyDeviceToHost, stream));
+      FM_HIP(hipMemcpyAsync(sp.data(), s->species.as<int>(), s->n * sizeof(int),
+                            hipMemcpyDeviceToHost, stream));
+      FM_HIP(hipStreamSynchronize(stream));
+      for (int i = 0; i < s->n; i++) s->nb_by_type[sp[i]] += bo[i + 1] - bo[i];
+    }
+    s->has_b2 = true;
+  });
+}
+
+int fm_structure_b2_sizes(fm_structure s, int *n_descriptors, int32_t *n_clusters_by_type,
+                          int32_t *n_neighbors_by_type) {
+  return guarded([&] {
+    FM_CHECK(s && s->has_b2, FM_ERR_STATE, "descriptors have not been computed");
+    if (n_descriptors) *n_descriptors = s->nd;
+    for (int t = 0; t < s->nos; t++) {
+      if (n_clusters_by_type) n_clusters_by_type[t] = s->n_by_type[t];
+      if (n_neighbors_by_type) n_neighbors_by_type[t] = s->nb_by_type[t];
+    }
+  });
+}
+
+int fm_structure_get_b2(fm_structure s, int t, double *descriptors, double *norms,
+                        int32_t *neighbor_counts, int32_t *cumulative_neighbor_counts,
+                        int32_t *atom_indices, int32_t *neighbor_indices,
+                        double *neighbor_coordinates) {
+  return guarded([&] {
+    FM_CHECK(s && s->has_b2, FM_ERR_STATE, "descriptors have not been computed");
+    FM_CHECK(t >= 0 && t < s->nos, FM_ERR_INVALID, "species out of range");
+    hipStream_t st = s->ctx->stream;
+    int ns = s->n_by_type[t], ro = s->type_off[t], nd = s->nd;
+    if (ns == 0) return;
+    if (descriptors) {
+      if (s->form == FM_FORM_FP64) {
+        FM_HIP(hipMemcpyAsync(descriptors, s->B.as<double>() + (size_t)ro * nd,
+                              (size_t)ns * nd * sizeof(double), hipMemcpyDeviceToHost, st));
+      } else {
+        std::vector<float> tmp((size_t)ns * nd);
+        FM_HIP(hipMemcpyAsync(tmp.data(), s->B.as<float>() + (size_t)ro * nd,
+                              tmp.size() * sizeof(float), hipMemcpyDeviceToHost, st));
+        FM_HIP(hipStreamSynchronize(st));
+        for (size_t k = 0; k < tmp.size(); k++) descriptors[k] = tmp[k];
+      }
+    }
+    if (norms)
+      FM_HIP(hipMemcpyAsync(norms, s->norm.as<double>() + ro, (size_t)ns * sizeof(double),
+                            hipMemcpyDeviceToHost, st));
+    std::vector<int> aor(ns), bo(s->n + 1);
+    FM_HIP(hipMemcpyAsync(aor.data(), s->atom_of_row.as<int>() + ro, ns * sizeof(int),
+                          hipMemcpyDeviceToHost, st));
+    FM_HIP(hipMemcpyAsync(bo.data(), s->bond_off.as<int>(), (s->n + 1) * sizeof(int),
+                          hipMemcpyDeviceToHost, st));
+    std::vector<int> bq(s->n_bonds), sidx(s->n_nb);
+    std::vector<double> rel(4 * (size_t)s->n_nb);
+    if (neighbor_indices || neighbor_coordinates) {
+      if (s->n_bonds)
+        FM_HIP(hipMemcpyAsync(bq.data(), s->bond_q.as<int>(), bq.size() * sizeof(int),
+                              hipMemcpyDeviceToHost, st));
+      if (s->n_nb) {
+        FM_HIP(hipMemcpyAsync(sidx.data(), s->sidx.as<int>(), sidx.size() * sizeof(int),
+                              hipMemcpyDeviceToHost, st));
+        FM_HIP(hipMemcpyAsync(rel.data(), s->rel.as<double>(), rel.size() * sizeof(double),
+                              hipMemcpyDeviceToHost, st));
+      }
+    }
+    FM_HIP(hipStreamSynchronize(st));
+    int cum = 0;
+    for (int r = 0; r < ns; r++) {
+      int i = aor[r];
+      int nb = bo[i + 1] - bo[i];
+      if (neighbor_counts) neighbor_counts[r] = nb;
+      if (cumulative_neighbor_counts) cumulative_neighbor_counts[r] = cum;
+      if (atom_indices) atom_indices[r] = i;
+      for (int b = 0; b < nb; b++) {
+        int q = bq.empty() ? 0 : bq[bo[i] + b];
+        if (neighbor_indices) neighbor_indices[cum + b] = sidx[q];
+        if (neighbor_coordinates)
+          for (int k = 0; k < 3; k++) neighbor_coordinates[3 * (size_t)(cum + b) + k] = rel[4 * (size_t)q + 1 + k];
+      }
+      cum += nb;
+    }
+  });
+}
+
+int fm_model_create(fm_context ctx, const fm_model_desc *d, fm_model *out) {
+  return guarded([&] {
+    FM_CHECK(ctx && d && out, FM_ERR_INVALID, "null argument");
+    FM_CHECK(d->n_species >= 1 && d->n_descriptors >= 1, FM_ERR_INVALID, "empty model");
+    FM_CHECK(d->n_sparse_by_type && d->alpha, FM_ERR_INVALID, "null sparse counts or alpha");
+    FM_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    auto *m = new fm_model_s();
+    m->ctx = ctx;
+    m->sigma = d->sigma;
+    m->power = d->power;
+    m->nos = d->n_species;
+    m->nd = d->n_descriptors;
+    int nd = m->nd, M = 0;
+    for (int t = 0; t < m->nos; t++) {
+      FM_CHECK(d->n_sparse_by_type[t] >= 0, FM_ERR_INVALID, "negative sparse count");
+      m->off.push_back(M);
+      m->M_by_type.push_back(d->n_sparse_by_type[t]);
+      M += d->n_sparse_by_type[t];
+    }
+    m->M = M;
+    FM_CHECK(M == 0 || (d->sparse_descriptors && d->sparse_norms), FM_ERR_INVALID,
+             "null sparse descriptors");
+    m->host_alpha.assign(d->alpha, d->alpha + M);
+    m->has_L = d->L_inv != nullptr;
+    fm::Pool *pool = &ctx->pool;
+    for (int t = 0; t < m->nos; t++) {
+      int Ms = m->M_by_type[t], c0 = m->off[t];
+      std::vector<double> X((size_t)Ms * nd, 0.0), XT((size_t)nd * Ms, 0.0), a(Ms, 0.0);
+      std::vector<float> X32((size_t)Ms * nd), XT32((size_t)nd * Ms);
+      for (int j = 0; j < Ms; j++) {
+        double nrm = d->sparse_norms[c0 + j];
+        // Empty sparse environments contribute nothing
+        // (normalized_dot_product.cpp:158-162).
+        if (nrm < 1e-8) continue;
+        for (int k = 0; k < nd; k++) {
+          double v = d->sparse_descriptors[(size_t)(c0 + j) * nd + k] / nrm;
+          X[(size_t)j * nd + k] = v;
+          XT[(size_t)k * Ms + j] = v;
+        }
+        a[j] = d->alpha[c0 + j];
+      }
+      for (size_t k = 0; k < X.size(); k++) X32[k] = (float)X[k];
+      for (size_t k = 0; k < XT.size(); k++) XT32[k] = (float)XT[k];
+      auto up = [&](fm::DevBuf &buf, const void *src, size_t bytes) {
+        buf.alloc(pool, std::max<size_t>(1, bytes));
+        if (bytes) FM_HIP(hipMemcpyAsync(buf.as<void>(), src, bytes, hipMemcpyHostToDevice, st));
+      };
+      m->X.emplace_back();
+      m->XT.emplace_back();
+      m->X32.emplace_back();
+      m->XT32.emplace_back();
+      m->alpha.emplace_back();
+      m->L.emplace_back();
+      up(m->X.back(), X.data(), X.size() * sizeof(double));
+      up(m->XT.back(), XT.data(), XT.size() * sizeof(double));
+      up(m->X32.back(), X32.data(), X32.size() * sizeof(float));
+      up(m->XT32.back(), XT32.data(), XT32.size() * sizeof(float));
+      up(m->alpha.back(), a.data(), a.size() * sizeof(double));
+      int rows = 0, lower = 1;
+      if (m->has_L && Ms > 0) {
+        // Rows [c0, r_end) of the L_inv column block; the reference takes
+        // the whole kernel block (sparse_gp.cpp:121-130), whose rows outside
+        // this range are zero in these columns.
+        int r_begin = c0, r_end = c0 + Ms;
+        for (int r = 0; r < c0; r++)
+          for (int j = 0; j < Ms; j++)
+            if (d->L_inv[(size_t)r * M + c0 + j] != 0.0) r_begin = std::min(r_begin, r);
+        for (int r = c0 + Ms; r < M; r++)
+          for (int j = 0; j < Ms; j++)
+            if (d->L_inv[(size_t)r * M + c0 + j] != 0.0) r_end = std::max(r_end, r + 1);
+        FM_CHECK(r_begin == c0, FM_ERR_INVALID,
+                 "L_inv has entries above the diagonal block of a species");
+        for (int r = 0; r < Ms; r++)
+          for (int j = r + 1; j < Ms; j++)
+            if (d->L_inv[(size_t)(c0 + r) * M + c0 + j] != 0.0) lower = 0;
+        rows = r_end - c0;
+        std::vector<double> L((size_t)rows * Ms);
+        for (int r = 0; r < rows; r++)
+          for (int j = 0; j < Ms; j++) L[(size_t)r * Ms + j] = d->L_inv[(size_t)(c0 + r) * M + c0 + j];
+        up(m->L.back(), L.data(), L.size() * sizeof(double));
+        FM_HIP(hipStreamSynchronize(st));
+      }
+      m->L_rows.push_back(rows);
+      m->L_lower.push_back(lower);
+      FM_HIP(hipStreamSynchronize(st));
+    }
+    *out = m;
+  });
+}
+
+int fm_model_destroy(fm_model m) {
+  return guarded([&] {
+    if (!m) return;
+    (void)hipStreamSynchronize(m->ctx->stream);
+    delete m;
+  });
+}
+
+int fm_predict(fm_model m, fm_structure s, int flags, int form, double *mean_efs, double *local_var,
+               double *local_energy) {
+  return guarded([&] {
+    FM_CHECK(m && s, FM_ERR_INVALID, "null argument");
+    FM_CHECK(m->ctx == s->ctx, FM_ERR_INVALID, "model and structure belong to different contexts");
+    FM_HIP(hipSetDevice(s->ctx->device));
+    fm::predict(m, s, flags, form, mean_efs, local_var, local_energy);
+  });
+}
+
+int fm_last_timings(fm_context ctx, double *ms_out, int32_t *launches) {
+  return guarded([&] {
+    FM_CHECK(ctx, FM_ERR_INVALID, "null argument");
+    FM_HIP(hipStreamSynchronize(ctx->stream));
+    for (int s = 0; s < fm::ST_COUNT; s++) {
+      float ms = 0;
+      if (ctx->recorded[s]) FM_HIP(hipEventElapsedTime(&ms, ctx->ev_begin[s], ctx->ev_end[s]));
+      if (ms_out) ms_out[s] = ms;
+      if (launches) launches[s] = ctx->launches[s];
+    }
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,157 @@
+// Host-side internals of libflare_mi: error handling, the per-context device
+// buffer pool, and the structure / model objects behind the C ABI handles.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/flare_mi.h"
+
+namespace fm {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define FM_HIP(call)                                                          \
+  do {                                                                        \
+    hipError_t e_ = (call);                                                   \
+    if (e_ != hipSuccess)                                                     \
+      throw ::fm::Error(FM_ERR_HIP, std::string(#call) + ": " +              \
+                                        hipGetErrorString(e_));               \
+  } while (0)
+
+#define FM_CHECK(cond, code, msg)                     \
+  do {                                                \
+    if (!(cond)) throw ::fm::Error((code), (msg));    \
+  } while (0)
+
+// Size-class caching pool over hipMalloc. Buffers return to the pool when
+// the owning object dies; the stream order of the single context stream
+// keeps reuse safe.
+class Pool {
+ public:
+  ~Pool();
+  void *get(size_t bytes, size_t *cls);
+  void put(void *p, size_t cls);
+  size_t bytes_cached() const { return cached_; }
+
+ private:
+  std::map<size_t, std::vector<void *>> free_;
+  size_t cached_ = 0;
+};
+
+class DevBuf {
+ public:
+  DevBuf() = default;
+  DevBuf(Pool *pool, size_t bytes) { alloc(pool, bytes); }
+  ~DevBuf() { release(); }
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  DevBuf(DevBuf &&o) noexcept { *this = std::move(o); }
+  DevBuf &operator=(DevBuf &&o) noexcept {
+    release();
+    p_ = o.p_;
+    cls_ = o.cls_;
+    pool_ = o.pool_;
+    bytes_ = o.bytes_;
+    o.p_ = nullptr;
+    return *this;
+  }
+  void alloc(Pool *pool, size_t bytes) {
+    release();
+    pool_ = pool;
+    bytes_ = bytes;
+    p_ = pool->get(bytes ? bytes : 1, &cls_);
+  }
+  void ensure(Pool *pool, size_t bytes) {
+    if (!p_ || bytes_ < bytes || cls_ < bytes) alloc(pool, bytes);
+    bytes_ = bytes;
+  }
+  void release() {
+    if (p_ && pool_) pool_->put(p_, cls_);
+    p_ = nullptr;
+  }
+  template <typename T>
+  T *as() const {
+    return reinterpret_cast<T *>(p_);
+  }
+  size_t bytes() const { return bytes_; }
+  bool empty() const { return p_ == nullptr; }
+
+ private:
+  void *p_ = nullptr;
+  size_t cls_ = 0, bytes_ = 0;
+  Pool *pool_ = nullptr;
+};
+
+enum Stage { ST_NEIGH = 0, ST_DESC, ST_SGEMM, ST_GGEMM, ST_VAR, ST_ADJ, ST_COUNT };
+
+}  // namespace fm
+
+struct fm_context_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  fm::Pool pool;
+  hipEvent_t ev_begin[fm::ST_COUNT], ev_end[fm::ST_COUNT];
+  bool recorded[fm::ST_COUNT] = {};
+  int launches[fm::ST_COUNT] = {};
+  void mark_begin(int s);
+  void mark_end(int s);
+};
+
+struct fm_structure_s {
+  fm_context ctx = nullptr;
+  int n = 0;
+  double cell[9], cell_t_inv[9], cell_dot[9], cell_dot_inv[9];
+  double volume = 0, single_sweep_cutoff = 0, cutoff = 0;
+  int sweep = 0;
+  int64_t n_nb = 0;
+  fm::DevBuf species, positions, wrapped, count, cumulative, sidx, nspecies, rel;
+  // B2 state
+  bool has_b2 = false;
+  int form = FM_FORM_FP64;
+  int nos = 0, N = 0, lmax = 0, cutkind = 0, nd = 0;
+  double r1 = 0;
+  std::vector<double> cutoffs;
+  int64_t n_bonds = 0;
+  std::vector<int> n_by_type, nb_by_type, type_off;
+  fm::DevBuf d_cutoffs, bond_off, bond_q, in_off, in_bond, row_of_atom,
+      atom_of_row, pair_table, B, norm, nn;
+};
+
+struct fm_model_s {
+  fm_context ctx = nullptr;
+  double sigma = 1, power = 2;
+  int nos = 0, nd = 0, M = 0;
+  std::vector<int> M_by_type, off;
+  bool has_L = false;
+  std::vector<int> L_rows;   // rows of the L_inv column block of each species
+  std::vector<int> L_lower;  // 1 when that block is lower triangular
+  // Per species: X-hat (M_s x d) and X-hat^T (d x M_s) in FP64 and FP32,
+  // alpha (zero for empty sparse environments) and the L_inv block
+  // (L_rows x M_s, starting at the species' first row).
+  std::vector<fm::DevBuf> X, XT, X32, XT32, alpha, L;
+  std::vector<double> host_alpha;
+};
+
+namespace fm {
+// Launchers (fm_neighbors.hip)
+void build_neighbors(fm_structure s);
+void build_bonds(fm_structure s);
+// fm_descriptors.hip
+void launch_b2(fm_structure s);
+void launch_adjoint(fm_structure s, const void *G, int64_t g_ld, void *fbond,
+                    double *virial_atom);
+// fm_predict.cpp
+void predict(fm_model m, fm_structure s, int flags, int form, double *mean_efs,
+             double *local_var, double *local_energy);
+// scans (fm_neighbors.hip)
+int scan_exclusive(fm_context ctx, const int *in, int *out, int n);
+}  // namespace fm

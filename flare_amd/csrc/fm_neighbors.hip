@@ -1,0 +1,560 @@
+// Periodic neighbour list on the device, bit-identical to the reference
+// Structure::compute_neighbors (src/flare_pp/structure.cpp:52-118): same
+// image vector ((d + s1 a) + s2 b) + s3 c, same test dist < cutoff && dist != 0,
+// same order (j, s1, s2, s3). Built through a cell list: atoms are binned
+// along the fractional axes (bin width >= cutoff perpendicular to each face),
+// every candidate in the stencil is tested with the exact expression, images
+// outside [-sweep, sweep]^3 are dropped (the reference loop never visits them)
+// and each atom's list is sorted by its (j, s) key in LDS.
+//
+// This file is compiled with -ffp-contract=off so the FP64 expressions round
+// exactly as written.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "fm_device.hpp"
+#include "fm_internal.hpp"
+
+namespace fm {
+
+// ---------------------------------------------------------------------------
+// Scans
+// ---------------------------------------------------------------------------
+constexpr int kScanBlock = 1024;
+
+__device__ __forceinline__ int block_excl_scan(int v, int *sh, int &total) {
+  // 1024 threads = 16 waves.
+  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int x = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    int y = __shfl_up(x, off, 64);
+    if (lane >= off) x += y;
+  }
+  if (lane == 63) sh[wid] = x;
+  __syncthreads();
+  if (wid == 0) {
+    int w = lane < 16 ? sh[lane] : 0;
+#pragma unroll
+    for (int off = 1; off < 16; off <<= 1) {
+      int y = __shfl_up(w, off, 64);
+      if (lane >= off) w += y;
+    }
+    if (lane < 16) sh[lane] = w;
+  }
+  __syncthreads();
+  int base = wid ? sh[wid - 1] : 0;
+  total = sh[15];
+  __syncthreads();
+  return base + x - v;
+}
+
+__global__ void k_scan_block_sums(const int *in, int n, int *bsum) {
+  __shared__ int sh[16];
+  int i = blockIdx.x * kScanBlock + threadIdx.x;
+  int v = i < n ? in[i] : 0;
+  int tot;
+  block_excl_scan(v, sh, tot);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+__global__ void k_scan_sums(int *bsum, int nb) {
+  __shared__ int sh[16];
+  int carry = 0;
+  for (int b0 = 0; b0 < nb; b0 += kScanBlock) {
+    int i = b0 + threadIdx.x;
+    int v = i < nb ? bsum[i] : 0;
+    int tot;
+    int e = block_excl_scan(v, sh, tot);
+    if (i < nb) bsum[i] = carry + e;
+    carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) bsum[nb] = carry;
+}
+
+__global__ void k_scan_apply(const int *in, int n, const int *bsum, int *out) {
+  __shared__ int sh[16];
+  int i = blockIdx.x * kScanBlock + threadIdx.x;
+  int v = i < n ? in[i] : 0;
+  int tot;
+  int e = block_excl_scan(v, sh, tot);
+  if (i < n) out[i] = bsum[blockIdx.x] + e;
+  if (i == n - 1) out[n] = bsum[blockIdx.x] + e + v;
+}
+
+// out[0..n] = exclusive scan of in[0..n); out[n] = total. Returns the total
+// (synchronises the context stream).
+int scan_exclusive(fm_context ctx, const int *in, int *out, int n) {
+  if (n == 0) {
+    FM_HIP(hipMemsetAsync(out, 0, sizeof(int), ctx->stream));
+    return 0;
+  }
+  int nb = (n + kScanBlock - 1) / kScanBlock;
+  DevBuf bsum(&ctx->pool, (nb + 1) * sizeof(int));
+  hipLaunchKernelGGL(k_scan_block_sums, dim3(nb), dim3(kScanBlock), 0, ctx->stream, in, n,
+                     bsum.as<int>());
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanBlock), 0, ctx->stream, bsum.as<int>(), nb);
+  hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(kScanBlock), 0, ctx->stream, in, n,
+                     bsum.as<int>(), out);
+  FM_HIP(hipGetLastError());
+  int total = 0;
+  FM_HIP(hipMemcpyAsync(&total, out + n, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  FM_HIP(hipStreamSynchronize(ctx->stream));
+  return total;
+}
+
+// ---------------------------------------------------------------------------
+// Geometry constants passed by value
+// ---------------------------------------------------------------------------
+struct CellArgs {
+  double cell[9], ct[9], cti[9], cd[9], cdi[9];
+  double cutoff;
+  int sweep;
+  int nb[3], K[3];
+};
+
+__device__ __forceinline__ void row3(const double *a, const double *B, double *o) {
+  for (int k = 0; k < 3; k++) o[k] = (a[0] * B[k] + a[1] * B[3 + k]) + a[2] * B[6 + k];
+}
+
+// structure.cpp:120-133 per atom, plus the bin of the wrapped position.
+__global__ void k_wrap(int n, const double *pos, CellArgs ca, double *wrapped, int *bin) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double p[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+  double r1[3], fr[3], w1[3], w[3];
+  row3(p, ca.ct, r1);
+  row3(r1, ca.cdi, fr);
+  for (int k = 0; k < 3; k++) fr[k] = fr[k] - floor(fr[k]);
+  row3(fr, ca.cd, w1);
+  row3(w1, ca.cti, w);
+  for (int k = 0; k < 3; k++) wrapped[3 * i + k] = w[k];
+  // Bin from the fractional coordinates of the wrapped position.
+  row3(w, ca.ct, r1);
+  row3(r1, ca.cdi, fr);
+  int q[3];
+  for (int k = 0; k < 3; k++) {
+    int b = (int)floor(fr[k] * ca.nb[k]);
+    q[k] = min(max(b, 0), ca.nb[k] - 1);
+  }
+  bin[i] = (q[0] * ca.nb[1] + q[1]) * ca.nb[2] + q[2];
+}
+
+__global__ void k_bin_count(int n, const int *bin, int *bcount) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) atomicAdd(&bcount[bin[i]], 1);
+}
+
+__global__ void k_bin_fill(int n, const int *bin, const int *bstart, int *cursor, int *order) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    int b = bin[i];
+    int slot = atomicAdd(&cursor[b], 1);
+    order[bstart[b] + slot] = i;
+  }
+}
+
+__device__ __forceinline__ int floor_div(int a, int b) {
+  int q = a / b;
+  if ((a % b != 0) && ((a < 0) != (b < 0))) q--;
+  return q;
+}
+
+// structure.cpp:72-75.
+__device__ __forceinline__ double image_dist(const double *wi, const double *wj,
+                                             const CellArgs &ca, int s1, int s2, int s3,
+                                             double *im) {
+  for (int k = 0; k < 3; k++) {
+    double d = wj[k] - wi[k];
+    im[k] = ((d + (double)s1 * ca.cell[k]) + (double)s2 * ca.cell[3 + k]) +
+            (double)s3 * ca.cell[6 + k];
+  }
+  return sqrt(im[0] * im[0] + im[1] * im[1] + im[2] * im[2]);
+}
+
+// One wavefront per atom. kFill = false counts the neighbours; kFill = true
+// collects the keys in LDS, sorts them and writes the list.
+constexpr int kNbrCap = 2048;  // keys per atom held in LDS
+constexpr int kNbrWaves = 4;
+
+template <bool kFill>
+__global__ __launch_bounds__(64 * kNbrWaves) void k_neighbors(
+    int n, const double *wrapped, const int *bin_of, const int *bstart,
+    const int *order, const int *species, CellArgs ca, int *count, const int *cum,
+    int *sidx, int *nspec, double *rel, int *max_count) {
+  __shared__ long long keys[kNbrWaves][kFill ? kNbrCap : 1];
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int i = blockIdx.x * kNbrWaves + w;
+  if (i >= n) return;
+  double wi[3] = {wrapped[3 * i], wrapped[3 * i + 1], wrapped[3 * i + 2]};
+  int b = bin_of[i];
+  int bx = b / (ca.nb[1] * ca.nb[2]), by = (b / ca.nb[2]) % ca.nb[1], bz = b % ca.nb[2];
+  int sw = ca.sweep, span = 2 * sw + 1;
+  int found = 0;
+  for (int o0 = -ca.K[0]; o0 <= ca.K[0]; o0++) {
+    int t0 = bx + o0, s1 = floor_div(t0, ca.nb[0]), b0 = t0 - s1 * ca.nb[0];
+    if (s1 < -sw || s1 > sw) continue;
+    for (int o1 = -ca.K[1]; o1 <= ca.K[1]; o1++) {
+      int t1 = by + o1, s2 = floor_div(t1, ca.nb[1]), b1 = t1 - s2 * ca.nb[1];
+      if (s2 < -sw || s2 > sw) continue;
+      for (int o2 = -ca.K[2]; o2 <= ca.K[2]; o2++) {
+        int t2 = bz + o2, s3 = floor_div(t2, ca.nb[2]), b2 = t2 - s3 * ca.nb[2];
+        if (s3 < -sw || s3 > sw) continue;
+        int q = (b0 * ca.nb[1] + b1) * ca.nb[2] + b2;
+        int p0 = bstart[q], p1 = bstart[q + 1];
+        for (int p = p0; p < p1; p += 64) {
+          bool ok = false;
+          int j = 0;
+          if (p + lane < p1) {
+            j = order[p + lane];
+            double wj[3] = {wrapped[3 * j], wrapped[3 * j + 1], wrapped[3 * j + 2]};
+            double im[3];
+            double dist = image_dist(wi, wj, ca, s1, s2, s3, im);
+            ok = (dist < ca.cutoff) && (dist != 0);
+          }
+          unsigned long long mask = __ballot(ok);
+          if (kFill && ok) {
+            int pre = __popcll(mask & ((1ull << lane) - 1ull));
+            int slot = found + pre;
+            if (slot < kNbrCap) {
+              long long key = ((long long)j * span + (s1 + sw)) * span;
+              key = (key + (s2 + sw)) * span + (s3 + sw);
+              keys[w][slot] = key;
+            }
+          }
+          found += __popcll(mask);
+        }
+      }
+    }
+  }
+  if (!kFill) {
+    if (lane == 0) {
+      count[i] = found;
+      atomicMax(max_count, found);
+    }
+    return;
+  }
+  // Bitonic sort of the keys (padded to a power of two with the largest key).
+  int np2 = 1;
+  while (np2 < found) np2 <<= 1;
+  for (int t = found + lane; t < np2; t += 64) keys[w][t] = 0x7fffffffffffffffLL;
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  for (int k = 2; k <= np2; k <<= 1) {
+    for (int jj = k >> 1; jj > 0; jj >>= 1) {
+      for (int t = lane; t < np2; t += 64) {
+        int partner = t ^ jj;
+        if (partner > t) {
+          long long a = keys[w][t], c = keys[w][partner];
+          bool up = (t & k) == 0;
+          if ((a > c) == up) {
+            keys[w][t] = c;
+            keys[w][partner] = a;
+          }
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+  }
+  int base = cum[i];
+  for (int t = lane; t < found; t += 64) {
+    long long key = keys[w][t];
+    int s3 = (int)(key % span) - sw;
+    key /= span;
+    int s2 = (int)(key % span) - sw;
+    key /= span;
+    int s1 = (int)(key % span) - sw;
+    int j = (int)(key / span);
+    double wj[3] = {wrapped[3 * j], wrapped[3 * j + 1], wrapped[3 * j + 2]};
+    double im[3];
+    double dist = image_dist(wi, wj, ca, s1, s2, s3, im);
+    int o = base + t;
+    sidx[o] = j;
+    nspec[o] = species[j];
+    rel[4 * (size_t)o] = dist;
+    rel[4 * (size_t)o + 1] = im[0];
+    rel[4 * (size_t)o + 2] = im[1];
+    rel[4 * (size_t)o + 3] = im[2];
+  }
+}
+
+// Fallback for atoms with more than kNbrCap neighbours (tiny cells with a
+// large sweep): one thread per atom, keys sorted in place in global memory.
+__global__ void k_neighbors_big(int n, const double *wrapped, const int *bin_of,
+                                const int *bstart, const int *order, const int *species,
+                                CellArgs ca, const int *cum, long long *scratch, int *sidx,
+                                int *nspec, double *rel) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double wi[3] = {wrapped[3 * i], wrapped[3 * i + 1], wrapped[3 * i + 2]};
+  int b = bin_of[i];
+  int bx = b / (ca.nb[1] * ca.nb[2]), by = (b / ca.nb[2]) % ca.nb[1], bz = b % ca.nb[2];
+  int sw = ca.sweep, span = 2 * sw + 1;
+  long long *keys = scratch + cum[i];
+  int cnt = cum[i + 1] - cum[i];
+  int found = 0;
+  for (int o0 = -ca.K[0]; o0 <= ca.K[0]; o0++) {
+    int t0 = bx + o0, s1 = floor_div(t0, ca.nb[0]), b0 = t0 - s1 * ca.nb[0];
+    if (s1 < -sw || s1 > sw) continue;
+    for (int o1 = -ca.K[1]; o1 <= ca.K[1]; o1++) {
+      int t1 = by + o1, s2 = floor_div(t1, ca.nb[1]), b1 = t1 - s2 * ca.nb[1];
+      if (s2 < -sw || s2 > sw) continue;
+      for (int o2 = -ca.K[2]; o2 <= ca.K[2]; o2++) {
+        int t2 = bz + o2, s3 = floor_div(t2, ca.nb[2]), b2 = t2 - s3 * ca.nb[2];
+        if (s3 < -sw || s3 > sw) continue;
+        int q = (b0 * ca.nb[1] + b1) * ca.nb[2] + b2;
+        for (int p = bstart[q]; p < bstart[q + 1]; p++) {
+          int j = order[p];
+          double wj[3] = {wrapped[3 * j], wrapped[3 * j + 1], wrapped[3 * j + 2]};
+          double im[3];
+          double dist = image_dist(wi, wj, ca, s1, s2, s3, im);
+          if ((dist < ca.cutoff) && (dist != 0) && found < cnt) {
+            long long key = ((long long)j * span + (s1 + sw)) * span;
+            key = (key + (s2 + sw)) * span + (s3 + sw);
+            keys[found++] = key;
+          }
+        }
+      }
+    }
+  }
+  for (int a = 1; a < found; a++) {  // insertion sort
+    long long v = keys[a];
+    int c = a - 1;
+    while (c >= 0 && keys[c] > v) {
+      keys[c + 1] = keys[c];
+      c--;
+    }
+    keys[c + 1] = v;
+  }
+  for (int t = 0; t < found; t++) {
+    long long key = keys[t];
+    int s3 = (int)(key % span) - sw;
+    key /= span;
+    int s2 = (int)(key % span) - sw;
+    key /= span;
+    int s1 = (int)(key % span) - sw;
+    int j = (int)(key / span);
+    double wj[3] = {wrapped[3 * j], wrapped[3 * j + 1], wrapped[3 * j + 2]};
+    double im[3];
+    double dist = image_dist(wi, wj, ca, s1, s2, s3, im);
+    int o = cum[i] + t;
+    sidx[o] = j;
+    nspec[o] = species[j];
+    rel[4 * (size_t)o] = dist;
+    rel[4 * (size_t)o + 1] = im[0];
+    rel[4 * (size_t)o + 2] = im[1];
+    rel[4 * (size_t)o + 3] = im[2];
+  }
+}
+
+static void cross3(const double *u, const double *v, double *o) {
+  o[0] = u[1] * v[2] - u[2] * v[1];
+  o[1] = u[2] * v[0] - u[0] * v[2];
+  o[2] = u[0] * v[1] - u[1] * v[0];
+}
+
+void build_neighbors(fm_structure s) {
+  fm_context ctx = s->ctx;
+  hipStream_t st = ctx->stream;
+  int n = s->n;
+  CellArgs ca;
+  for (int k = 0; k < 9; k++) {
+    ca.cell[k] = s->cell[k];
+    ca.cti[k] = s->cell_t_inv[k];
+    ca.cd[k] = s->cell_dot[k];
+    ca.cdi[k] = s->cell_dot_inv[k];
+  }
+  for (int r = 0; r < 3; r++)
+    for (int c = 0; c < 3; c++) ca.ct[3 * r + c] = s->cell[3 * c + r];
+  ca.cutoff = s->cutoff;
+  ca.sweep = s->sweep;
+  double cr[3][3];
+  cross3(s->cell + 3, s->cell + 6, cr[0]);
+  cross3(s->cell + 6, s->cell, cr[1]);
+  cross3(s->cell, s->cell + 3, cr[2]);
+  double rc = s->cutoff * (1 + 1e-9) + 1e-9;
+  long long ncell = 1;
+  for (int ax = 0; ax < 3; ax++) {
+    double h = s->volume / std::sqrt(cr[ax][0] * cr[ax][0] + cr[ax][1] * cr[ax][1] +
+                                     cr[ax][2] * cr[ax][2]);
+    int nb = std::max(1, (int)std::floor(h / rc));
+    nb = std::min(nb, 1024);
+    ca.nb[ax] = nb;
+    ca.K[ax] = (int)std::ceil(rc / (h / nb));
+    ncell *= nb;
+  }
+  FM_CHECK(ncell < (1LL << 30), FM_ERR_CAPACITY, "cell list too large");
+  Pool *pool = &ctx->pool;
+  s->wrapped.alloc(pool, 3 * (size_t)n * sizeof(double));
+  s->count.alloc(pool, (size_t)n * sizeof(int));
+  s->cumulative.alloc(pool, ((size_t)n + 1) * sizeof(int));
+  DevBuf bin(pool, (size_t)n * sizeof(int));
+  DevBuf bcount(pool, (size_t)ncell * sizeof(int));
+  DevBuf bstart(pool, ((size_t)ncell + 1) * sizeof(int));
+  DevBuf cursor(pool, (size_t)ncell * sizeof(int));
+  DevBuf order(pool, (size_t)n * sizeof(int));
+  DevBuf maxc(pool, sizeof(int));
+  int tpb = 256, nblk = (n + tpb - 1) / tpb;
+  hipLaunchKernelGGL(k_wrap, dim3(nblk), dim3(tpb), 0, st, n, s->positions.as<double>(), ca,
+                     s->wrapped.as<double>(), bin.as<int>());
+  FM_HIP(hipMemsetAsync(bcount.as<int>(), 0, ncell * sizeof(int), st));
+  FM_HIP(hipMemsetAsync(cursor.as<int>(), 0, ncell * sizeof(int), st));
+  FM_HIP(hipMemsetAsync(maxc.as<int>(), 0, sizeof(int), st));
+  hipLaunchKernelGGL(k_bin_count, dim3(nblk), dim3(tpb), 0, st, n, bin.as<int>(), bcount.as<int>());
+  scan_exclusive(ctx, bcount.as<int>(), bstart.as<int>(), (int)ncell);
+  hipLaunchKernelGGL(k_bin_fill, dim3(nblk), dim3(tpb), 0, st, n, bin.as<int>(), bstart.as<int>(),
+                     cursor.as<int>(), order.as<int>());
+  int wblk = (n + kNbrWaves - 1) / kNbrWaves;
+  hipLaunchKernelGGL(k_neighbors<false>, dim3(wblk), dim3(64 * kNbrWaves), 0, st, n,
+                     s->wrapped.as<double>(), bin.as<int>(), bstart.as<int>(), order.as<int>(),
+                     s->species.as<int>(), ca, s->count.as<int>(), nullptr, nullptr, nullptr,
+                     nullptr, maxc.as<int>());
+  FM_HIP(hipGetLastError());
+  int total = scan_exclusive(ctx, s->count.as<int>(), s->cumulative.as<int>(), n);
+  int max_count = 0;
+  FM_HIP(hipMemcpyAsync(&max_count, maxc.as<int>(), sizeof(int), hipMemcpyDeviceToHost, st));
+  FM_HIP(hipStreamSynchronize(st));
+  s->n_nb = total;
+  s->sidx.alloc(pool, (size_t)total * sizeof(int));
+  s->nspecies.alloc(pool, (size_t)total * sizeof(int));
+  s->rel.alloc(pool, 4 * (size_t)total * sizeof(double));
+  if (max_count <= kNbrCap) {
+    hipLaunchKernelGGL(k_neighbors<true>, dim3(wblk), dim3(64 * kNbrWaves), 0, st, n,
+                       s->wrapped.as<double>(), bin.as<int>(), bstart.as<int>(), order.as<int>(),
+                       s->species.as<int>(), ca, nullptr, s->cumulative.as<int>(),
+                       s->sidx.as<int>(), s->nspecies.as<int>(), s->rel.as<double>(), nullptr);
+  } else {
+    DevBuf scratch(pool, (size_t)total * sizeof(long long));
+    hipLaunchKernelGGL(k_neighbors_big, dim3(nblk), dim3(tpb), 0, st, n, s->wrapped.as<double>(),
+                       bin.as<int>(), bstart.as<int>(), order.as<int>(), s->species.as<int>(), ca,
+                       s->cumulative.as<int>(), scratch.as<long long>(), s->sidx.as<int>(),
+                       s->nspecies.as<int>(), s->rel.as<double>());
+    FM_HIP(hipStreamSynchronize(st));
+  }
+  FM_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Descriptor bond list (b2.cpp:262-304: r <= rc(s_i, s_j), list order kept),
+// its transpose by neighbour atom (for the deterministic force gather), and
+// the species-sorted row of each atom (b2.cpp:148-176).
+// ---------------------------------------------------------------------------
+__global__ void k_bond_count(int n, const int *species, const int *cum, const int *nspec,
+                             const double *rel, const double *cutoffs, int nos, int *bcount) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int si = species[i], c = 0;
+  for (int q = cum[i]; q < cum[i + 1]; q++)
+    if (rel[4 * (size_t)q] <= cutoffs[si * nos + nspec[q]]) c++;
+  bcount[i] = c;
+}
+
+__global__ void k_bond_fill(int n, const int *species, const int *cum, const int *nspec,
+                            const double *rel, const double *cutoffs, int nos, const int *boff,
+                            const int *sidx, int *bond_q, int *in_count) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int si = species[i], o = boff[i];
+  for (int q = cum[i]; q < cum[i + 1]; q++)
+    if (rel[4 * (size_t)q] <= cutoffs[si * nos + nspec[q]]) {
+      bond_q[o++] = q;
+      atomicAdd(&in_count[sidx[q]], 1);
+    }
+}
+
+__global__ void k_in_fill(int nbonds, const int *bond_q, const int *sidx, const int *in_off,
+                          int *cursor, int *in_bond) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbonds) return;
+  int j = sidx[bond_q[b]];
+  int slot = atomicAdd(&cursor[j], 1);
+  in_bond[in_off[j] + slot] = b;
+}
+
+__global__ void k_in_sort(int n, const int *in_off, int *in_bond) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  int a0 = in_off[j], a1 = in_off[j + 1];
+  for (int a = a0 + 1; a < a1; a++) {
+    int v = in_bond[a], c = a - 1;
+    while (c >= a0 && in_bond[c] > v) {
+      in_bond[c + 1] = in_bond[c];
+      c--;
+    }
+    in_bond[c + 1] = v;
+  }
+}
+
+__global__ void k_species_flag(int n, const int *species, int t, int *flag) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) flag[i] = species[i] == t ? 1 : 0;
+}
+
+__global__ void k_species_rows(int n, const int *species, int t, const int *rank, int off,
+                               int *row_of_atom, int *atom_of_row) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && species[i] == t) {
+    int r = off + rank[i];
+    row_of_atom[i] = r;
+    atom_of_row[r] = i;
+  }
+}
+
+void build_bonds(fm_structure s) {
+  fm_context ctx = s->ctx;
+  hipStream_t st = ctx->stream;
+  Pool *pool = &ctx->pool;
+  int n = s->n, tpb = 256, nblk = (n + tpb - 1) / tpb;
+  s->d_cutoffs.alloc(pool, s->cutoffs.size() * sizeof(double));
+  FM_HIP(hipMemcpyAsync(s->d_cutoffs.as<double>(), s->cutoffs.data(),
+                        s->cutoffs.size() * sizeof(double), hipMemcpyHostToDevice, st));
+  DevBuf bcount(pool, (size_t)n * sizeof(int));
+  s->bond_off.alloc(pool, ((size_t)n + 1) * sizeof(int));
+  hipLaunchKernelGGL(k_bond_count, dim3(nblk), dim3(tpb), 0, st, n, s->species.as<int>(),
+                     s->cumulative.as<int>(), s->nspecies.as<int>(), s->rel.as<double>(),
+                     s->d_cutoffs.as<double>(), s->nos, bcount.as<int>());
+  int nbonds = scan_exclusive(ctx, bcount.as<int>(), s->bond_off.as<int>(), n);
+  s->n_bonds = nbonds;
+  s->bond_q.alloc(pool, (size_t)nbonds * sizeof(int));
+  DevBuf in_count(pool, (size_t)n * sizeof(int));
+  FM_HIP(hipMemsetAsync(in_count.as<int>(), 0, (size_t)n * sizeof(int), st));
+  hipLaunchKernelGGL(k_bond_fill, dim3(nblk), dim3(tpb), 0, st, n, s->species.as<int>(),
+                     s->cumulative.as<int>(), s->nspecies.as<int>(), s->rel.as<double>(),
+                     s->d_cutoffs.as<double>(), s->nos, s->bond_off.as<int>(), s->sidx.as<int>(),
+                     s->bond_q.as<int>(), in_count.as<int>());
+  s->in_off.alloc(pool, ((size_t)n + 1) * sizeof(int));
+  scan_exclusive(ctx, in_count.as<int>(), s->in_off.as<int>(), n);
+  s->in_bond.alloc(pool, (size_t)nbonds * sizeof(int));
+  DevBuf cursor(pool, (size_t)n * sizeof(int));
+  FM_HIP(hipMemsetAsync(cursor.as<int>(), 0, (size_t)n * sizeof(int), st));
+  if (nbonds > 0)
+    hipLaunchKernelGGL(k_in_fill, dim3((nbonds + tpb - 1) / tpb), dim3(tpb), 0, st, nbonds,
+                       s->bond_q.as<int>(), s->sidx.as<int>(), s->in_off.as<int>(),
+                       cursor.as<int>(), s->in_bond.as<int>());
+  hipLaunchKernelGGL(k_in_sort, dim3(nblk), dim3(tpb), 0, st, n, s->in_off.as<int>(),
+                     s->in_bond.as<int>());
+  // Species-sorted rows.
+  s->row_of_atom.alloc(pool, (size_t)n * sizeof(int));
+  s->atom_of_row.alloc(pool, (size_t)n * sizeof(int));
+  DevBuf flag(pool, (size_t)n * sizeof(int)), rank(pool, ((size_t)n + 1) * sizeof(int));
+  s->n_by_type.assign(s->nos, 0);
+  s->type_off.assign(s->nos + 1, 0);
+  for (int t = 0; t < s->nos; t++) {
+    hipLaunchKernelGGL(k_species_flag, dim3(nblk), dim3(tpb), 0, st, n, s->species.as<int>(), t,
+                       flag.as<int>());
+    int cnt = scan_exclusive(ctx, flag.as<int>(), rank.as<int>(), n);
+    hipLaunchKernelGGL(k_species_rows, dim3(nblk), dim3(tpb), 0, st, n, s->species.as<int>(), t,
+                       rank.as<int>(), s->type_off[t], s->row_of_atom.as<int>(),
+                       s->atom_of_row.as<int>());
+    s->n_by_type[t] = cnt;
+    s->type_off[t + 1] = s->type_off[t] + cnt;
+  }
+  FM_HIP(hipGetLastError());
+}
+
+}  // namespace fm

@@ -1,0 +1,264 @@
+// Prediction pipeline on the device: per species, the kernel contraction
+// (S = B_s X_hat^T with its fused epilogue), the gradient contraction
+// G = W X_hat, the optional variance contraction Q = L_s^-1 K^T, then the
+// adjoint bond forces and a deterministic gather of forces, energy and
+// stress (sparse_gp.cpp:643-665 and 731-757; output layout of
+// normalized_dot_product.cpp:117-214).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <vector>
+
+#include "fm_device.hpp"
+#include "fm_gemm.hpp"
+#include "fm_internal.hpp"
+
+namespace fm {
+
+// Per row (species-sorted atom): E = sum of the energy partials over column
+// tiles; var = sigma^2 (B.B / |B|^2)^p - sum of the |Q|^2 partials over row
+// tiles (the diagonal of envs_envs(cluster, cluster) minus diag(Q1^T Q1),
+// sparse_gp.cpp:116-136). Partials are summed in tile order.
+__global__ void k_species_final(int n_s, int ro, const double *Epart, int n_etiles,
+                                const double *Qpart, int n_qtiles, const double *norm,
+                                const double *nn, double sig2, double power, double *E_row,
+                                double *var_row) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n_s) return;
+  double e = 0;
+  for (int t = 0; t < n_etiles; t++) e += Epart[(size_t)t * n_s + r];
+  E_row[ro + r] = e;
+  if (var_row) {
+    double nrm = norm[ro + r];
+    double v = 0;
+    if (nrm >= kEmptyThresh) {
+      double nd = nn[ro + r] / (nrm * nrm);
+      double ks = sig2 * (power == 2.0 ? nd * nd : pow(nd, power));
+      double q = 0;
+      for (int t = 0; t < n_qtiles; t++) q += Qpart[(size_t)t * n_s + r];
+      v = ks - q;
+    }
+    var_row[ro + r] = v;
+  }
+}
+
+// Forces: F_i = sum_{bonds b of i} f_b - sum_{bonds b with neighbour i} f_b
+// (normalized_dot_product.cpp:197-199 after K^T alpha), each list in fixed
+// order. Also per-block partial sums of the energies and virials.
+template <typename TF>
+__global__ void k_gather(int n, const int *bond_off, const int *in_off, const int *in_bond,
+                         const TF *fbond, const int *row_of_atom, const double *E_row,
+                         const double *vir, double *out_F, double *local_energy,
+                         double *blk_part) {
+  __shared__ double sh[7][256];
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double e = 0, v[6] = {0, 0, 0, 0, 0, 0};
+  if (i < n) {
+    double f[3] = {0, 0, 0};
+    for (int b = bond_off[i]; b < bond_off[i + 1]; b++)
+      for (int c = 0; c < 3; c++) f[c] += (double)fbond[3 * (size_t)b + c];
+    for (int a = in_off[i]; a < in_off[i + 1]; a++) {
+      int b = in_bond[a];
+      for (int c = 0; c < 3; c++) f[c] -= (double)fbond[3 * (size_t)b + c];
+    }
+    for (int c = 0; c < 3; c++) out_F[3 * (size_t)i + c] = f[c];
+    e = E_row[row_of_atom[i]];
+    if (local_energy) local_energy[i] = e;
+    for (int k = 0; k < 6; k++) v[k] = vir[6 * (size_t)i + k];
+  }
+  sh[0][threadIdx.x] = e;
+  for (int k = 0; k < 6; k++) sh[1 + k][threadIdx.x] = v[k];
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+      for (int k = 0; k < 7; k++) sh[k][threadIdx.x] += sh[k][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x < 7) blk_part[(size_t)blockIdx.x * 7 + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+// Sum of the block partials in block order: energy and stress
+// (-1/V sum_b f (x) r, normalized_dot_product.cpp:201-206).
+__global__ void k_final_sum(int nblk, const double *blk_part, double vol_inv, double *E,
+                            double *S) {
+  int k = threadIdx.x;
+  if (k >= 7) return;
+  double acc = 0;
+  for (int b = 0; b < nblk; b++) acc += blk_part[(size_t)b * 7 + k];
+  if (k == 0) *E = acc;
+  else S[k - 1] = acc * vol_inv;
+}
+
+__global__ void k_fill(double *p, long long n, double v) {
+  long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+void predict(fm_model m, fm_structure s, int flags, int form, double *mean_efs,
+             double *local_var, double *local_energy) {
+  fm_context ctx = s->ctx;
+  hipStream_t st = ctx->stream;
+  Pool *pool = &ctx->pool;
+  FM_CHECK(s->has_b2, FM_ERR_STATE, "descriptors have not been computed for this structure");
+  FM_CHECK(s->form == form, FM_ERR_STATE, "structure descriptors were computed in another form");
+  FM_CHECK(m->nos == s->nos && m->nd == s->nd, FM_ERR_INVALID,
+           "model and descriptor settings disagree (species or descriptor size)");
+  bool want_var = (flags & FM_PREDICT_LOCAL) != 0;
+  FM_CHECK(!want_var || m->has_L, FM_ERR_STATE,
+           "L_inv must be up to date to evaluate uncertainties");
+  bool f64 = form == FM_FORM_FP64;
+  size_t esz = f64 ? sizeof(double) : sizeof(float);
+  int n = s->n, nd = s->nd, nos = s->nos;
+  int n_out = 1 + 3 * n + 6;
+  // Per-species slices of the atom-major w and k matrices (n_s x M_s each)
+  // and of the energy / variance partials.
+  std::vector<size_t> wo(nos + 1, 0), eo(nos + 1, 0), qo(nos + 1, 0);
+  for (int t = 0; t < nos; t++) {
+    size_t ns = s->n_by_type[t], Ms = m->M_by_type[t];
+    wo[t + 1] = wo[t] + ns * Ms;
+    eo[t + 1] = eo[t] + ns * (size_t)std::max(1, (int)((Ms + 63) / 64));
+    int Lr = want_var ? m->L_rows[t] : 0;
+    qo[t + 1] = qo[t] + ns * (size_t)((Lr + 63) / 64);
+  }
+  DevBuf W(pool, std::max<size_t>(1, wo[nos] * esz));
+  DevBuf Kb;
+  if (want_var) Kb.alloc(pool, std::max<size_t>(1, wo[nos] * sizeof(double)));
+  DevBuf G(pool, std::max<size_t>(1, (size_t)n * nd * esz));
+  DevBuf Epart(pool, std::max<size_t>(1, eo[nos] * sizeof(double)));
+  DevBuf Qpart(pool, std::max<size_t>(1, qo[nos] * sizeof(double)));
+  DevBuf E_row(pool, std::max<size_t>(1, (size_t)n * sizeof(double)));
+  DevBuf var_row(pool, std::max<size_t>(1, (size_t)n * sizeof(double)));
+  DevBuf out(pool, (size_t)n_out * sizeof(double));
+  double sig2 = m->sigma * m->sigma;
+
+  // Kernel contraction S = B_s X_hat_s^T with the fused epilogue.
+  ctx->mark_begin(ST_SGEMM);
+  for (int t = 0; t < nos; t++) {
+    int ns = s->n_by_type[t], Ms = m->M_by_type[t], ro = s->type_off[t];
+    if (ns == 0) continue;
+    if (Ms == 0) {
+      hipLaunchKernelGGL(k_fill, dim3((ns + 255) / 256), dim3(256), 0, st,
+                         Epart.as<double>() + eo[t], (long long)ns, 0.0);
+      continue;
+    }
+    EpArgs ek;
+    ek.sig2 = sig2;
+    ek.power = m->power;
+    ek.row_norm = s->norm.as<double>() + ro;
+    ek.alpha = m->alpha[t].as<double>();
+    ek.W = (char *)W.as<char>() + wo[t] * esz;
+    ek.ldw = Ms;
+    if (want_var) {
+      ek.K = Kb.as<double>() + wo[t];
+      ek.ldk = Ms;
+    }
+    ek.part = Epart.as<double>() + eo[t];
+    ek.part_ld = ns;
+    if (f64)
+      gemm_f64(st, EP_KERNEL, ns, Ms, nd, s->B.as<double>() + (size_t)ro * nd, nd,
+               m->X[t].as<double>(), nd, 0, ek);
+    else
+      gemm_bf16x3(st, EP_KERNEL, ns, Ms, nd, s->B.as<float>() + (size_t)ro * nd, nd,
+                  m->X32[t].as<float>(), nd, 0, ek);
+    ctx->launches[ST_SGEMM]++;
+  }
+  ctx->mark_end(ST_SGEMM);
+
+  // Gradient contraction G = W X_hat_s.
+  ctx->mark_begin(ST_GGEMM);
+  for (int t = 0; t < nos; t++) {
+    int ns = s->n_by_type[t], Ms = m->M_by_type[t], ro = s->type_off[t];
+    if (ns == 0) continue;
+    void *Grow = (char *)G.as<char>() + (size_t)ro * nd * esz;
+    if (Ms == 0) {
+      FM_HIP(hipMemsetAsync(Grow, 0, (size_t)ns * nd * esz, st));
+      continue;
+    }
+    EpArgs eg;
+    eg.W = Grow;
+    eg.ldw = nd;
+    if (f64)
+      gemm_f64(st, EP_STORE, ns, nd, Ms, W.as<double>() + wo[t], Ms, m->XT[t].as<double>(), Ms,
+               0, eg);
+    else
+      gemm_bf16x3(st, EP_STORE, ns, nd, Ms, W.as<float>() + wo[t], Ms, m->XT32[t].as<float>(),
+                  Ms, 0, eg);
+    ctx->launches[ST_GGEMM]++;
+  }
+  ctx->mark_end(ST_GGEMM);
+
+  // Variance contraction Q = L_s^-1 K^T (FP64 in both forms).
+  ctx->mark_begin(ST_VAR);
+  if (want_var) {
+    for (int t = 0; t < nos; t++) {
+      int ns = s->n_by_type[t], Ms = m->M_by_type[t];
+      if (ns == 0 || Ms == 0 || m->L_rows[t] == 0) continue;
+      EpArgs eq;
+      eq.part = Qpart.as<double>() + qo[t];
+      eq.part_ld = ns;
+      gemm_f64(st, EP_SUMSQ, m->L_rows[t], ns, Ms, m->L[t].as<double>(), Ms,
+               Kb.as<double>() + wo[t], Ms, m->L_lower[t], eq);
+      ctx->launches[ST_VAR]++;
+    }
+  }
+  for (int t = 0; t < nos; t++) {
+    int ns = s->n_by_type[t], Ms = m->M_by_type[t], ro = s->type_off[t];
+    if (ns == 0) continue;
+    int n_et = Ms == 0 ? 1 : (Ms + 63) / 64;
+    int n_qt = (want_var && Ms > 0) ? (m->L_rows[t] + 63) / 64 : 0;
+    hipLaunchKernelGGL(k_species_final, dim3((ns + 255) / 256), dim3(256), 0, st, ns, ro,
+                       Epart.as<double>() + eo[t], n_et, Qpart.as<double>() + qo[t], n_qt,
+                       s->norm.as<double>(), s->nn.as<double>(), sig2, m->power,
+                       E_row.as<double>(), want_var ? var_row.as<double>() : nullptr);
+  }
+  FM_HIP(hipGetLastError());
+  ctx->mark_end(ST_VAR);
+
+  // Adjoint bond forces and the deterministic gather.
+  ctx->mark_begin(ST_ADJ);
+  DevBuf fbond(pool, std::max<size_t>(1, (size_t)3 * s->n_bonds * esz));
+  DevBuf vir(pool, std::max<size_t>(1, (size_t)6 * n * sizeof(double)));
+  double *o = out.as<double>();
+  if (n > 0) {
+    launch_adjoint(s, G.as<void>(), nd, fbond.as<void>(), vir.as<double>());
+    int tpb = 256, nblk = (n + tpb - 1) / tpb;
+    DevBuf blk(pool, (size_t)nblk * 7 * sizeof(double));
+    if (f64)
+      hipLaunchKernelGGL(k_gather<double>, dim3(nblk), dim3(tpb), 0, st, n, s->bond_off.as<int>(),
+                         s->in_off.as<int>(), s->in_bond.as<int>(), fbond.as<double>(),
+                         s->row_of_atom.as<int>(), E_row.as<double>(), vir.as<double>(), o + 1,
+                         nullptr, blk.as<double>());
+    else
+      hipLaunchKernelGGL(k_gather<float>, dim3(nblk), dim3(tpb), 0, st, n, s->bond_off.as<int>(),
+                         s->in_off.as<int>(), s->in_bond.as<int>(), fbond.as<float>(),
+                         s->row_of_atom.as<int>(), E_row.as<double>(), vir.as<double>(), o + 1,
+                         nullptr, blk.as<double>());
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(64), 0, st, nblk, blk.as<double>(),
+                       1.0 / s->volume, o, o + 1 + 3 * n);
+    ctx->launches[ST_ADJ] += 3;
+  } else {
+    hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, o, (long long)n_out, 0.0);
+  }
+  FM_HIP(hipGetLastError());
+  ctx->mark_end(ST_ADJ);
+
+  if (mean_efs)
+    FM_HIP(hipMemcpyAsync(mean_efs, o, (size_t)n_out * sizeof(double), hipMemcpyDeviceToHost, st));
+  if (local_var && want_var && n > 0)
+    FM_HIP(hipMemcpyAsync(local_var, var_row.as<double>(), (size_t)n * sizeof(double),
+                          hipMemcpyDeviceToHost, st));
+  if (local_energy && n > 0) {
+    // E_row is in species-sorted order; map to atom order on the host.
+    std::vector<double> er(n);
+    std::vector<int> aor(n);
+    FM_HIP(hipMemcpyAsync(er.data(), E_row.as<double>(), (size_t)n * sizeof(double),
+                          hipMemcpyDeviceToHost, st));
+    FM_HIP(hipMemcpyAsync(aor.data(), s->atom_of_row.as<int>(), (size_t)n * sizeof(int),
+                          hipMemcpyDeviceToHost, st));
+    FM_HIP(hipStreamSynchronize(st));
+    for (int r = 0; r < n; r++) local_energy[aor[r]] = er[r];
+  }
+  FM_HIP(hipStreamSynchronize(st));
+}
+
+}  // namespace fm

@@ -1,0 +1,72 @@
+"""Seeded synthetic structures and models for the benchmark workloads and
+tests (there are no datasets in this environment).
+
+  fcc_al_108      108-atom single-species FCC (a = 4.05 A), Gaussian
+                  displacements sigma = 0.05 A, rng seed 1000 + frame.
+  fcc_alloy_32k   32,000-atom FCC (a = 4.0 A, 20^3 cells), two species 50/50
+                  by random permutation (seed 2), displacements 0.08 A.
+"""
+import numpy as np
+
+FCC_BASIS = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]], dtype=np.float64)
+BCC_BASIS = np.array([[0, 0, 0], [0.5, 0.5, 0.5]], dtype=np.float64)
+
+
+def lattice(basis, a, reps):
+    nx, ny, nz = reps
+    g = np.stack(np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij"), -1)
+    g = g.reshape(-1, 1, 3)
+    pos = (g + basis[None, :, :]).reshape(-1, 3) * a
+    cell = np.diag([nx * a, ny * a, nz * a]).astype(np.float64)
+    return cell, pos
+
+
+def fcc_al_108(frame=0, sigma=0.05):
+    """Config 1 frames: seed 1000 + frame."""
+    cell, pos = lattice(FCC_BASIS, 4.05, (3, 3, 3))
+    rng = np.random.default_rng(1000 + frame)
+    pos = pos + rng.normal(scale=sigma, size=pos.shape)
+    return cell, np.zeros(len(pos), np.int32), pos
+
+
+def fcc_alloy_32k(sigma=0.08):
+    """Config 2 structure: 32,000 atoms, species by permutation (seed 2)."""
+    cell, pos = lattice(FCC_BASIS, 4.0, (20, 20, 20))
+    n = len(pos)
+    rng = np.random.default_rng(2)
+    species = np.zeros(n, np.int32)
+    species[rng.permutation(n)[: n // 2]] = 1
+    pos = pos + np.random.default_rng(22).normal(scale=sigma, size=pos.shape)
+    return cell, species, pos
+
+
+def random_alloy(n_cells, a=4.0, nos=2, sigma=0.08, seed=0, basis=FCC_BASIS):
+    cell, pos = lattice(basis, a, n_cells)
+    rng = np.random.default_rng(seed)
+    species = rng.integers(0, nos, len(pos)).astype(np.int32)
+    pos = pos + rng.normal(scale=sigma, size=pos.shape)
+    return cell, species, pos
+
+
+def lennard_jones(cell, positions, eps=0.4, sig=2.6, rc=7.8):
+    """Shifted LJ energy and forces (the config-1 labels), brute force over
+    images; used to label training frames."""
+    pos = np.asarray(positions)
+    n = len(pos)
+    inv = np.linalg.inv(cell)
+    shift = 4 * eps * ((sig / rc) ** 12 - (sig / rc) ** 6)
+    E = 0.0
+    F = np.zeros_like(pos)
+    reps = [np.array([i, j, k]) for i in (-1, 0, 1) for j in (-1, 0, 1) for k in (-1, 0, 1)]
+    for i in range(n):
+        d = pos - pos[i]
+        d = d - np.round(d @ inv) @ cell
+        for r in reps:
+            dd = d + r @ cell
+            rr = np.linalg.norm(dd, axis=1)
+            m = (rr > 1e-9) & (rr < rc)
+            x = sig / rr[m]
+            E += 0.5 * np.sum(4 * eps * (x ** 12 - x ** 6) - shift)
+            fmag = 24 * eps * (2 * x ** 12 - x ** 6) / rr[m] ** 2
+            F[i] -= np.sum(fmag[:, None] * dd[m], axis=0)
+    return E, F

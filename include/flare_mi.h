@@ -1,0 +1,159 @@
+/*
+ * flare_mi.h — C ABI of libflare_mi, the MI355X (gfx950) implementation of
+ * FLARE's sparse-GP force evaluation: periodic neighbour list, B2
+ * descriptor, NormalizedDotProduct kernel against the sparse set, and
+ * SparseGP mean / local-uncertainty prediction.
+ *
+ * Plain pointers and sizes only. Every call returns an int status (FM_OK on
+ * success); fm_last_error() gives a message for the calling thread. Output
+ * buffers are owned by the caller. Handles are not thread-safe: one host
+ * thread drives a context at a time.
+ *
+ * Reference interfaces cited as file:line of the mir-group/flare tree.
+ */
+#ifndef FLARE_MI_H
+#define FLARE_MI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. The reference raises std::runtime_error / invalid_argument
+ * (sparse_gp.cpp:97-101, normalized_dot_product.cpp:289-292); the ABI
+ * returns these codes instead. */
+#define FM_OK 0
+#define FM_ERR_INVALID 1   /* bad argument or unsupported setting */
+#define FM_ERR_HIP 2       /* HIP runtime error */
+#define FM_ERR_STATE 3     /* call made in the wrong state (stale model) */
+#define FM_ERR_CAPACITY 4  /* request larger than a fixed limit */
+
+/* Arithmetic forms of a prediction. */
+#define FM_FORM_FP64 0     /* FP64 everywhere: the drop-in form */
+#define FM_FORM_CHOSEN 1   /* FP32 descriptors/adjoint, split-BF16x3 MFMA for
+                              the kernel contraction (FP32 accumulate), FP64
+                              variance solve */
+
+/* Cutoff functions (cutoffs.cpp:44-73). */
+#define FM_CUTOFF_QUADRATIC 0
+#define FM_CUTOFF_COSINE 1
+
+/* Prediction flags. */
+#define FM_PREDICT_MEAN 1   /* energy, forces, stress (sparse_gp.cpp:643-665) */
+#define FM_PREDICT_LOCAL 2  /* local variances (sparse_gp.cpp:93-139,731-757) */
+
+typedef struct fm_context_s *fm_context;
+typedef struct fm_structure_s *fm_structure;
+typedef struct fm_model_s *fm_model;
+
+const char *fm_last_error(void);
+
+/* Device context: one HIP stream plus a caching pool of device buffers.
+ * No reference counterpart (the reference is a CPU library). */
+int fm_context_create(int device, fm_context *out);
+int fm_context_destroy(fm_context ctx);
+int fm_context_synchronize(fm_context ctx);
+/* The hipStream_t the context launches on. */
+int fm_context_stream(fm_context ctx, void **stream);
+
+/* Structure with its neighbour list, built on the device.
+ * Replaces Structure(cell, species, positions, cutoff, descriptors)
+ * (src/flare_pp/structure.cpp:27-43; wrapping :120-133, sweep :135-165,
+ * neighbours :52-118). cell is 3x3 row-major with lattice vectors as rows;
+ * positions N x 3. The neighbour list equals the reference's bit for bit:
+ * order (j, s1, s2, s3), dist < cutoff and dist != 0. */
+int fm_structure_create(fm_context ctx, const double *cell, int n_atoms,
+                        const int32_t *species, const double *positions,
+                        double cutoff, fm_structure *out);
+/* As above with species and positions already in device memory. */
+int fm_structure_create_device(fm_context ctx, const double *cell, int n_atoms,
+                               const int32_t *species_dev,
+                               const double *positions_dev, double cutoff,
+                               fm_structure *out);
+int fm_structure_destroy(fm_structure s);
+/* sweep (structure.cpp:35), total neighbours, volume, single_sweep_cutoff. */
+int fm_structure_info(fm_structure s, int *sweep, int64_t *n_neighbors,
+                      double *volume, double *single_sweep_cutoff);
+/* Copies of Structure::wrapped_positions, neighbor_count,
+ * cumulative_neighbor_count, structure_indices, neighbor_species,
+ * relative_positions (structure.h:16-45). Any pointer may be NULL. */
+int fm_structure_get_neighbors(fm_structure s, double *wrapped, int32_t *count,
+                               int32_t *cumulative, int32_t *structure_indices,
+                               int32_t *neighbor_species, double *relative);
+
+/* B2 descriptor settings: B2(radial_basis="chebyshev", cutoff_function,
+ * radial_hyps=[r1, rc], cutoff_hyps=[], [n_species, N, lmax], cutoffs)
+ * (descriptors/b2.cpp:13-50). cutoffs is n_species^2, [central][neighbour]. */
+typedef struct {
+  int n_species;
+  int n_max;
+  int l_max;
+  int cutoff_function;
+  double radial_r1;
+  const double *cutoffs;
+} fm_b2_settings;
+
+/* B2::compute_struc (descriptors/b2.cpp:76-399) on the device; the
+ * descriptors stay resident. form is FM_FORM_FP64 or FM_FORM_CHOSEN (FP32). */
+int fm_structure_compute_b2(fm_structure s, const fm_b2_settings *settings,
+                            int form);
+/* DescriptorValues sizes (descriptor.h:31-57): n_descriptors, and per species
+ * n_clusters_by_type and n_neighbors_by_type (arrays of n_species). */
+int fm_structure_b2_sizes(fm_structure s, int *n_descriptors,
+                          int32_t *n_clusters_by_type,
+                          int32_t *n_neighbors_by_type);
+/* DescriptorValues fields of one species (descriptor.h:37-43, b2.cpp:148-176):
+ * descriptors (n_s x d), descriptor_norms, neighbor_counts,
+ * cumulative_neighbor_counts, atom_indices, neighbor_indices,
+ * neighbor_coordinates (nb_s x 3). Any pointer may be NULL. */
+int fm_structure_get_b2(fm_structure s, int species, double *descriptors,
+                        double *norms, int32_t *neighbor_counts,
+                        int32_t *cumulative_neighbor_counts,
+                        int32_t *atom_indices, int32_t *neighbor_indices,
+                        double *neighbor_coordinates);
+
+/* Sparse-GP prediction state for one kernel (one descriptor):
+ * NormalizedDotProduct(sigma, power) (kernels/normalized_dot_product.cpp:15-26),
+ * its ClusterDescriptor sparse set (descriptor.h:61-90: raw rows and norms
+ * per species, species-major), the kernel's slice of alpha and the diagonal
+ * block of L_inv (sparse_gp.cpp:607-641; M x M row-major, lower triangular;
+ * NULL disables local variances). */
+typedef struct {
+  double sigma;
+  double power;
+  int n_species;
+  int n_descriptors;
+  const int32_t *n_sparse_by_type;
+  const double *sparse_descriptors; /* M x d, species-major */
+  const double *sparse_norms;       /* M */
+  const double *alpha;              /* M */
+  const double *L_inv;              /* M x M or NULL */
+} fm_model_desc;
+
+int fm_model_create(fm_context ctx, const fm_model_desc *desc, fm_model *out);
+int fm_model_destroy(fm_model m);
+
+/* SparseGP::predict_mean / predict_local_uncertainties for one kernel
+ * (sparse_gp.cpp:643-665, 731-757; kernel columns of
+ * normalized_dot_product.cpp:117-214 contracted with alpha by the adjoint,
+ * so dB/dr is never formed). Outputs, any may be NULL:
+ *   mean_efs     1 + 3N + 6: [E, F (atom-major xyz), S (xx,xy,xz,yy,yz,zz)]
+ *   local_var    N, species-sorted order (compute_cluster_uncertainties)
+ *   local_energy N, atom order (this kernel's per-atom energies). */
+int fm_predict(fm_model m, fm_structure s, int flags, int form,
+               double *mean_efs, double *local_var, double *local_energy);
+
+/* Device-side durations (HIP events on the context stream) of the stages
+ * last run on a context, in milliseconds, and the number of kernel launches
+ * in each: [0] neighbour list, [1] descriptors (bond list + B2), [2] kernel
+ * contraction S = B X_hat^T with its epilogue, [3] gradient contraction
+ * G = W X_hat, [4] variance contraction, [5] adjoint + force gather.
+ * Either pointer may be NULL. No reference counterpart. */
+#define FM_N_STAGES 6
+int fm_last_timings(fm_context ctx, double *ms, int32_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FLARE_MI_H */
