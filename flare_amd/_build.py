@@ -21,6 +21,7 @@ SOURCES = {
     "fm_descriptors.hip": [],
     "fm_gemm.hip": [],
     "fm_predict.hip": [],
+    "fm_mapped.hip": [],
 }
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

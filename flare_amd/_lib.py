@@ -35,6 +35,8 @@ EXPORTS = [
     "fm_structure_destroy", "fm_structure_info", "fm_structure_get_neighbors",
     "fm_structure_compute_b2", "fm_structure_b2_sizes", "fm_structure_get_b2",
     "fm_model_create", "fm_model_destroy", "fm_predict", "fm_last_timings",
+    "fm_model_mapping_coefficients", "fm_mapped_create", "fm_mapped_read_file",
+    "fm_mapped_info", "fm_mapped_destroy", "fm_mapped_compute",
 ]
 
 
@@ -48,6 +50,12 @@ class B2Settings(C.Structure):
     _fields_ = [("n_species", C.c_int), ("n_max", C.c_int), ("l_max", C.c_int),
                 ("cutoff_function", C.c_int), ("radial_r1", C.c_double),
                 ("cutoffs", C.POINTER(C.c_double))]
+
+
+class MappedDesc(C.Structure):
+    _fields_ = [("power", C.c_int), ("normalized", C.c_int), ("n_species", C.c_int),
+                ("n_max", C.c_int), ("l_max", C.c_int), ("cutoff_function", C.c_int),
+                ("cutoffs", C.POINTER(C.c_double)), ("beta", C.POINTER(C.c_double))]
 
 
 class ModelDesc(C.Structure):
@@ -91,6 +99,12 @@ def load():
         L.fm_model_destroy.argtypes = [vp]
         L.fm_predict.argtypes = [vp, vp, ip, ip, vp, vp, vp]
         L.fm_last_timings.argtypes = [vp, vp, vp]
+        L.fm_model_mapping_coefficients.argtypes = [vp, vp, C.POINTER(C.c_int64)]
+        L.fm_mapped_create.argtypes = [vp, C.POINTER(MappedDesc), pvp]
+        L.fm_mapped_read_file.argtypes = [vp, C.c_char_p, pvp]
+        L.fm_mapped_info.argtypes = [vp] * 8
+        L.fm_mapped_destroy.argtypes = [vp]
+        L.fm_mapped_compute.argtypes = [vp, vp, vp, vp, vp]
         for name in EXPORTS:
             getattr(L, name).restype = C.c_char_p if name == "fm_last_error" else C.c_int
         _lib = L

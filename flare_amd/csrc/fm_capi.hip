@@ -548,6 +548,108 @@ int fm_predict(fm_model m, fm_structure s, int flags, int form, double *mean_efs
   });
 }
 
+int fm_model_mapping_coefficients(fm_model m, double *beta, int64_t *beta_size) {
+  return guarded([&] {
+    FM_CHECK(m && beta_size, FM_ERR_INVALID, "null argument");
+    int d = m->nd;
+    FM_CHECK(m->power == 1.0 || m->power == 2.0, FM_ERR_INVALID,
+             "Mapping coefficients of the normalized dot product kernel are implemented for "
+             "power 1 and 2 only");
+    *beta_size = m->power == 1.0 ? d : (int64_t)d * (d + 1) / 2;
+    if (!beta) return;
+    FM_HIP(hipSetDevice(m->ctx->device));
+    std::vector<double> b;
+    int64_t bs;
+    fm::mapping_coefficients(m, b, bs);
+    std::memcpy(beta, b.data(), b.size() * sizeof(double));
+  });
+}
+
+static void mapped_check(fm_mapped mp) {
+  FM_CHECK(mp->power == 1 || mp->power == 2, FM_ERR_INVALID, "Power should be 1 or 2.");
+  FM_CHECK(mp->nos >= 1 && mp->N >= 1 && mp->N <= 16 && mp->lmax >= 0 && mp->lmax <= 10,
+           FM_ERR_INVALID, "unsupported descriptor settings");
+  FM_CHECK(mp->cutkind == FM_CUTOFF_QUADRATIC || mp->cutkind == FM_CUTOFF_COSINE,
+           FM_ERR_INVALID, "Please use quadratic or cosine cutoff function.");
+  int nr = mp->N * mp->nos;
+  mp->nd = nr * (nr + 1) / 2 * (mp->lmax + 1);
+}
+
+int fm_mapped_create(fm_context ctx, const fm_mapped_desc *d, fm_mapped *out) {
+  return guarded([&] {
+    FM_CHECK(ctx && d && out && d->cutoffs && d->beta, FM_ERR_INVALID, "null argument");
+    FM_HIP(hipSetDevice(ctx->device));
+    auto *mp = new fm_mapped_s();
+    mp->ctx = ctx;
+    mp->power = d->power;
+    mp->normalized = d->normalized ? 1 : 0;
+    mp->nos = d->n_species;
+    mp->N = d->n_max;
+    mp->lmax = d->l_max;
+    mp->cutkind = d->cutoff_function;
+    try {
+      mapped_check(mp);
+      mp->cutoffs.assign(d->cutoffs, d->cutoffs + mp->nos * mp->nos);
+      fm::mapped_upload(mp, d->beta);
+    } catch (...) {
+      delete mp;
+      throw;
+    }
+    *out = mp;
+  });
+}
+
+int fm_mapped_read_file(fm_context ctx, const char *path, fm_mapped *out) {
+  return guarded([&] {
+    FM_CHECK(ctx && path && out, FM_ERR_INVALID, "null argument");
+    FM_HIP(hipSetDevice(ctx->device));
+    auto *mp = new fm_mapped_s();
+    mp->ctx = ctx;
+    try {
+      std::vector<double> beta;
+      fm::mapped_parse(path, mp, beta);
+      mapped_check(mp);
+      fm::mapped_upload(mp, beta.data());
+    } catch (...) {
+      delete mp;
+      throw;
+    }
+    *out = mp;
+  });
+}
+
+int fm_mapped_info(fm_mapped mp, int *power, int *normalized, int *n_species, int *n_max,
+                   int *l_max, int *cutoff_function, double *cutoffs) {
+  return guarded([&] {
+    FM_CHECK(mp, FM_ERR_INVALID, "null potential");
+    if (power) *power = mp->power;
+    if (normalized) *normalized = mp->normalized;
+    if (n_species) *n_species = mp->nos;
+    if (n_max) *n_max = mp->N;
+    if (l_max) *l_max = mp->lmax;
+    if (cutoff_function) *cutoff_function = mp->cutkind;
+    if (cutoffs) std::memcpy(cutoffs, mp->cutoffs.data(), mp->cutoffs.size() * sizeof(double));
+  });
+}
+
+int fm_mapped_destroy(fm_mapped mp) {
+  return guarded([&] {
+    if (!mp) return;
+    (void)hipStreamSynchronize(mp->ctx->stream);
+    delete mp;
+  });
+}
+
+int fm_mapped_compute(fm_mapped mp, fm_structure s, double *atom_energy, double *forces,
+                      double *virial) {
+  return guarded([&] {
+    FM_CHECK(mp && s, FM_ERR_INVALID, "null argument");
+    FM_CHECK(mp->ctx == s->ctx, FM_ERR_INVALID, "potential and structure belong to different contexts");
+    FM_HIP(hipSetDevice(s->ctx->device));
+    fm::mapped_compute(mp, s, atom_energy, forces, virial);
+  });
+}
+
 int fm_last_timings(fm_context ctx, double *ms_out, int32_t *launches) {
   return guarded([&] {
     FM_CHECK(ctx, FM_ERR_INVALID, "null argument");

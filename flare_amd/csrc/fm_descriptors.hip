@@ -84,7 +84,9 @@ __device__ void accumulate_c(const B2Args &a, int i, int lane, T *c) {
     for (int ch = 0; ch < kMaxHC; ch++) {
       if (!act[ch]) continue;
       int lm = lane + 64 * ch;
-      for (int nn = 0; nn < a.N; nn++) c[(t * a.N + nn) * a.H + lm] = acc[ch][nn];
+#pragma unroll
+      for (int nn = 0; nn < kMaxN; nn++)
+        if (nn < a.N) c[(t * a.N + nn) * a.H + lm] = acc[ch][nn];
     }
   }
 }
@@ -123,17 +125,26 @@ __global__ __launch_bounds__(64 * kAtomsPerBlock) void k_b2(B2Args a, T *B, doub
   }
 }
 
-// Adjoint per atom: w = dE/dB = (G - (G.B_hat) B_hat)/|B| from the kernel
-// contraction G = sum_j sigma^2 alpha_j p S_j^(p-1) x_hat_j; u = dE/dc
-// (lammps_descriptor.cpp:286-314); per bond f = dE_i/dr_ij =
-// sum_{n,lm} u[s_j n, lm] grad(g_n Y_lm) (pair_flare.cpp:141-150).
-// Writes f for every bond and the atom's virial -sum_b f (x) r_b
-// (xx, xy, xz, yy, yz, zz).
+// Adjoint per atom. With `project` set, w = dE/dB = (G - (G.B_hat) B_hat)/|B|
+// from G = dE/dB_hat |B| (for the sparse GP, G = sum_j sigma^2 alpha_j p
+// S_j^(p-1) x_hat_j, the closed form of normalized_dot_product.cpp:191-195);
+// without it w = G (unnormalised kernels). Then u = dE/dc
+// (lammps_descriptor.cpp:286-314) and per bond f = dE_i/dr_ij =
+// sum_{n,lm} u[s_j n, lm] grad(g_n Y_lm) (pair_flare.cpp:141-150). Writes f
+// for every bond and the atom's virial -sum_b f (x) r_b (xx, xy, xz, yy, yz,
+// zz). An atom with |B| < empty_norm or B.B < empty_nn contributes nothing.
+struct AdjArgs {
+  int project;
+  double empty_norm, empty_nn;
+};
+
 template <typename T>
 __global__ __launch_bounds__(64 * kAtomsPerBlock) void k_adjoint(B2Args a, const T *B,
                                                                    const double *norm,
+                                                                   const double *nnv,
                                                                    const T *G, long long g_ld,
-                                                                   T *fbond, double *vir) {
+                                                                   AdjArgs aa, T *fbond,
+                                                                   double *vir) {
   extern __shared__ __align__(16) unsigned char smem[];
   int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   int i = blockIdx.x * (blockDim.x >> 6) + w;
@@ -145,8 +156,9 @@ __global__ __launch_bounds__(64 * kAtomsPerBlock) void k_adjoint(B2Args a, const
   int row = a.row_of_atom[i];
   int b0 = a.bond_off[i], b1 = a.bond_off[i + 1];
   double nrm = norm[row];
-  if (nrm < kEmptyThresh) {
-    // Empty environment: no force contribution (normalized_dot_product.cpp:171).
+  if (nrm < aa.empty_norm || nnv[row] < aa.empty_nn) {
+    // Empty environment: no force contribution (normalized_dot_product.cpp:171,
+    // pair_flare.cpp:134-136).
     for (int b = b0 + lane; b < b1; b += 64) {
       fbond[3 * (size_t)b] = 0;
       fbond[3 * (size_t)b + 1] = 0;
@@ -158,11 +170,15 @@ __global__ __launch_bounds__(64 * kAtomsPerBlock) void k_adjoint(B2Args a, const
   accumulate_c<T>(a, i, lane, c);
   const T *Brow = B + (size_t)row * a.nd;
   const T *Grow = G + (size_t)row * g_ld;
-  T ninv = T(1) / (T)nrm;
-  T gb = 0;
-  for (int k = lane; k < a.nd; k += 64) gb += Grow[k] * Brow[k];
-  gb = wave_sum(gb) * ninv;
-  for (int k = lane; k < a.nd; k += 64) wv[k] = (Grow[k] - gb * Brow[k] * ninv) * ninv;
+  if (aa.project) {
+    T ninv = T(1) / (T)nrm;
+    T gb = 0;
+    for (int k = lane; k < a.nd; k += 64) gb += Grow[k] * Brow[k];
+    gb = wave_sum(gb) * ninv;
+    for (int k = lane; k < a.nd; k += 64) wv[k] = (Grow[k] - gb * Brow[k] * ninv) * ninv;
+  } else {
+    for (int k = lane; k < a.nd; k += 64) wv[k] = Grow[k];
+  }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -319,7 +335,7 @@ void launch_b2(fm_structure s) {
 }
 
 void launch_adjoint(fm_structure s, const void *G, int64_t g_ld, void *fbond,
-                    double *virial_atom) {
+                    double *virial_atom, int project, double empty_norm, double empty_nn) {
   fm_context ctx = s->ctx;
   hipStream_t st = ctx->stream;
   B2Args a = make_args(s);
@@ -329,14 +345,15 @@ void launch_adjoint(fm_structure s, const void *G, int64_t g_ld, void *fbond,
   size_t lds = apb * per;
   int nblk = (s->n + apb - 1) / apb;
   if (nblk == 0) return;
+  AdjArgs aa{project, empty_norm, empty_nn};
   if (s->form == FM_FORM_FP64)
     hipLaunchKernelGGL(k_adjoint<double>, dim3(nblk), dim3(64 * apb), lds, st, a,
-                       s->B.as<double>(), s->norm.as<double>(), (const double *)G,
-                       (long long)g_ld, (double *)fbond, virial_atom);
+                       s->B.as<double>(), s->norm.as<double>(), s->nn.as<double>(),
+                       (const double *)G, (long long)g_ld, aa, (double *)fbond, virial_atom);
   else
     hipLaunchKernelGGL(k_adjoint<float>, dim3(nblk), dim3(64 * apb), lds, st, a,
-                       s->B.as<float>(), s->norm.as<double>(), (const float *)G,
-                       (long long)g_ld, (float *)fbond, virial_atom);
+                       s->B.as<float>(), s->norm.as<double>(), s->nn.as<double>(),
+                       (const float *)G, (long long)g_ld, aa, (float *)fbond, virial_atom);
   FM_HIP(hipGetLastError());
 }
 

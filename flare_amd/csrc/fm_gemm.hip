@@ -1,7 +1,7 @@
 // Kernel contraction on the matrix cores. Every contraction of the path has
 // the form C = A . B^T with both operands contiguous along the reduction
-// index, tiled 64 x 64 per workgroup (4 wavefronts of 32 x 32), with a fused
-// epilogue:
+// index; a workgroup of 4 wavefronts (2 x 2) computes a tile, each wave
+// TM x TN MFMA tiles of 16 x 16, with a fused epilogue:
 //
 //   EpKernel  C = B_s . X_hat_s^T (atoms x sparse): S = C / |B_i|,
 //             k = sigma^2 S^p, w = sigma^2 alpha_j p S^(p-1); writes w and
@@ -13,9 +13,11 @@
 //             variance term |L_inv k|^2 (sparse_gp.cpp:121-131); A is lower
 //             triangular so each row tile stops at its last nonzero column.
 //
-// FP64 form: v_mfma_f64_16x16x4f64. Chosen form: split-BF16 x3 on
-// v_mfma_f32_16x16x32_bf16 (a = a_hi + a_lo, ab ~ a_hi b_hi + a_hi b_lo +
-// a_lo b_hi, FP32 accumulation).
+// FP64 form: v_mfma_f64_16x16x4f64, 64 x 64 tiles. Chosen form: split-BF16
+// x3 on v_mfma_f32_16x16x32_bf16 (a = a_hi + a_lo, ab ~ a_hi b_hi +
+// a_hi b_lo + a_lo b_hi, FP32 accumulation), 128 x 128 tiles; the three
+// products of a k-step run over all 16 accumulators of a wave before the
+// next, so consecutive MFMAs are independent.
 #include <hip/hip_runtime.h>
 
 #include "fm_device.hpp"
@@ -28,15 +30,13 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
-constexpr int BM = 64, BN = 64;
-
 // ---------------------------------------------------------------------------
-// Epilogues. Each receives the wave's 2 x 2 tiles of 16 x 16 values with the
-// MFMA output layout: lane l holds rows 4 (l / 16) + i, column l % 16.
+// Epilogues. The wave's tiles use the MFMA output layout: lane l holds rows
+// 4 (l / 16) + i (i < 4) and column l % 16 of each 16 x 16 tile.
 // ---------------------------------------------------------------------------
-template <typename Acc>
+template <typename Acc, int TM, int TN>
 struct TileOut {
-  Acc v[2][2][4];
+  Acc v[TM][TN][4];
 };
 
 __device__ __forceinline__ double fpow(double x, double p, int pi) {
@@ -50,28 +50,29 @@ __device__ __forceinline__ float fpow(float x, float p, int pi) {
   return powf(x, p);
 }
 
-template <typename Acc, typename TW>
-__device__ void ep_kernel(const EpArgs &e, const TileOut<Acc> &t, int row0, int col0, int M,
-                          int N, int lane, int wr, int wc, double *red) {
+template <typename Acc, typename TW, int TM, int TN>
+__device__ void ep_kernel(const EpArgs &e, const TileOut<Acc, TM, TN> &t, int row0, int col0,
+                          int M, int N, int lane, int wr, int wc, double *red) {
+  constexpr int BMc = 32 * TM;
   // rows: atoms of this species block, cols: sparse environments.
   int pi = e.power == 2.0 ? 2 : (e.power == 1.0 ? 1 : 0);
-  double rs[2][4];
+  double rs[TM][4];
 #pragma unroll
-  for (int ti = 0; ti < 2; ti++)
+  for (int ti = 0; ti < TM; ti++)
 #pragma unroll
     for (int i = 0; i < 4; i++) rs[ti][i] = 0;
 #pragma unroll
-  for (int ti = 0; ti < 2; ti++)
+  for (int ti = 0; ti < TM; ti++)
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-      int r = row0 + wr * 32 + ti * 16 + 4 * (lane >> 4) + i;
+      int r = row0 + wr * 16 * TM + ti * 16 + 4 * (lane >> 4) + i;
       if (r >= M) continue;
       double nrm = e.row_norm[r];
       bool empty = nrm < kEmptyThresh;
       Acc ninv = empty ? Acc(0) : Acc(1.0 / nrm);
 #pragma unroll
-      for (int tj = 0; tj < 2; tj++) {
-        int c = col0 + wc * 32 + tj * 16 + (lane & 15);
+      for (int tj = 0; tj < TN; tj++) {
+        int c = col0 + wc * 16 * TN + tj * 16 + (lane & 15);
         if (c >= N) continue;
         Acc s = t.v[ti][tj][i] * ninv;
         Acc a = (Acc)e.alpha[c];
@@ -86,7 +87,7 @@ __device__ void ep_kernel(const EpArgs &e, const TileOut<Acc> &t, int row0, int 
   // Sum over the 16 columns of each lane group, then over the two wave
   // columns through LDS; one partial per (column tile, row).
 #pragma unroll
-  for (int ti = 0; ti < 2; ti++)
+  for (int ti = 0; ti < TM; ti++)
 #pragma unroll
     for (int i = 0; i < 4; i++) {
       double v = rs[ti][i];
@@ -96,51 +97,55 @@ __device__ void ep_kernel(const EpArgs &e, const TileOut<Acc> &t, int row0, int 
     }
   if ((lane & 15) == 0) {
 #pragma unroll
-    for (int ti = 0; ti < 2; ti++)
+    for (int ti = 0; ti < TM; ti++)
 #pragma unroll
-      for (int i = 0; i < 4; i++) red[wc * 64 + wr * 32 + ti * 16 + 4 * (lane >> 4) + i] = rs[ti][i];
+      for (int i = 0; i < 4; i++)
+        red[wc * BMc + wr * 16 * TM + ti * 16 + 4 * (lane >> 4) + i] = rs[ti][i];
   }
   __syncthreads();
-  if (threadIdx.x < 64) {
-    int r = row0 + threadIdx.x;
-    if (r < M) e.part[(size_t)blockIdx.x * e.part_ld + r] = red[threadIdx.x] + red[64 + threadIdx.x];
+  for (int q = threadIdx.x; q < BMc; q += blockDim.x) {
+    int r = row0 + q;
+    if (r < M) e.part[(size_t)blockIdx.x * e.part_ld + r] = red[q] + red[BMc + q];
   }
 }
 
-template <typename Acc, typename TW>
-__device__ void ep_store(const EpArgs &e, const TileOut<Acc> &t, int row0, int col0, int M, int N,
-                         int lane, int wr, int wc) {
+template <typename Acc, typename TW, int TM, int TN>
+__device__ void ep_store(const EpArgs &e, const TileOut<Acc, TM, TN> &t, int row0, int col0,
+                         int M, int N, int lane, int wr, int wc) {
 #pragma unroll
-  for (int ti = 0; ti < 2; ti++)
+  for (int ti = 0; ti < TM; ti++)
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-      int r = row0 + wr * 32 + ti * 16 + 4 * (lane >> 4) + i;
+      int r = row0 + wr * 16 * TM + ti * 16 + 4 * (lane >> 4) + i;
       if (r >= M) continue;
 #pragma unroll
-      for (int tj = 0; tj < 2; tj++) {
-        int c = col0 + wc * 32 + tj * 16 + (lane & 15);
+      for (int tj = 0; tj < TN; tj++) {
+        int c = col0 + wc * 16 * TN + tj * 16 + (lane & 15);
         if (c < N) e.W_as<TW>()[(size_t)r * e.ldw + c] = (TW)t.v[ti][tj][i];
       }
     }
 }
 
-template <typename Acc>
-__device__ void ep_sumsq(const EpArgs &e, const TileOut<Acc> &t, int row0, int col0, int M,
-                         int N, int lane, int wr, int wc, double *red) {
+template <typename Acc, int TM, int TN>
+__device__ void ep_sumsq(const EpArgs &e, const TileOut<Acc, TM, TN> &t, int row0, int col0,
+                         int M, int N, int lane, int wr, int wc, double *red) {
+  constexpr int BNc = 32 * TN;
   // rows: L rows, cols: atoms. Column sums of squares over this row tile.
-  double cs[2] = {0, 0};
+  double cs[TN];
 #pragma unroll
-  for (int tj = 0; tj < 2; tj++)
+  for (int tj = 0; tj < TN; tj++) {
+    cs[tj] = 0;
 #pragma unroll
-    for (int ti = 0; ti < 2; ti++)
+    for (int ti = 0; ti < TM; ti++)
 #pragma unroll
       for (int i = 0; i < 4; i++) {
-        int r = row0 + wr * 32 + ti * 16 + 4 * (lane >> 4) + i;
+        int r = row0 + wr * 16 * TM + ti * 16 + 4 * (lane >> 4) + i;
         double v = r < M ? (double)t.v[ti][tj][i] : 0.0;
         cs[tj] += v * v;
       }
+  }
 #pragma unroll
-  for (int tj = 0; tj < 2; tj++) {
+  for (int tj = 0; tj < TN; tj++) {
     double v = cs[tj];
     v += __shfl_xor(v, 16, 64);
     v += __shfl_xor(v, 32, 64);
@@ -148,19 +153,30 @@ __device__ void ep_sumsq(const EpArgs &e, const TileOut<Acc> &t, int row0, int c
   }
   if (lane < 16) {
 #pragma unroll
-    for (int tj = 0; tj < 2; tj++) red[wr * 64 + wc * 32 + tj * 16 + lane] = cs[tj];
+    for (int tj = 0; tj < TN; tj++) red[wr * BNc + wc * 16 * TN + tj * 16 + lane] = cs[tj];
   }
   __syncthreads();
-  if (threadIdx.x < 64) {
-    int c = col0 + threadIdx.x;
-    if (c < N) e.part[(size_t)blockIdx.y * e.part_ld + c] = red[threadIdx.x] + red[64 + threadIdx.x];
+  for (int q = threadIdx.x; q < BNc; q += blockDim.x) {
+    int c = col0 + q;
+    if (c < N) e.part[(size_t)blockIdx.y * e.part_ld + c] = red[q] + red[BNc + q];
   }
 }
 
+template <int EP, typename Acc, typename TW, int TM, int TN>
+__device__ __forceinline__ void run_epilogue(const EpArgs &e, const TileOut<Acc, TM, TN> &t,
+                                             int row0, int col0, int M, int N, int lane, int wr,
+                                             int wc, double *red) {
+  if (EP == EP_KERNEL) ep_kernel<Acc, TW, TM, TN>(e, t, row0, col0, M, N, lane, wr, wc, red);
+  else if (EP == EP_STORE) ep_store<Acc, TW, TM, TN>(e, t, row0, col0, M, N, lane, wr, wc);
+  else ep_sumsq<Acc, TM, TN>(e, t, row0, col0, M, N, lane, wr, wc, red);
+}
+
 // ---------------------------------------------------------------------------
-// FP64 MFMA GEMM. Grid: x = column tiles (of B rows), y = row tiles (of A).
-// For EpKernel the block x index selects the energy partial slot.
+// FP64 MFMA GEMM, 64 x 64 tiles. Grid: x = column tiles (rows of B),
+// y = row tiles (rows of A). LDS holds k-major tiles so a fragment read
+// (16 rows at one k) is contiguous.
 // ---------------------------------------------------------------------------
+constexpr int F64_TM = 2, F64_TN = 2, F64_BM = 32 * F64_TM, F64_BN = 32 * F64_TN;
 constexpr int BK64 = 16;
 constexpr int PAD64 = 4;
 
@@ -168,19 +184,19 @@ template <int EP>
 __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K, const double *A, long long lda,
                                                   const double *B, long long ldb, int lower_tri,
                                                   EpArgs e) {
-  __shared__ double As[BK64][BM + PAD64];
-  __shared__ double Bs[BK64][BN + PAD64];
-  __shared__ double red[128];
+  __shared__ double As[BK64][F64_BM + PAD64];
+  __shared__ double Bs[BK64][F64_BN + PAD64];
+  __shared__ double red[2 * (F64_BM > F64_BN ? F64_BM : F64_BN)];
   int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   int wr = wid >> 1, wc = wid & 1;
-  int row0 = blockIdx.y * BM, col0 = blockIdx.x * BN;
+  int row0 = blockIdx.y * F64_BM, col0 = blockIdx.x * F64_BN;
   int kend = K;
-  if (lower_tri) kend = min(K, row0 + BM);
-  v4d acc[2][2];
+  if (lower_tri) kend = min(K, row0 + F64_BM);
+  v4d acc[F64_TM][F64_TN];
 #pragma unroll
-  for (int a = 0; a < 2; a++)
+  for (int a = 0; a < F64_TM; a++)
 #pragma unroll
-    for (int b = 0; b < 2; b++) acc[a][b] = v4d{0, 0, 0, 0};
+    for (int b = 0; b < F64_TN; b++) acc[a][b] = v4d{0, 0, 0, 0};
   int lr = tid >> 2, lk = (tid & 3) * 4;
   double ra[4], rb[4];
   auto gload = [&](int k0) {
@@ -204,71 +220,76 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K, const dou
     if (k0 + BK64 < kend) gload(k0 + BK64);
 #pragma unroll
     for (int kk = 0; kk < BK64; kk += 4) {
-      double af[2], bf[2];
+      double af[F64_TM], bf[F64_TN];
 #pragma unroll
-      for (int t = 0; t < 2; t++) {
-        af[t] = As[kk + (lane >> 4)][wr * 32 + t * 16 + (lane & 15)];
-        bf[t] = Bs[kk + (lane >> 4)][wc * 32 + t * 16 + (lane & 15)];
-      }
+      for (int t = 0; t < F64_TM; t++) af[t] = As[kk + (lane >> 4)][wr * 16 * F64_TM + t * 16 + (lane & 15)];
 #pragma unroll
-      for (int a = 0; a < 2; a++)
+      for (int t = 0; t < F64_TN; t++) bf[t] = Bs[kk + (lane >> 4)][wc * 16 * F64_TN + t * 16 + (lane & 15)];
 #pragma unroll
-        for (int b = 0; b < 2; b++)
+      for (int a = 0; a < F64_TM; a++)
+#pragma unroll
+        for (int b = 0; b < F64_TN; b++)
           acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[a], bf[b], acc[a][b], 0, 0, 0);
     }
   }
-  TileOut<double> t;
+  TileOut<double, F64_TM, F64_TN> t;
 #pragma unroll
-  for (int a = 0; a < 2; a++)
+  for (int a = 0; a < F64_TM; a++)
 #pragma unroll
-    for (int b = 0; b < 2; b++)
+    for (int b = 0; b < F64_TN; b++)
 #pragma unroll
       for (int i = 0; i < 4; i++) t.v[a][b][i] = acc[a][b][i];
-  if (EP == EP_KERNEL) ep_kernel<double, double>(e, t, row0, col0, M, N, lane, wr, wc, red);
-  else if (EP == EP_STORE) ep_store<double, double>(e, t, row0, col0, M, N, lane, wr, wc);
-  else ep_sumsq<double>(e, t, row0, col0, M, N, lane, wr, wc, red);
+  run_epilogue<EP, double, double, F64_TM, F64_TN>(e, t, row0, col0, M, N, lane, wr, wc, red);
 }
 
 // ---------------------------------------------------------------------------
-// Split-BF16 x3 GEMM on FP32 operands (chosen form). LDS holds hi and lo
-// BF16 tiles row-major along k (lane l reads 8 consecutive k at row l % 16,
-// k offset 8 (l / 16)).
+// Split-BF16 x3 GEMM on FP32 operands (chosen form), 128 x 128 tiles. Each
+// thread loads 16 consecutive k of one row of A and of B per k-step, splits
+// them into hi/lo BF16 and stores them to LDS as 16-byte vectors; LDS rows
+// are k-contiguous so lane l reads 8 consecutive k at row l % 16, k offset
+// 8 (l / 16).
 // ---------------------------------------------------------------------------
+constexpr int B16_TM = 4, B16_TN = 4, B16_BM = 32 * B16_TM, B16_BN = 32 * B16_TN;
 constexpr int BK16 = 32;
 constexpr int PADB = 8;
+constexpr int LDK = BK16 + PADB;
 
-__device__ __forceinline__ void split_bf16(float x, __bf16 &hi, __bf16 &lo) {
-  hi = (__bf16)x;
-  lo = (__bf16)(x - (float)hi);
+__device__ __forceinline__ void split8(const float *x, v8bf &hi, v8bf &lo) {
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    __bf16 h = (__bf16)x[q];
+    hi[q] = h;
+    lo[q] = (__bf16)(x[q] - (float)h);
+  }
 }
 
 template <int EP>
 __global__ __launch_bounds__(256) void k_gemm_bf16x3(int M, int N, int K, const float *A,
                                                      long long lda, const float *B, long long ldb,
                                                      int lower_tri, EpArgs e) {
-  __shared__ __align__(16) __bf16 Ah[BM][BK16 + PADB];
-  __shared__ __align__(16) __bf16 Al[BM][BK16 + PADB];
-  __shared__ __align__(16) __bf16 Bh[BN][BK16 + PADB];
-  __shared__ __align__(16) __bf16 Bl[BN][BK16 + PADB];
-  __shared__ double red[128];
+  __shared__ __align__(16) __bf16 Ah[B16_BM * LDK];
+  __shared__ __align__(16) __bf16 Al[B16_BM * LDK];
+  __shared__ __align__(16) __bf16 Bh[B16_BN * LDK];
+  __shared__ __align__(16) __bf16 Bl[B16_BN * LDK];
+  __shared__ double red[2 * (B16_BM > B16_BN ? B16_BM : B16_BN)];
   int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   int wr = wid >> 1, wc = wid & 1;
-  int row0 = blockIdx.y * BM, col0 = blockIdx.x * BN;
+  int row0 = blockIdx.y * B16_BM, col0 = blockIdx.x * B16_BN;
   int kend = K;
-  if (lower_tri) kend = min(K, row0 + BM);
-  v4f acc[2][2];
+  if (lower_tri) kend = min(K, row0 + B16_BM);
+  v4f acc[B16_TM][B16_TN];
 #pragma unroll
-  for (int a = 0; a < 2; a++)
+  for (int a = 0; a < B16_TM; a++)
 #pragma unroll
-    for (int b = 0; b < 2; b++) acc[a][b] = v4f{0, 0, 0, 0};
-  // Each thread loads 8 consecutive k of one row of A and of B.
-  int lr = tid >> 2, lk = (tid & 3) * 8;
-  float ra[8], rb[8];
+    for (int b = 0; b < B16_TN; b++) acc[a][b] = v4f{0, 0, 0, 0};
+  // 128 rows x 32 k per operand: thread -> row tid / 2, k half (tid & 1) * 16.
+  int lr = tid >> 1, lk = (tid & 1) * 16;
+  float ra[16], rb[16];
   auto gload = [&](int k0) {
+    int r = row0 + lr, c = col0 + lr;
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
+    for (int q = 0; q < 16; q++) {
       int k = k0 + lk + q;
-      int r = row0 + lr, c = col0 + lr;
       ra[q] = (r < M && k < kend) ? A[(size_t)r * lda + k] : 0.0f;
       rb[q] = (c < N && k < kend) ? B[(size_t)c * ldb + k] : 0.0f;
     }
@@ -277,52 +298,71 @@ __global__ __launch_bounds__(256) void k_gemm_bf16x3(int M, int N, int K, const 
   for (int k0 = 0; k0 < kend; k0 += BK16) {
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
-      __bf16 h, l;
-      split_bf16(ra[q], h, l);
-      Ah[lr][lk + q] = h;
-      Al[lr][lk + q] = l;
-      split_bf16(rb[q], h, l);
-      Bh[lr][lk + q] = h;
-      Bl[lr][lk + q] = l;
+    for (int h = 0; h < 2; h++) {
+      v8bf hi, lo;
+      split8(ra + 8 * h, hi, lo);
+      *reinterpret_cast<v8bf *>(&Ah[lr * LDK + lk + 8 * h]) = hi;
+      *reinterpret_cast<v8bf *>(&Al[lr * LDK + lk + 8 * h]) = lo;
+      split8(rb + 8 * h, hi, lo);
+      *reinterpret_cast<v8bf *>(&Bh[lr * LDK + lk + 8 * h]) = hi;
+      *reinterpret_cast<v8bf *>(&Bl[lr * LDK + lk + 8 * h]) = lo;
     }
     __syncthreads();
     if (k0 + BK16 < kend) gload(k0 + BK16);
-    v8bf ah[2], al[2], bh[2], bl[2];
+    v8bf ah[B16_TM], al[B16_TM], bh[B16_TN], bl[B16_TN];
+    int kk = 8 * (lane >> 4);
 #pragma unroll
-    for (int t = 0; t < 2; t++) {
-      int ar = wr * 32 + t * 16 + (lane & 15), br = wc * 32 + t * 16 + (lane & 15);
-      int kk = 8 * (lane >> 4);
-      ah[t] = *reinterpret_cast<const v8bf *>(&Ah[ar][kk]);
-      al[t] = *reinterpret_cast<const v8bf *>(&Al[ar][kk]);
-      bh[t] = *reinterpret_cast<const v8bf *>(&Bh[br][kk]);
-      bl[t] = *reinterpret_cast<const v8bf *>(&Bl[br][kk]);
+    for (int t = 0; t < B16_TM; t++) {
+      int ar = wr * 16 * B16_TM + t * 16 + (lane & 15);
+      ah[t] = *reinterpret_cast<const v8bf *>(&Ah[ar * LDK + kk]);
+      al[t] = *reinterpret_cast<const v8bf *>(&Al[ar * LDK + kk]);
     }
 #pragma unroll
-    for (int a = 0; a < 2; a++)
+    for (int t = 0; t < B16_TN; t++) {
+      int br = wc * 16 * B16_TN + t * 16 + (lane & 15);
+      bh[t] = *reinterpret_cast<const v8bf *>(&Bh[br * LDK + kk]);
+      bl[t] = *reinterpret_cast<const v8bf *>(&Bl[br * LDK + kk]);
+    }
 #pragma unroll
-      for (int b = 0; b < 2; b++) {
+    for (int a = 0; a < B16_TM; a++)
+#pragma unroll
+      for (int b = 0; b < B16_TN; b++)
         acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
+#pragma unroll
+    for (int a = 0; a < B16_TM; a++)
+#pragma unroll
+      for (int b = 0; b < B16_TN; b++)
         acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
+#pragma unroll
+    for (int a = 0; a < B16_TM; a++)
+#pragma unroll
+      for (int b = 0; b < B16_TN; b++)
         acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
-      }
   }
-  TileOut<float> t;
+  TileOut<float, B16_TM, B16_TN> t;
 #pragma unroll
-  for (int a = 0; a < 2; a++)
+  for (int a = 0; a < B16_TM; a++)
 #pragma unroll
-    for (int b = 0; b < 2; b++)
+    for (int b = 0; b < B16_TN; b++)
 #pragma unroll
       for (int i = 0; i < 4; i++) t.v[a][b][i] = acc[a][b][i];
-  if (EP == EP_KERNEL) ep_kernel<float, float>(e, t, row0, col0, M, N, lane, wr, wc, red);
-  else if (EP == EP_STORE) ep_store<float, float>(e, t, row0, col0, M, N, lane, wr, wc);
-  else ep_sumsq<float>(e, t, row0, col0, M, N, lane, wr, wc, red);
+  run_epilogue<EP, float, float, B16_TM, B16_TN>(e, t, row0, col0, M, N, lane, wr, wc, red);
+}
+
+int gemm_col_tiles(int form, int N) {
+  int bn = form == FM_FORM_FP64 ? F64_BN : B16_BN;
+  return (N + bn - 1) / bn;
+}
+
+int gemm_row_tiles(int form, int M) {
+  int bm = form == FM_FORM_FP64 ? F64_BM : B16_BM;
+  return (M + bm - 1) / bm;
 }
 
 void gemm_f64(hipStream_t st, int ep, int M, int N, int K, const double *A, long long lda,
               const double *B, long long ldb, int lower_tri, const EpArgs &e) {
   if (M <= 0 || N <= 0) return;
-  dim3 grid((N + BN - 1) / BN, (M + BM - 1) / BM);
+  dim3 grid((N + F64_BN - 1) / F64_BN, (M + F64_BM - 1) / F64_BM);
   if (ep == EP_KERNEL)
     hipLaunchKernelGGL(k_gemm_f64<EP_KERNEL>, grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb,
                        lower_tri, e);
@@ -338,7 +378,7 @@ void gemm_f64(hipStream_t st, int ep, int M, int N, int K, const double *A, long
 void gemm_bf16x3(hipStream_t st, int ep, int M, int N, int K, const float *A, long long lda,
                  const float *B, long long ldb, int lower_tri, const EpArgs &e) {
   if (M <= 0 || N <= 0) return;
-  dim3 grid((N + BN - 1) / BN, (M + BM - 1) / BM);
+  dim3 grid((N + B16_BN - 1) / B16_BN, (M + B16_BM - 1) / B16_BM);
   if (ep == EP_KERNEL)
     hipLaunchKernelGGL(k_gemm_bf16x3<EP_KERNEL>, grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb,
                        lower_tri, e);

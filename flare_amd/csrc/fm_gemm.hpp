@@ -25,6 +25,12 @@ struct EpArgs {
   }
 };
 
+// Number of column / row tiles a contraction of the given form uses (the
+// energy partials have one slot per column tile, the variance partials one
+// per row tile).
+int gemm_col_tiles(int form, int N);
+int gemm_row_tiles(int form, int M);
+
 void gemm_f64(hipStream_t st, int ep, int M, int N, int K, const double *A, long long lda,
               const double *B, long long ldb, int lower_tri, const EpArgs &e);
 void gemm_bf16x3(hipStream_t st, int ep, int M, int N, int K, const float *A, long long lda,

@@ -141,14 +141,27 @@ struct fm_model_s {
   std::vector<double> host_alpha;
 };
 
+struct fm_mapped_s {
+  fm_context ctx = nullptr;
+  int power = 2, normalized = 1, nos = 0, N = 0, lmax = 0, cutkind = 0, nd = 0;
+  std::vector<double> cutoffs;       // nos x nos
+  std::vector<fm::DevBuf> beta;      // per species: d x d (power 2) or d
+};
+
 namespace fm {
+// fm_mapped.hip
+void mapping_coefficients(fm_model m, std::vector<double> &beta, int64_t &beta_size);
+void mapped_compute(fm_mapped mp, fm_structure s, double *atom_energy, double *forces,
+                    double *virial);
+void mapped_upload(fm_mapped mp, const double *beta);
+void mapped_parse(const std::string &path, fm_mapped mp, std::vector<double> &beta);
 // Launchers (fm_neighbors.hip)
 void build_neighbors(fm_structure s);
 void build_bonds(fm_structure s);
 // fm_descriptors.hip
 void launch_b2(fm_structure s);
 void launch_adjoint(fm_structure s, const void *G, int64_t g_ld, void *fbond,
-                    double *virial_atom);
+                    double *virial_atom, int project, double empty_norm, double empty_nn);
 // fm_predict.cpp
 void predict(fm_model m, fm_structure s, int flags, int form, double *mean_efs,
              double *local_var, double *local_energy);

@@ -1,0 +1,317 @@
+// The mapped model: mapping coefficients of a NormalizedDotProduct sparse GP
+// (normalized_dot_product.cpp:631-767) and the pair_style flare evaluation
+// from them (pair_flare.cpp:68-184, 262-419; lammps_descriptor.cpp:209-315).
+//
+// beta_s = sum_j sigma^2 alpha_j x_hat_j x_hat_j^T is one FP64 contraction
+// over the sparse set per species. Evaluation: P = B_s beta_s on the matrix
+// cores, then per atom E = B.P / |B|^2 (power 2, normalized) and dE/dB, and
+// the same adjoint kernel as the sparse-GP path turns dE/dB into bond
+// forces.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "fm_device.hpp"
+#include "fm_gemm.hpp"
+#include "fm_internal.hpp"
+
+namespace fm {
+
+// XTa[k][j] = sigma^2 alpha_j XT[k][j]
+__global__ void k_scale_cols(const double *XT, const double *alpha, double sig2, int d, int Ms,
+                             double *out) {
+  long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (t >= (long long)d * Ms) return;
+  int j = (int)(t % Ms);
+  out[t] = XT[t] * sig2 * alpha[j];
+}
+
+void mapping_coefficients(fm_model m, std::vector<double> &beta, int64_t &beta_size) {
+  fm_context ctx = m->ctx;
+  hipStream_t st = ctx->stream;
+  int p = (int)m->power;
+  FM_CHECK(m->power == 1.0 || m->power == 2.0, FM_ERR_INVALID,
+           "Mapping coefficients of the normalized dot product kernel are implemented for "
+           "power 1 and 2 only");
+  int d = m->nd;
+  beta_size = p == 1 ? d : (int64_t)d * (d + 1) / 2;
+  beta.assign((size_t)m->nos * beta_size, 0.0);
+  double sig2 = m->sigma * m->sigma;
+  std::vector<double> full((size_t)d * (p == 1 ? 1 : d));
+  for (int t = 0; t < m->nos; t++) {
+    int Ms = m->M_by_type[t];
+    if (Ms == 0) continue;
+    DevBuf XTa(&ctx->pool, (size_t)d * Ms * sizeof(double));
+    long long tot = (long long)d * Ms;
+    hipLaunchKernelGGL(k_scale_cols, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
+                       m->XT[t].as<double>(), m->alpha[t].as<double>(), sig2, d, Ms,
+                       XTa.as<double>());
+    DevBuf C(&ctx->pool, full.size() * sizeof(double));
+    EpArgs e;
+    e.W = C.as<void>();
+    if (p == 2) {
+      e.ldw = d;
+      gemm_f64(st, EP_STORE, d, d, Ms, XTa.as<double>(), Ms, m->XT[t].as<double>(), Ms, 0, e);
+    } else {
+      // beta = sum_j sigma^2 alpha_j x_hat_j: a ones-vector contraction with
+      // the alpha-scaled X_hat^T (1 x d).
+      std::vector<double> ones(Ms, 1.0);
+      DevBuf a(&ctx->pool, (size_t)Ms * sizeof(double));
+      FM_HIP(hipMemcpyAsync(a.as<double>(), ones.data(), Ms * sizeof(double),
+                            hipMemcpyHostToDevice, st));
+      FM_HIP(hipStreamSynchronize(st));
+      e.ldw = d;
+      gemm_f64(st, EP_STORE, 1, d, Ms, a.as<double>(), Ms, XTa.as<double>(), Ms, 0, e);
+      FM_HIP(hipStreamSynchronize(st));
+    }
+    FM_HIP(hipMemcpyAsync(full.data(), C.as<double>(), full.size() * sizeof(double),
+                          hipMemcpyDeviceToHost, st));
+    FM_HIP(hipStreamSynchronize(st));
+    double *row = &beta[(size_t)t * beta_size];
+    if (p == 1) {
+      for (int k = 0; k < d; k++) row[k] = full[k];
+    } else {
+      int64_t c = 0;
+      for (int k = 0; k < d; k++)
+        for (int l = k; l < d; l++) row[c++] = (k == l ? 1.0 : 2.0) * full[(size_t)k * d + l];
+    }
+  }
+}
+
+// Per atom: energy and G (the input of the adjoint) from P = B beta.
+//   normalized p=2: E = B.P/|B|^2, G = 2P/|B| (projected in the adjoint)
+//   normalized p=1: E = B.beta/|B|, G = beta
+//   unnormalised p=2: E = B.P, w = 2P;  p=1: E = B.beta, w = beta
+// (lammps_descriptor.cpp:259-282). Atoms with |B|^2 < 1e-8 get E = 0
+// (pair_flare.cpp:134-136).
+__global__ void k_mapped_energy(int ns, int ro, int d, const double *B, const double *P,
+                                const double *beta, int power, int normalized,
+                                const double *norm, const double *nn, double *E_row,
+                                double *G) {
+  int lane = threadIdx.x & 63;
+  int r = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (r >= ns) return;
+  int row = ro + r;
+  const double *b = B + (size_t)row * d;
+  double *g = G + (size_t)row * d;
+  double n2 = nn[row], nrm = norm[row];
+  if (n2 < 1e-8) {
+    for (int k = lane; k < d; k += 64) g[k] = 0;
+    if (lane == 0) E_row[row] = 0;
+    return;
+  }
+  const double *v = power == 2 ? P + (size_t)r * d : beta;
+  double acc = 0;
+  for (int k = lane; k < d; k += 64) acc += b[k] * v[k];
+  acc = wave_sum(acc);
+  double e;
+  if (normalized) e = power == 2 ? acc / n2 : acc / nrm;
+  else e = acc;
+  double scale = power == 2 ? (normalized ? 2.0 / nrm : 2.0) : 1.0;
+  for (int k = lane; k < d; k += 64) g[k] = scale * v[k];
+  if (lane == 0) E_row[row] = e;
+}
+
+__global__ void k_mapped_gather(int n, const int *bond_off, const int *in_off, const int *in_bond,
+                                const double *fbond, const int *row_of_atom, const double *E_row,
+                                const double *vir, double *F, double *Eatom, double *blk_part) {
+  __shared__ double sh[6][256];
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double v[6] = {0, 0, 0, 0, 0, 0};
+  if (i < n) {
+    double f[3] = {0, 0, 0};
+    for (int b = bond_off[i]; b < bond_off[i + 1]; b++)
+      for (int c = 0; c < 3; c++) f[c] += fbond[3 * (size_t)b + c];
+    for (int a = in_off[i]; a < in_off[i + 1]; a++) {
+      int b = in_bond[a];
+      for (int c = 0; c < 3; c++) f[c] -= fbond[3 * (size_t)b + c];
+    }
+    for (int c = 0; c < 3; c++) F[3 * (size_t)i + c] = f[c];
+    Eatom[i] = E_row[row_of_atom[i]];
+    for (int k = 0; k < 6; k++) v[k] = vir[6 * (size_t)i + k];
+  }
+  for (int k = 0; k < 6; k++) sh[k][threadIdx.x] = v[k];
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+      for (int k = 0; k < 6; k++) sh[k][threadIdx.x] += sh[k][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) blk_part[(size_t)blockIdx.x * 6 + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+__global__ void k_mapped_sum(int nblk, const double *blk_part, double *vir) {
+  int k = threadIdx.x;
+  if (k >= 6) return;
+  double acc = 0;
+  for (int b = 0; b < nblk; b++) acc += blk_part[(size_t)b * 6 + k];
+  vir[k] = acc;
+}
+
+void mapped_compute(fm_mapped mp, fm_structure s, double *atom_energy, double *forces,
+                    double *virial) {
+  fm_context ctx = s->ctx;
+  hipStream_t st = ctx->stream;
+  Pool *pool = &ctx->pool;
+  FM_CHECK(s->has_b2 && s->form == FM_FORM_FP64, FM_ERR_STATE,
+           "the structure needs FP64 B2 descriptors for the mapped potential");
+  FM_CHECK(s->nos == mp->nos && s->N == mp->N && s->lmax == mp->lmax &&
+               s->cutkind == mp->cutkind && s->r1 == 0.0,
+           FM_ERR_INVALID, "structure descriptors do not match the potential's settings");
+  for (size_t k = 0; k < mp->cutoffs.size(); k++)
+    FM_CHECK(s->cutoffs[k] == mp->cutoffs[k], FM_ERR_INVALID,
+             "structure cutoffs do not match the potential's cutoffs");
+  int n = s->n, d = s->nd;
+  DevBuf G(pool, std::max<size_t>(1, (size_t)n * d * sizeof(double)));
+  DevBuf E_row(pool, std::max<size_t>(1, (size_t)n * sizeof(double)));
+  size_t maxns = 0;
+  for (int t = 0; t < s->nos; t++) maxns = std::max(maxns, (size_t)s->n_by_type[t]);
+  DevBuf P;
+  if (mp->power == 2) P.alloc(pool, std::max<size_t>(1, maxns * d * sizeof(double)));
+  for (int t = 0; t < s->nos; t++) {
+    int ns = s->n_by_type[t], ro = s->type_off[t];
+    if (ns == 0) continue;
+    const double *Bs = s->B.as<double>() + (size_t)ro * d;
+    if (mp->power == 2) {
+      EpArgs e;
+      e.W = P.as<void>();
+      e.ldw = d;
+      gemm_f64(st, EP_STORE, ns, d, d, Bs, d, mp->beta[t].as<double>(), d, 0, e);
+    }
+    hipLaunchKernelGGL(k_mapped_energy, dim3((ns + 3) / 4), dim3(256), 0, st, ns, ro, d,
+                       s->B.as<double>(), mp->power == 2 ? P.as<double>() : nullptr,
+                       mp->beta[t].as<double>(), mp->power, mp->normalized,
+                       s->norm.as<double>(), s->nn.as<double>(), E_row.as<double>(),
+                       G.as<double>());
+  }
+  DevBuf fbond(pool, std::max<size_t>(1, (size_t)3 * s->n_bonds * sizeof(double)));
+  DevBuf vir(pool, std::max<size_t>(1, (size_t)6 * n * sizeof(double)));
+  DevBuf F(pool, std::max<size_t>(1, (size_t)3 * n * sizeof(double)));
+  DevBuf Ea(pool, std::max<size_t>(1, (size_t)n * sizeof(double)));
+  DevBuf vsum(pool, 6 * sizeof(double));
+  if (n > 0) {
+    launch_adjoint(s, G.as<void>(), d, fbond.as<void>(), vir.as<double>(),
+                   mp->normalized ? 1 : 0, -1.0, 1e-8);
+    int tpb = 256, nblk = (n + tpb - 1) / tpb;
+    DevBuf blk(pool, (size_t)nblk * 6 * sizeof(double));
+    hipLaunchKernelGGL(k_mapped_gather, dim3(nblk), dim3(tpb), 0, st, n, s->bond_off.as<int>(),
+                       s->in_off.as<int>(), s->in_bond.as<int>(), fbond.as<double>(),
+                       s->row_of_atom.as<int>(), E_row.as<double>(), vir.as<double>(),
+                       F.as<double>(), Ea.as<double>(), blk.as<double>());
+    hipLaunchKernelGGL(k_mapped_sum, dim3(1), dim3(64), 0, st, nblk, blk.as<double>(),
+                       vsum.as<double>());
+    FM_HIP(hipGetLastError());
+    if (atom_energy)
+      FM_HIP(hipMemcpyAsync(atom_energy, Ea.as<double>(), n * sizeof(double),
+                            hipMemcpyDeviceToHost, st));
+    if (forces)
+      FM_HIP(hipMemcpyAsync(forces, F.as<double>(), 3 * (size_t)n * sizeof(double),
+                            hipMemcpyDeviceToHost, st));
+    if (virial)
+      FM_HIP(hipMemcpyAsync(virial, vsum.as<double>(), 6 * sizeof(double), hipMemcpyDeviceToHost,
+                            st));
+  } else if (virial) {
+    for (int k = 0; k < 6; k++) virial[k] = 0;
+  }
+  FM_HIP(hipStreamSynchronize(st));
+}
+
+// Builds the device beta matrices: power 2 expands the upper triangle into a
+// symmetric matrix with off-diagonal entries halved (pair_flare.cpp:396-418).
+void mapped_upload(fm_mapped mp, const double *beta) {
+  fm_context ctx = mp->ctx;
+  hipStream_t st = ctx->stream;
+  int d = mp->nd;
+  int64_t bs = mp->power == 1 ? d : (int64_t)d * (d + 1) / 2;
+  for (int t = 0; t < mp->nos; t++) {
+    const double *src = beta + (size_t)t * bs;
+    std::vector<double> full;
+    if (mp->power == 1) {
+      full.assign(src, src + d);
+    } else {
+      full.assign((size_t)d * d, 0.0);
+      int64_t c = 0;
+      for (int i = 0; i < d; i++)
+        for (int j = i; j < d; j++) {
+          if (i == j) full[(size_t)i * d + j] = src[c];
+          else {
+            double h = src[c] / 2;
+            full[(size_t)i * d + j] = h;
+            full[(size_t)j * d + i] = h;
+          }
+          c++;
+        }
+    }
+    mp->beta.emplace_back(&ctx->pool, full.size() * sizeof(double));
+    FM_HIP(hipMemcpyAsync(mp->beta.back().as<double>(), full.data(), full.size() * sizeof(double),
+                          hipMemcpyHostToDevice, st));
+    FM_HIP(hipStreamSynchronize(st));
+  }
+}
+
+// PairFLARE::read_file (pair_flare.cpp:262-419) with grab (:427-439): header
+// lines, then whitespace-separated cutoffs and beta values across lines.
+void mapped_parse(const std::string &path, fm_mapped mp, std::vector<double> &beta) {
+  std::ifstream f(path);
+  FM_CHECK(f.good(), FM_ERR_INVALID, "Cannot open potential file " + path);
+  std::string line;
+  auto next = [&](const char *what) {
+    FM_CHECK((bool)std::getline(f, line), FM_ERR_INVALID, std::string("truncated file: ") + what);
+  };
+  next("date line");
+  next("kernel line");
+  std::string kernel;
+  {
+    std::istringstream ls(line);
+    ls >> mp->power >> kernel;
+  }
+  next("radial line");
+  std::string radial;
+  {
+    std::istringstream ls(line);
+    ls >> radial;
+  }
+  next("sizes line");
+  int64_t beta_size = 0;
+  {
+    std::istringstream ls(line);
+    ls >> mp->nos >> mp->N >> mp->lmax >> beta_size;
+  }
+  next("cutoff line");
+  std::string cut;
+  {
+    std::istringstream ls(line);
+    ls >> cut;
+  }
+  FM_CHECK(mp->nos >= 1 && mp->N >= 1 && mp->lmax >= 0, FM_ERR_INVALID, "bad descriptor sizes");
+  mp->cutoffs.clear();
+  double v;
+  while ((int)mp->cutoffs.size() < mp->nos * mp->nos && f >> v) mp->cutoffs.push_back(v);
+  FM_CHECK((int)mp->cutoffs.size() == mp->nos * mp->nos, FM_ERR_INVALID, "truncated cutoffs");
+  int nr = mp->N * mp->nos;
+  mp->nd = nr * (nr + 1) / 2 * (mp->lmax + 1);
+  int64_t check;
+  if (mp->power == 1) check = mp->nd;
+  else if (mp->power == 2) check = (int64_t)mp->nd * (mp->nd + 1) / 2;
+  else throw Error(FM_ERR_INVALID, "Power should be 1 or 2.");
+  FM_CHECK(check == beta_size, FM_ERR_INVALID, "Beta size doesn't match the number of descriptors.");
+  FM_CHECK(radial == "chebyshev", FM_ERR_INVALID, "Please use chebyshev radial basis function.");
+  if (cut == "quadratic") mp->cutkind = FM_CUTOFF_QUADRATIC;
+  else if (cut == "cosine") mp->cutkind = FM_CUTOFF_COSINE;
+  else throw Error(FM_ERR_INVALID, "Please use quadratic or cosine cutoff function.");
+  if (kernel == "NormalizedDotProduct") mp->normalized = 1;
+  else if (kernel == "DotProduct") mp->normalized = 0;
+  else throw Error(FM_ERR_INVALID, "Kernel string not recognized, expected <power> <kernel string>");
+  beta.clear();
+  beta.reserve((size_t)beta_size * mp->nos);
+  while ((int64_t)beta.size() < beta_size * mp->nos && f >> v) beta.push_back(v);
+  FM_CHECK((int64_t)beta.size() == beta_size * mp->nos, FM_ERR_INVALID, "truncated beta block");
+}
+
+}  // namespace fm

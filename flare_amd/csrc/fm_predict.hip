@@ -116,9 +116,9 @@ void predict(fm_model m, fm_structure s, int flags, int form, double *mean_efs,
   for (int t = 0; t < nos; t++) {
     size_t ns = s->n_by_type[t], Ms = m->M_by_type[t];
     wo[t + 1] = wo[t] + ns * Ms;
-    eo[t + 1] = eo[t] + ns * (size_t)std::max(1, (int)((Ms + 63) / 64));
+    eo[t + 1] = eo[t] + ns * (size_t)std::max(1, gemm_col_tiles(form, (int)Ms));
     int Lr = want_var ? m->L_rows[t] : 0;
-    qo[t + 1] = qo[t] + ns * (size_t)((Lr + 63) / 64);
+    qo[t + 1] = qo[t] + ns * (size_t)gemm_row_tiles(FM_FORM_FP64, Lr);
   }
   DevBuf W(pool, std::max<size_t>(1, wo[nos] * esz));
   DevBuf Kb;
@@ -204,8 +204,8 @@ void predict(fm_model m, fm_structure s, int flags, int form, double *mean_efs,
   for (int t = 0; t < nos; t++) {
     int ns = s->n_by_type[t], Ms = m->M_by_type[t], ro = s->type_off[t];
     if (ns == 0) continue;
-    int n_et = Ms == 0 ? 1 : (Ms + 63) / 64;
-    int n_qt = (want_var && Ms > 0) ? (m->L_rows[t] + 63) / 64 : 0;
+    int n_et = Ms == 0 ? 1 : gemm_col_tiles(form, Ms);
+    int n_qt = (want_var && Ms > 0) ? gemm_row_tiles(FM_FORM_FP64, m->L_rows[t]) : 0;
     hipLaunchKernelGGL(k_species_final, dim3((ns + 255) / 256), dim3(256), 0, st, ns, ro,
                        Epart.as<double>() + eo[t], n_et, Qpart.as<double>() + qo[t], n_qt,
                        s->norm.as<double>(), s->nn.as<double>(), sig2, m->power,
@@ -220,7 +220,8 @@ void predict(fm_model m, fm_structure s, int flags, int form, double *mean_efs,
   DevBuf vir(pool, std::max<size_t>(1, (size_t)6 * n * sizeof(double)));
   double *o = out.as<double>();
   if (n > 0) {
-    launch_adjoint(s, G.as<void>(), nd, fbond.as<void>(), vir.as<double>());
+    launch_adjoint(s, G.as<void>(), nd, fbond.as<void>(), vir.as<double>(), 1, kEmptyThresh,
+                   -1.0);
     int tpb = 256, nblk = (n + tpb - 1) / tpb;
     DevBuf blk(pool, (size_t)nblk * 7 * sizeof(double));
     if (f64)

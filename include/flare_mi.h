@@ -46,6 +46,7 @@ extern "C" {
 typedef struct fm_context_s *fm_context;
 typedef struct fm_structure_s *fm_structure;
 typedef struct fm_model_s *fm_model;
+typedef struct fm_mapped_s *fm_mapped;
 
 const char *fm_last_error(void);
 
@@ -143,6 +144,45 @@ int fm_model_destroy(fm_model m);
  *   local_energy N, atom order (this kernel's per-atom energies). */
 int fm_predict(fm_model m, fm_structure s, int flags, int form,
                double *mean_efs, double *local_var, double *local_energy);
+
+/* Mapping coefficients of a power-1 or power-2 model
+ * (NormalizedDotProduct::compute_mapping_coefficients,
+ * normalized_dot_product.cpp:631-767): per species beta = sum_j sigma^2
+ * alpha_j x_hat_j (power 1, beta_size = d) or the upper triangle of
+ * sum_j sigma^2 alpha_j x_hat_j x_hat_j^T with off-diagonal entries doubled
+ * (power 2, beta_size = d(d+1)/2). beta is n_species x beta_size; pass NULL
+ * to query beta_size. */
+int fm_model_mapping_coefficients(fm_model m, double *beta, int64_t *beta_size);
+
+/* The pair_style flare potential (lammps_plugins/pair_flare.cpp), from a
+ * coefficient file (PairFLARE::read_file, pair_flare.cpp:262-419) or from
+ * arrays in the same layout: power 1 or 2, NormalizedDotProduct
+ * (normalized = 1) or DotProduct (0), chebyshev radial basis, cutoff matrix
+ * [central][neighbour], beta n_species x beta_size. */
+typedef struct {
+  int power;
+  int normalized;
+  int n_species;
+  int n_max;
+  int l_max;
+  int cutoff_function;
+  const double *cutoffs;
+  const double *beta;
+} fm_mapped_desc;
+
+int fm_mapped_create(fm_context ctx, const fm_mapped_desc *desc, fm_mapped *out);
+int fm_mapped_read_file(fm_context ctx, const char *path, fm_mapped *out);
+/* Settings of a potential; cutoffs receives n_species^2 values (may be NULL). */
+int fm_mapped_info(fm_mapped mp, int *power, int *normalized, int *n_species,
+                   int *n_max, int *l_max, int *cutoff_function, double *cutoffs);
+int fm_mapped_destroy(fm_mapped mp);
+/* PairFLARE::compute (pair_flare.cpp:68-184) on a periodic structure whose
+ * FP64 B2 descriptors were computed with the potential's settings
+ * (radial_r1 = 0): per-atom energies (atom order), forces N x 3, and the
+ * virial sum_bonds -f (x) r (xx, xy, xz, yy, yz, zz). Any output may be
+ * NULL. */
+int fm_mapped_compute(fm_mapped mp, fm_structure s, double *atom_energy,
+                      double *forces, double *virial);
 
 /* Device-side durations (HIP events on the context stream) of the stages
  * last run on a context, in milliseconds, and the number of kernel launches

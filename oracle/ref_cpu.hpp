@@ -6,9 +6,8 @@
 // cpu_baseline leg may use it.
 //
 // Each function cites the reference file:line whose algorithm it restates
-// (paths relative to the mir-group/flare source tree).
-//
-// Parity status of each part (how it is pinned) is listed in oracle/README.md.
+// (paths relative to the mir-group/flare source tree). The checks that pin
+// each part live in tests/test_oracle_*.py.
 #pragma once
 #include <cstdint>
 #include <string>

@@ -15,48 +15,65 @@ namespace ref {
 static const double kEmpty = 1e-8;  // normalized_dot_product.cpp:45,131
 
 // ---------------------------------------------------------------------------
-// Blocked GEMM, C = A * B^T (row-major), used by the scalable mode.
+// Blocked GEMM, C = A * B^T (row-major), used by the scalable mode: B is
+// packed k-major per (j, k) block and a 4 x 8 register block of C is
+// accumulated with 4-wide vectors.
 // ---------------------------------------------------------------------------
+typedef double v4df __attribute__((vector_size(32)));
+
 void gemm_abt(int m, int n, int k, const double *A, const double *B, double *C) {
-  const int MB = 32, NB = 32, KB = 256;
-#pragma omp parallel for collapse(2) schedule(dynamic)
-  for (int i0 = 0; i0 < m; i0 += MB)
-    for (int j0 = 0; j0 < n; j0 += NB) {
-      int i1 = std::min(m, i0 + MB), j1 = std::min(n, j0 + NB);
-      for (int i = i0; i < i1; i++)
-        for (int j = j0; j < j1; j++) C[(size_t)i * n + j] = 0;
-      for (int k0 = 0; k0 < k; k0 += KB) {
-        int k1 = std::min(k, k0 + KB);
-        int i = i0;
-        for (; i + 4 <= i1; i += 4) {
-          int j = j0;
-          for (; j + 4 <= j1; j += 4) {
-            double acc[4][4] = {{0}};
-            const double *a0 = A + (size_t)i * k, *b0 = B + (size_t)j * k;
-            for (int kk = k0; kk < k1; kk++) {
-              double av[4] = {a0[kk], a0[k + kk], a0[2 * (size_t)k + kk], a0[3 * (size_t)k + kk]};
-              double bv[4] = {b0[kk], b0[k + kk], b0[2 * (size_t)k + kk], b0[3 * (size_t)k + kk]};
-              for (int p = 0; p < 4; p++)
-                for (int q = 0; q < 4; q++) acc[p][q] += av[p] * bv[q];
+  const int MB = 64, NB = 128, KB = 256;
+  int nib = (m + MB - 1) / MB, njb = (n + NB - 1) / NB;
+#pragma omp parallel
+  {
+    std::vector<double> Bp((size_t)KB * NB);
+#pragma omp for collapse(2) schedule(dynamic)
+    for (int ib = 0; ib < nib; ib++)
+      for (int jb = 0; jb < njb; jb++) {
+        int i0 = ib * MB, j0 = jb * NB;
+        int i1 = std::min(m, i0 + MB), j1 = std::min(n, j0 + NB);
+        int nb = j1 - j0, nbp = (nb + 7) / 8 * 8;
+        for (int i = i0; i < i1; i++)
+          for (int j = j0; j < j1; j++) C[(size_t)i * n + j] = 0;
+        for (int k0 = 0; k0 < k; k0 += KB) {
+          int kb = std::min(KB, k - k0);
+          for (int kk = 0; kk < kb; kk++)
+            for (int jj = 0; jj < nbp; jj++)
+              Bp[(size_t)kk * nbp + jj] = jj < nb ? B[(size_t)(j0 + jj) * k + k0 + kk] : 0.0;
+          int i = i0;
+          for (; i + 4 <= i1; i += 4) {
+            for (int jj = 0; jj < nbp; jj += 8) {
+              v4df acc[4][2];
+              for (int r = 0; r < 4; r++) acc[r][0] = acc[r][1] = v4df{0, 0, 0, 0};
+              const double *a0 = A + (size_t)i * k + k0;
+              for (int kk = 0; kk < kb; kk++) {
+                v4df b0, b1;
+                __builtin_memcpy(&b0, &Bp[(size_t)kk * nbp + jj], 32);
+                __builtin_memcpy(&b1, &Bp[(size_t)kk * nbp + jj + 4], 32);
+                for (int r = 0; r < 4; r++) {
+                  double av = a0[(size_t)r * k + kk];
+                  v4df a = {av, av, av, av};
+                  acc[r][0] += a * b0;
+                  acc[r][1] += a * b1;
+                }
+              }
+              for (int r = 0; r < 4; r++)
+                for (int q = 0; q < 8; q++) {
+                  int j = j0 + jj + q;
+                  if (j < j1) C[(size_t)(i + r) * n + j] += acc[r][q >> 2][q & 3];
+                }
             }
-            for (int p = 0; p < 4; p++)
-              for (int q = 0; q < 4; q++) C[(size_t)(i + p) * n + j + q] += acc[p][q];
           }
-          for (; j < j1; j++)
-            for (int p = 0; p < 4; p++) {
+          for (; i < i1; i++)
+            for (int jj = 0; jj < nb; jj++) {
               double acc = 0;
-              for (int kk = k0; kk < k1; kk++) acc += A[(size_t)(i + p) * k + kk] * B[(size_t)j * k + kk];
-              C[(size_t)(i + p) * n + j] += acc;
+              for (int kk = 0; kk < kb; kk++)
+                acc += A[(size_t)i * k + k0 + kk] * Bp[(size_t)kk * nbp + jj];
+              C[(size_t)i * n + j0 + jj] += acc;
             }
         }
-        for (; i < i1; i++)
-          for (int j = j0; j < j1; j++) {
-            double acc = 0;
-            for (int kk = k0; kk < k1; kk++) acc += A[(size_t)i * k + kk] * B[(size_t)j * k + kk];
-            C[(size_t)i * n + j] += acc;
-          }
       }
-    }
+  }
 }
 
 // ---------------------------------------------------------------------------

@@ -1,0 +1,119 @@
+"""The mapped model on the GPU: mapping coefficients
+(normalized_dot_product.cpp:631-767), the coefficient file
+(sparse_gp.cpp:1200-1264) and pair_style flare (pair_flare.cpp:68-184,
+262-419) against the oracle; the file evaluated by the GPU potential equals
+the sparse-GP prediction (tests/test_lammps.py:33-120)."""
+import os
+
+import numpy as np
+import pytest
+
+from helpers import random_cell, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def C():
+    from flare_amd import _C_flare
+    return _C_flare
+
+
+def setup(C, power, nos=2, settings=(2, 4, 3), cut=None, seed=12):
+    cell, species, pos = random_cell(30, seed, nos=nos, scale=7.5, skew=0.5)
+    cut = np.array([[4.5, 4.0], [4.0, 4.2]]) if cut is None else cut
+    rc = float(cut.max())
+    calc = C.B2("chebyshev", "quadratic", [0.0, rc], [], list(settings), cut)
+    s = C.Structure(cell, species, pos, rc, [calc])
+    sgp = C.SparseGP([C.NormalizedDotProduct(1.5, power)], 0.01, 0.05, 0.005)
+    d = s.descriptors[0]
+    atoms = [int(d.atom_indices[t][r]) for t in range(nos) for r in range(0, d.n_clusters_by_type[t], 3)]
+    sgp.add_specific_environments(s, atoms)
+    sgp.update_L_inverse()
+    sgp.set_alpha(np.random.default_rng(seed).normal(scale=0.1, size=sgp.n_sparse))
+    return cell, species, pos, cut, calc, s, sgp
+
+
+@pytest.mark.parametrize("power", [1.0, 2.0])
+def test_mapping_coefficients(C, oracle, power):
+    from flare_amd.mapping import mapping_coefficients
+    cell, species, pos, cut, calc, s, sgp = setup(C, power)
+    cd = sgp.sparse_descriptors[0]
+    m = oracle.Model(1.5, power, cd.descriptors, cd.descriptor_norms, sgp.alpha)
+    beta = mapping_coefficients(sgp)
+    ref = m.mapping()
+    assert beta.shape == ref.shape
+    assert rel_err(beta, ref) < 1e-12
+
+
+@pytest.mark.parametrize("power", [1.0, 2.0])
+def test_coefficient_file_roundtrip(C, oracle, power, tmp_path):
+    from flare_amd.mapping import MappedPotential, write_mapping_coefficients
+    cell, species, pos, cut, calc, s, sgp = setup(C, power)
+    path = os.path.join(tmp_path, "lmp.flare")
+    write_mapping_coefficients(sgp, path, "tester", 0, calc, date="Thu Jan  1 00:00:00 1970")
+    text = open(path).read()
+    cd = sgp.sparse_descriptors[0]
+    m = oracle.Model(1.5, power, cd.descriptors, cd.descriptor_norms, sgp.alpha)
+    ref_text = m.file_text(2, 4, 3, cut, contributor="tester")
+    got, want = text.split("\n"), ref_text.split("\n")
+    assert got[:6] == want[:6]  # header lines byte for byte
+    assert [len(l.split()) for l in got[6:]] == [len(l.split()) for l in want[6:]]
+    gv = np.array(" ".join(got[6:]).split(), dtype=float)
+    wv = np.array(" ".join(want[6:]).split(), dtype=float)
+    assert rel_err(gv, wv) < 1e-12
+    # GPU pair_style flare on the file == oracle pair_flare on the file ==
+    # the sparse-GP prediction.
+    pot = MappedPotential(path)
+    e, F, vir = pot.compute(cell, species, pos)
+    nb = oracle.Neighbors(cell, species, pos, float(cut.max()), "literal")
+    e_ref, F_ref, vir_ref = oracle.pair_flare(text, nb)
+    assert rel_err(e, e_ref) < 1e-10
+    assert rel_err(F, F_ref) < 1e-10
+    assert rel_err(vir, vir_ref) < 1e-10
+    sgp.predict_mean(s)
+    n = len(pos)
+    assert e.sum() == pytest.approx(s.mean_efs[0], rel=1e-10)
+    assert rel_err(F.reshape(-1), s.mean_efs[1:1 + 3 * n]) < 1e-10
+    V = abs(np.linalg.det(cell))
+    assert rel_err(vir / V, s.mean_efs[-6:]) < 1e-10
+
+
+@pytest.mark.parametrize("power", [1, 2])
+def test_dot_product_potential(C, oracle, power, tmp_path):
+    """Unnormalised DotProduct coefficient files (pair_flare.cpp:374-380)."""
+    from flare_amd.mapping import MappedPotential
+    cell, species, pos = random_cell(25, 3, nos=1, scale=7.0)
+    nos, N, lmax = 1, 3, 2
+    d = (N * (N + 1) // 2) * (lmax + 1)
+    bs = d if power == 1 else d * (d + 1) // 2
+    beta = np.random.default_rng(2).normal(size=(1, bs)) * 1e-3
+    path = os.path.join(tmp_path, "dp.flare")
+    with open(path, "w") as f:
+        f.write("DATE: x CONTRIBUTOR: y\n%d DotProduct\nchebyshev\n1 %d %d %d\nquadratic\n4.00 \n"
+                % (power, N, lmax, bs))
+        for k, v in enumerate(beta.reshape(-1)):
+            f.write(("  " if v > 0 else " ") + "%.16e" % v + ("\n" if k % 5 == 4 else ""))
+    pot = MappedPotential(path)
+    assert pot.normalized == 0 and pot.power == power
+    e, F, vir = pot.compute(cell, species, pos)
+    nb = oracle.Neighbors(cell, species, pos, 4.0, "literal")
+    e_ref, F_ref, vir_ref = oracle.pair_flare(open(path).read(), nb)
+    assert rel_err(e, e_ref) < 1e-10
+    assert rel_err(F, F_ref) < 1e-10
+    assert rel_err(vir, vir_ref) < 1e-10
+
+
+def test_bad_files_raise(C, tmp_path):
+    from flare_amd.mapping import MappedPotential
+    path = os.path.join(tmp_path, "bad.flare")
+    with open(path, "w") as f:
+        f.write("DATE: x CONTRIBUTOR: y\n3 NormalizedDotProduct\nchebyshev\n1 2 1 6\nquadratic\n4.0\n")
+    with pytest.raises(ValueError, match="Power should be 1 or 2"):
+        MappedPotential(path)
+    with open(path, "w") as f:
+        f.write("DATE: x CONTRIBUTOR: y\n2 NormalizedDotProduct\nchebyshev\n1 2 1 7\nquadratic\n4.0\n")
+    with pytest.raises(ValueError, match="Beta size"):
+        MappedPotential(path)
+    with pytest.raises(ValueError, match="Cannot open"):
+        MappedPotential(os.path.join(tmp_path, "missing.flare"))
