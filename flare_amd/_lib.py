@@ -37,6 +37,7 @@ EXPORTS = [
     "fm_model_create", "fm_model_destroy", "fm_predict", "fm_last_timings",
     "fm_model_mapping_coefficients", "fm_mapped_create", "fm_mapped_read_file",
     "fm_mapped_info", "fm_mapped_destroy", "fm_mapped_compute",
+    "fm_batch_create", "fm_batch_create_device", "fm_batch_info",
 ]
 
 
@@ -105,6 +106,10 @@ def load():
         L.fm_mapped_info.argtypes = [vp] * 8
         L.fm_mapped_destroy.argtypes = [vp]
         L.fm_mapped_compute.argtypes = [vp, vp, vp, vp, vp]
+        L.fm_batch_create.argtypes = [vp, ip, vp, vp, vp, vp, dp, pvp]
+        L.fm_batch_create_device.argtypes = [vp, ip, vp, vp, vp, vp, dp, pvp]
+        L.fm_batch_info.argtypes = [vp, ip, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                    C.POINTER(C.c_double)]
         for name in EXPORTS:
             getattr(L, name).restype = C.c_char_p if name == "fm_last_error" else C.c_int
         _lib = L

@@ -75,25 +75,47 @@ double single_sweep_cutoff(const double *c) {
   return best;
 }
 
-void set_cell(fm_structure s, const double *cell, double cutoff) {
-  for (int i = 0; i < 9; i++) s->cell[i] = cell[i];
-  double ct[9];
+void cross3(const double *u, const double *v, double *o) {
+  o[0] = u[1] * v[2] - u[2] * v[1];
+  o[1] = u[2] * v[0] - u[0] * v[2];
+  o[2] = u[0] * v[1] - u[1] * v[0];
+}
+
+// The geometry of one structure: structure.cpp:7-43 cell algebra and sweep,
+// plus the cell-list bins (width >= cutoff perpendicular to each face).
+void make_cellargs(const double *cell, double cutoff, fm::CellArgs &ca, double &volume,
+                   double &sscut) {
+  for (int i = 0; i < 9; i++) ca.cell[i] = cell[i];
   for (int r = 0; r < 3; r++)
-    for (int c = 0; c < 3; c++) ct[3 * r + c] = cell[3 * c + r];
-  inverse3(ct, s->cell_t_inv);
+    for (int c = 0; c < 3; c++) ca.ct[3 * r + c] = cell[3 * c + r];
+  inverse3(ca.ct, ca.cti);
   for (int r = 0; r < 3; r++)
     for (int c = 0; c < 3; c++)
-      s->cell_dot[3 * r + c] = (cell[3 * r] * cell[3 * c] + cell[3 * r + 1] * cell[3 * c + 1]) +
-                               cell[3 * r + 2] * cell[3 * c + 2];
-  inverse3(s->cell_dot, s->cell_dot_inv);
-  s->volume = std::fabs(det3(cell));
-  s->single_sweep_cutoff = single_sweep_cutoff(cell);
-  s->cutoff = cutoff;
-  s->sweep = (int)std::ceil(cutoff / s->single_sweep_cutoff);
-  FM_CHECK(std::isfinite(s->volume) && s->volume > 0, FM_ERR_INVALID, "cell is singular");
+      ca.cd[3 * r + c] = (cell[3 * r] * cell[3 * c] + cell[3 * r + 1] * cell[3 * c + 1]) +
+                         cell[3 * r + 2] * cell[3 * c + 2];
+  inverse3(ca.cd, ca.cdi);
+  volume = std::fabs(det3(cell));
+  sscut = single_sweep_cutoff(cell);
+  FM_CHECK(std::isfinite(volume) && volume > 0, FM_ERR_INVALID, "cell is singular");
   FM_CHECK(cutoff > 0 && std::isfinite(cutoff), FM_ERR_INVALID, "cutoff must be positive");
-  FM_CHECK(s->sweep >= 1 && s->sweep <= 64, FM_ERR_CAPACITY,
+  double sw = std::ceil(cutoff / sscut);
+  FM_CHECK(sw >= 1 && sw <= 64, FM_ERR_CAPACITY,
            "cutoff spans too many periodic images of the cell");
+  ca.cutoff = cutoff;
+  ca.sweep = (int)sw;
+  double cr[3][3];
+  cross3(cell + 3, cell + 6, cr[0]);
+  cross3(cell + 6, cell, cr[1]);
+  cross3(cell, cell + 3, cr[2]);
+  double rc = cutoff * (1 + 1e-9) + 1e-9;
+  for (int ax = 0; ax < 3; ax++) {
+    double h = volume / std::sqrt(cr[ax][0] * cr[ax][0] + cr[ax][1] * cr[ax][1] +
+                                  cr[ax][2] * cr[ax][2]);
+    int nb = std::max(1, (int)std::floor(h / rc));
+    nb = std::min(nb, 1024);
+    ca.nb[ax] = nb;
+    ca.K[ax] = (int)std::ceil(rc / (h / nb));
+  }
 }
 
 template <typename T>
@@ -200,44 +222,102 @@ int fm_context_stream(fm_context ctx, void **stream) {
   return guarded([&] { *stream = (void *)ctx->stream; });
 }
 
-static int structure_create(fm_context ctx, const double *cell, int n, const int32_t *species,
+static int structure_create(fm_context ctx, int n_struct, const double *cells,
+                            const int32_t *n_atoms, const int32_t *species,
                             const double *positions, double cutoff, bool device_ptrs,
                             fm_structure *out) {
   return guarded([&] {
-    FM_CHECK(ctx && out && cell, FM_ERR_INVALID, "null argument");
-    FM_CHECK(n >= 0, FM_ERR_INVALID, "negative atom count");
+    FM_CHECK(ctx && out && cells && n_atoms, FM_ERR_INVALID, "null argument");
+    FM_CHECK(n_struct >= 1, FM_ERR_INVALID, "a batch needs at least one structure");
+    long long ntot = 0;
+    for (int k = 0; k < n_struct; k++) {
+      FM_CHECK(n_atoms[k] >= 0, FM_ERR_INVALID, "negative atom count");
+      ntot += n_atoms[k];
+    }
+    FM_CHECK(ntot < (1LL << 31), FM_ERR_CAPACITY, "too many atoms");
+    int n = (int)ntot;
     FM_CHECK(n == 0 || (species && positions), FM_ERR_INVALID, "null species or positions");
     FM_HIP(hipSetDevice(ctx->device));
     auto *s = new fm_structure_s();
     s->ctx = ctx;
     s->n = n;
+    s->n_struct = n_struct;
     try {
-      set_cell(s, cell, cutoff);
       fm::Pool *pool = &ctx->pool;
+      hipStream_t st = ctx->stream;
+      s->atom_off.assign(n_struct + 1, 0);
+      s->cellargs.resize(n_struct);
+      s->volumes.resize(n_struct);
+      s->sweep_cutoffs.resize(n_struct);
+      std::vector<int> ast(n);
+      std::vector<double> vol_inv(n_struct);
+      for (int k = 0; k < n_struct; k++) {
+        make_cellargs(cells + 9 * (size_t)k, cutoff, s->cellargs[k], s->volumes[k],
+                      s->sweep_cutoffs[k]);
+        vol_inv[k] = 1.0 / s->volumes[k];
+        s->atom_off[k + 1] = s->atom_off[k] + n_atoms[k];
+        for (int i = s->atom_off[k]; i < s->atom_off[k + 1]; i++) ast[i] = k;
+      }
+      const fm::CellArgs &c0 = s->cellargs[0];
+      for (int i = 0; i < 9; i++) {
+        s->cell[i] = c0.cell[i];
+        s->cell_t_inv[i] = c0.cti[i];
+        s->cell_dot[i] = c0.cd[i];
+        s->cell_dot_inv[i] = c0.cdi[i];
+      }
+      s->volume = s->volumes[0];
+      s->single_sweep_cutoff = s->sweep_cutoffs[0];
+      s->cutoff = cutoff;
+      s->sweep = c0.sweep;
+      auto up = [&](fm::DevBuf &buf, const void *src, size_t bytes) {
+        buf.alloc(pool, std::max<size_t>(1, bytes));
+        if (bytes) FM_HIP(hipMemcpyAsync(buf.as<void>(), src, bytes, hipMemcpyHostToDevice, st));
+      };
+      up(s->d_cellargs, s->cellargs.data(), s->cellargs.size() * sizeof(fm::CellArgs));
+      up(s->atom_struct, ast.data(), ast.size() * sizeof(int));
+      up(s->d_atom_off, s->atom_off.data(), s->atom_off.size() * sizeof(int));
+      up(s->d_vol_inv, vol_inv.data(), vol_inv.size() * sizeof(double));
+      // mean_efs offsets and reduction chunks per structure.
+      s->mean_off.assign(n_struct + 1, 0);
+      std::vector<int> cb, ce, sco(n_struct + 1, 0);
+      for (int k = 0; k < n_struct; k++) {
+        s->mean_off[k + 1] = s->mean_off[k] + 1 + 3 * (int64_t)n_atoms[k] + 6;
+        for (int a = s->atom_off[k]; a < s->atom_off[k + 1]; a += 256) {
+          cb.push_back(a);
+          ce.push_back(std::min(a + 256, s->atom_off[k + 1]));
+        }
+        sco[k + 1] = (int)cb.size();
+      }
+      s->n_chunks = (int)cb.size();
+      std::vector<long long> mo(s->mean_off.begin(), s->mean_off.end());
+      up(s->d_mean_off, mo.data(), mo.size() * sizeof(long long));
+      up(s->d_chunk_begin, cb.data(), cb.size() * sizeof(int));
+      up(s->d_chunk_end, ce.data(), ce.size() * sizeof(int));
+      up(s->d_struct_chunk_off, sco.data(), sco.size() * sizeof(int));
       s->species.alloc(pool, std::max<size_t>(1, (size_t)n * sizeof(int)));
       s->positions.alloc(pool, std::max<size_t>(1, 3 * (size_t)n * sizeof(double)));
       hipMemcpyKind kind = device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
       if (n > 0) {
-        FM_HIP(hipMemcpyAsync(s->species.as<int>(), species, (size_t)n * sizeof(int), kind,
-                              ctx->stream));
+        FM_HIP(hipMemcpyAsync(s->species.as<int>(), species, (size_t)n * sizeof(int), kind, st));
         FM_HIP(hipMemcpyAsync(s->positions.as<double>(), positions, 3 * (size_t)n * sizeof(double),
-                              kind, ctx->stream));
+                              kind, st));
         fm::DevBuf bad(pool, sizeof(int));
-        FM_HIP(hipMemsetAsync(bad.as<int>(), 0, sizeof(int), ctx->stream));
-        hipLaunchKernelGGL(k_check_positions, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n,
+        FM_HIP(hipMemsetAsync(bad.as<int>(), 0, sizeof(int), st));
+        hipLaunchKernelGGL(k_check_positions, dim3((n + 255) / 256), dim3(256), 0, st, n,
                            s->positions.as<double>(), s->species.as<int>(), bad.as<int>());
         int hb = 0;
-        FM_HIP(hipMemcpyAsync(&hb, bad.as<int>(), sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-        FM_HIP(hipStreamSynchronize(ctx->stream));
+        FM_HIP(hipMemcpyAsync(&hb, bad.as<int>(), sizeof(int), hipMemcpyDeviceToHost, st));
+        FM_HIP(hipStreamSynchronize(st));
         FM_CHECK(!(hb & 1), FM_ERR_INVALID, "positions contain NaN or Inf");
         FM_CHECK(!(hb & 2), FM_ERR_INVALID, "species must be non-negative");
       }
+      FM_HIP(hipStreamSynchronize(st));
       ctx->mark_begin(fm::ST_NEIGH);
       if (n > 0) {
         fm::build_neighbors(s);
       } else {
         s->cumulative.alloc(pool, sizeof(int));
-        FM_HIP(hipMemsetAsync(s->cumulative.as<int>(), 0, sizeof(int), ctx->stream));
+        FM_HIP(hipMemsetAsync(s->cumulative.as<int>(), 0, sizeof(int), st));
       }
       ctx->mark_end(fm::ST_NEIGH);
     } catch (...) {
@@ -250,13 +330,37 @@ static int structure_create(fm_context ctx, const double *cell, int n, const int
 
 int fm_structure_create(fm_context ctx, const double *cell, int n_atoms, const int32_t *species,
                         const double *positions, double cutoff, fm_structure *out) {
-  return structure_create(ctx, cell, n_atoms, species, positions, cutoff, false, out);
+  return structure_create(ctx, 1, cell, &n_atoms, species, positions, cutoff, false, out);
 }
 
 int fm_structure_create_device(fm_context ctx, const double *cell, int n_atoms,
                                const int32_t *species_dev, const double *positions_dev,
                                double cutoff, fm_structure *out) {
-  return structure_create(ctx, cell, n_atoms, species_dev, positions_dev, cutoff, true, out);
+  return structure_create(ctx, 1, cell, &n_atoms, species_dev, positions_dev, cutoff, true, out);
+}
+
+int fm_batch_create(fm_context ctx, int n_structures, const double *cells, const int32_t *n_atoms,
+                    const int32_t *species, const double *positions, double cutoff,
+                    fm_structure *out) {
+  return structure_create(ctx, n_structures, cells, n_atoms, species, positions, cutoff, false,
+                          out);
+}
+
+int fm_batch_create_device(fm_context ctx, int n_structures, const double *cells,
+                           const int32_t *n_atoms, const int32_t *species_dev,
+                           const double *positions_dev, double cutoff, fm_structure *out) {
+  return structure_create(ctx, n_structures, cells, n_atoms, species_dev, positions_dev, cutoff,
+                          true, out);
+}
+
+int fm_batch_info(fm_structure s, int k, int *atom_offset, int *sweep, double *volume) {
+  return guarded([&] {
+    FM_CHECK(s, FM_ERR_INVALID, "null structure");
+    FM_CHECK(k >= 0 && k < s->n_struct, FM_ERR_INVALID, "structure index out of range");
+    if (atom_offset) *atom_offset = s->atom_off[k];
+    if (sweep) *sweep = s->cellargs[k].sweep;
+    if (volume) *volume = s->volumes[k];
+  });
 }
 
 int fm_structure_destroy(fm_structure s) {
@@ -350,6 +454,25 @@ int fm_structure_compute_b2(fm_structure s, const fm_b2_settings *st_in, int for
     s->nn.alloc(pool, std::max<size_t>(1, (size_t)s->n * sizeof(double)));
     if (s->n > 0) fm::launch_b2(s);
     ctx->mark_end(fm::ST_DESC);
+    // Per-structure species-sorted position of every global row.
+    s->var_pos.assign(s->n, 0);
+    if (s->n > 0) {
+      std::vector<int> sp(s->n);
+      FM_HIP(hipMemcpyAsync(sp.data(), s->species.as<int>(), s->n * sizeof(int),
+                            hipMemcpyDeviceToHost, stream));
+      FM_HIP(hipStreamSynchronize(stream));
+      std::vector<int> grank(s->nos, 0);
+      for (int k = 0; k < s->n_struct; k++) {
+        std::vector<int> cnt(s->nos, 0), off(s->nos, 0), rk(s->nos, 0);
+        for (int i = s->atom_off[k]; i < s->atom_off[k + 1]; i++) cnt[sp[i]]++;
+        for (int t = 1; t < s->nos; t++) off[t] = off[t - 1] + cnt[t - 1];
+        for (int i = s->atom_off[k]; i < s->atom_off[k + 1]; i++) {
+          int t = sp[i];
+          int row = s->type_off[t] + grank[t]++;
+          s->var_pos[row] = s->atom_off[k] + off[t] + rk[t]++;
+        }
+      }
+    }
     // Neighbour counts by species (b2.cpp:106-112).
     s->nb_by_type.assign(s->nos, 0);
     if (s->n > 0) {
@@ -383,6 +506,7 @@ int fm_structure_get_b2(fm_structure s, int t, double *descriptors, double *norm
                         double *neighbor_coordinates) {
   return guarded([&] {
     FM_CHECK(s && s->has_b2, FM_ERR_STATE, "descriptors have not been computed");
+    FM_CHECK(s->n_struct == 1, FM_ERR_INVALID, "descriptor export is per single structure");
     FM_CHECK(t >= 0 && t < s->nos, FM_ERR_INVALID, "species out of range");
     hipStream_t st = s->ctx->stream;
     int ns = s->n_by_type[t], ro = s->type_off[t], nd = s->nd;

@@ -298,7 +298,7 @@ static B2Args make_args(fm_structure s) {
 }
 
 static void upload_knorm(hipStream_t st, int lmax) {
-  static double host[kMaxH];
+  static thread_local double host[kMaxH];
   for (int l = 0; l <= lmax; l++)
     for (int m = -l; m <= l; m++) {
       int mm = m < 0 ? -m : m;

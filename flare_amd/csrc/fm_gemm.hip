@@ -31,13 +31,19 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
 // ---------------------------------------------------------------------------
-// Epilogues. The wave's tiles use the MFMA output layout: lane l holds rows
-// 4 (l / 16) + i (i < 4) and column l % 16 of each 16 x 16 tile.
+// Epilogues. Lane l holds column l % 16 of each 16 x 16 output tile; the
+// row of its value i (i < 4) is 4 (l / 16) + i for the FP32-output MFMAs and
+// l / 16 + 4 i for v_mfma_f64_16x16x4f64.
 // ---------------------------------------------------------------------------
 template <typename Acc, int TM, int TN>
 struct TileOut {
   Acc v[TM][TN][4];
 };
+
+template <typename Acc>
+__device__ __forceinline__ int tile_row(int lane, int i) {
+  return sizeof(Acc) == 8 ? (lane >> 4) + 4 * i : 4 * (lane >> 4) + i;
+}
 
 __device__ __forceinline__ double fpow(double x, double p, int pi) {
   if (pi == 2) return x * x;
@@ -65,7 +71,7 @@ __device__ void ep_kernel(const EpArgs &e, const TileOut<Acc, TM, TN> &t, int ro
   for (int ti = 0; ti < TM; ti++)
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-      int r = row0 + wr * 16 * TM + ti * 16 + 4 * (lane >> 4) + i;
+      int r = row0 + wr * 16 * TM + ti * 16 + tile_row<Acc>(lane, i);
       if (r >= M) continue;
       double nrm = e.row_norm[r];
       bool empty = nrm < kEmptyThresh;
@@ -100,7 +106,7 @@ __device__ void ep_kernel(const EpArgs &e, const TileOut<Acc, TM, TN> &t, int ro
     for (int ti = 0; ti < TM; ti++)
 #pragma unroll
       for (int i = 0; i < 4; i++)
-        red[wc * BMc + wr * 16 * TM + ti * 16 + 4 * (lane >> 4) + i] = rs[ti][i];
+        red[wc * BMc + wr * 16 * TM + ti * 16 + tile_row<Acc>(lane, i)] = rs[ti][i];
   }
   __syncthreads();
   for (int q = threadIdx.x; q < BMc; q += blockDim.x) {
@@ -116,7 +122,7 @@ __device__ void ep_store(const EpArgs &e, const TileOut<Acc, TM, TN> &t, int row
   for (int ti = 0; ti < TM; ti++)
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-      int r = row0 + wr * 16 * TM + ti * 16 + 4 * (lane >> 4) + i;
+      int r = row0 + wr * 16 * TM + ti * 16 + tile_row<Acc>(lane, i);
       if (r >= M) continue;
 #pragma unroll
       for (int tj = 0; tj < TN; tj++) {
@@ -139,7 +145,7 @@ __device__ void ep_sumsq(const EpArgs &e, const TileOut<Acc, TM, TN> &t, int row
     for (int ti = 0; ti < TM; ti++)
 #pragma unroll
       for (int i = 0; i < 4; i++) {
-        int r = row0 + wr * 16 * TM + ti * 16 + 4 * (lane >> 4) + i;
+        int r = row0 + wr * 16 * TM + ti * 16 + tile_row<Acc>(lane, i);
         double v = r < M ? (double)t.v[ti][tj][i] : 0.0;
         cs[tj] += v * v;
       }

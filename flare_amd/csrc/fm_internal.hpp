@@ -91,6 +91,17 @@ class DevBuf {
   Pool *pool_ = nullptr;
 };
 
+// Geometry of one periodic structure for the neighbour kernels: cell (rows
+// are lattice vectors), its transpose, (cell^T)^-1, cell cell^T and its
+// inverse, the cutoff and sweep, and the cell-list bins per axis (nb) with
+// the stencil reach (K).
+struct CellArgs {
+  double cell[9], ct[9], cti[9], cd[9], cdi[9];
+  double cutoff;
+  int sweep;
+  int nb[3], K[3];
+};
+
 enum Stage { ST_NEIGH = 0, ST_DESC, ST_SGEMM, ST_GGEMM, ST_VAR, ST_ADJ, ST_COUNT };
 
 }  // namespace fm
@@ -109,6 +120,22 @@ struct fm_context_s {
 struct fm_structure_s {
   fm_context ctx = nullptr;
   int n = 0;
+  // A structure object holds a batch of n_struct periodic structures; atoms
+  // of structure k are [atom_off[k], atom_off[k+1]). Single structures have
+  // n_struct = 1 and the fields below (cell ... sweep) describe structure 0.
+  int n_struct = 1;
+  std::vector<int> atom_off;
+  std::vector<fm::CellArgs> cellargs;
+  std::vector<double> volumes, sweep_cutoffs;
+  fm::DevBuf d_cellargs, atom_struct, d_bin_off, d_atom_off, d_mean_off, d_vol_inv;
+  // Output offsets of each structure in mean_efs (1 + 3 N_k + 6 per structure)
+  // and the reduction chunks (<= 256 atoms, never crossing a structure).
+  std::vector<int64_t> mean_off;
+  int n_chunks = 0;
+  fm::DevBuf d_chunk_begin, d_chunk_end, d_struct_chunk_off;
+  // Position of each species-sorted row in the per-structure species-sorted
+  // order of the local variances (identity for a single structure).
+  std::vector<int> var_pos;
   double cell[9], cell_t_inv[9], cell_dot[9], cell_dot_inv[9];
   double volume = 0, single_sweep_cutoff = 0, cutoff = 0;
   int sweep = 0;

@@ -118,40 +118,46 @@ __global__ void k_mapped_energy(int ns, int ro, int d, const double *B, const do
   if (lane == 0) E_row[row] = e;
 }
 
-__global__ void k_mapped_gather(int n, const int *bond_off, const int *in_off, const int *in_bond,
-                                const double *fbond, const int *row_of_atom, const double *E_row,
-                                const double *vir, double *F, double *Eatom, double *blk_part) {
+// Forces (atom order), per-atom energies, and per-chunk virial partials;
+// one block per chunk of <= 256 atoms of one structure.
+__global__ void k_mapped_gather(const int *chunk_begin, const int *chunk_end, const int *bond_off,
+                                const int *in_off, const int *in_bond, const double *fbond,
+                                const int *row_of_atom, const double *E_row, const double *vir,
+                                double *F, double *Eatom, double *blk_part) {
   __shared__ double sh[6][256];
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int c = blockIdx.x;
+  int i = chunk_begin[c] + threadIdx.x;
   double v[6] = {0, 0, 0, 0, 0, 0};
-  if (i < n) {
+  if (i < chunk_end[c]) {
     double f[3] = {0, 0, 0};
     for (int b = bond_off[i]; b < bond_off[i + 1]; b++)
-      for (int c = 0; c < 3; c++) f[c] += fbond[3 * (size_t)b + c];
+      for (int q = 0; q < 3; q++) f[q] += fbond[3 * (size_t)b + q];
     for (int a = in_off[i]; a < in_off[i + 1]; a++) {
       int b = in_bond[a];
-      for (int c = 0; c < 3; c++) f[c] -= fbond[3 * (size_t)b + c];
+      for (int q = 0; q < 3; q++) f[q] -= fbond[3 * (size_t)b + q];
     }
-    for (int c = 0; c < 3; c++) F[3 * (size_t)i + c] = f[c];
+    for (int q = 0; q < 3; q++) F[3 * (size_t)i + q] = f[q];
     Eatom[i] = E_row[row_of_atom[i]];
-    for (int k = 0; k < 6; k++) v[k] = vir[6 * (size_t)i + k];
+    for (int q = 0; q < 6; q++) v[q] = vir[6 * (size_t)i + q];
   }
-  for (int k = 0; k < 6; k++) sh[k][threadIdx.x] = v[k];
+  for (int q = 0; q < 6; q++) sh[q][threadIdx.x] = v[q];
   __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (threadIdx.x < s)
-      for (int k = 0; k < 6; k++) sh[k][threadIdx.x] += sh[k][threadIdx.x + s];
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st)
+      for (int q = 0; q < 6; q++) sh[q][threadIdx.x] += sh[q][threadIdx.x + st];
     __syncthreads();
   }
-  if (threadIdx.x < 6) blk_part[(size_t)blockIdx.x * 6 + threadIdx.x] = sh[threadIdx.x][0];
+  if (threadIdx.x < 6) blk_part[(size_t)c * 6 + threadIdx.x] = sh[threadIdx.x][0];
 }
 
-__global__ void k_mapped_sum(int nblk, const double *blk_part, double *vir) {
-  int k = threadIdx.x;
-  if (k >= 6) return;
-  double acc = 0;
-  for (int b = 0; b < nblk; b++) acc += blk_part[(size_t)b * 6 + k];
-  vir[k] = acc;
+__global__ void k_mapped_sum(int n_struct, const int *struct_chunk_off, const double *blk_part,
+                             double *vir) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n_struct) return;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int c = struct_chunk_off[k]; c < struct_chunk_off[k + 1]; c++)
+    for (int q = 0; q < 6; q++) acc[q] += blk_part[(size_t)c * 6 + q];
+  for (int q = 0; q < 6; q++) vir[6 * (size_t)k + q] = acc[q];
 }
 
 void mapped_compute(fm_mapped mp, fm_structure s, double *atom_energy, double *forces,
@@ -194,18 +200,19 @@ void mapped_compute(fm_mapped mp, fm_structure s, double *atom_energy, double *f
   DevBuf vir(pool, std::max<size_t>(1, (size_t)6 * n * sizeof(double)));
   DevBuf F(pool, std::max<size_t>(1, (size_t)3 * n * sizeof(double)));
   DevBuf Ea(pool, std::max<size_t>(1, (size_t)n * sizeof(double)));
-  DevBuf vsum(pool, 6 * sizeof(double));
+  DevBuf vsum(pool, 6 * (size_t)s->n_struct * sizeof(double));
+  FM_HIP(hipMemsetAsync(vsum.as<double>(), 0, 6 * (size_t)s->n_struct * sizeof(double), st));
   if (n > 0) {
     launch_adjoint(s, G.as<void>(), d, fbond.as<void>(), vir.as<double>(),
                    mp->normalized ? 1 : 0, -1.0, 1e-8);
-    int tpb = 256, nblk = (n + tpb - 1) / tpb;
-    DevBuf blk(pool, (size_t)nblk * 6 * sizeof(double));
-    hipLaunchKernelGGL(k_mapped_gather, dim3(nblk), dim3(tpb), 0, st, n, s->bond_off.as<int>(),
+    DevBuf blk(pool, (size_t)s->n_chunks * 6 * sizeof(double));
+    hipLaunchKernelGGL(k_mapped_gather, dim3(s->n_chunks), dim3(256), 0, st,
+                       s->d_chunk_begin.as<int>(), s->d_chunk_end.as<int>(), s->bond_off.as<int>(),
                        s->in_off.as<int>(), s->in_bond.as<int>(), fbond.as<double>(),
                        s->row_of_atom.as<int>(), E_row.as<double>(), vir.as<double>(),
                        F.as<double>(), Ea.as<double>(), blk.as<double>());
-    hipLaunchKernelGGL(k_mapped_sum, dim3(1), dim3(64), 0, st, nblk, blk.as<double>(),
-                       vsum.as<double>());
+    hipLaunchKernelGGL(k_mapped_sum, dim3((s->n_struct + 63) / 64), dim3(64), 0, st, s->n_struct,
+                       s->d_struct_chunk_off.as<int>(), blk.as<double>(), vsum.as<double>());
     FM_HIP(hipGetLastError());
     if (atom_energy)
       FM_HIP(hipMemcpyAsync(atom_energy, Ea.as<double>(), n * sizeof(double),
@@ -213,12 +220,10 @@ void mapped_compute(fm_mapped mp, fm_structure s, double *atom_energy, double *f
     if (forces)
       FM_HIP(hipMemcpyAsync(forces, F.as<double>(), 3 * (size_t)n * sizeof(double),
                             hipMemcpyDeviceToHost, st));
-    if (virial)
-      FM_HIP(hipMemcpyAsync(virial, vsum.as<double>(), 6 * sizeof(double), hipMemcpyDeviceToHost,
-                            st));
-  } else if (virial) {
-    for (int k = 0; k < 6; k++) virial[k] = 0;
   }
+  if (virial)
+    FM_HIP(hipMemcpyAsync(virial, vsum.as<double>(), 6 * (size_t)s->n_struct * sizeof(double),
+                          hipMemcpyDeviceToHost, st));
   FM_HIP(hipStreamSynchronize(st));
 }
 

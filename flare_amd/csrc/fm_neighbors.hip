@@ -107,23 +107,20 @@ int scan_exclusive(fm_context ctx, const int *in, int *out, int n) {
 }
 
 // ---------------------------------------------------------------------------
-// Geometry constants passed by value
+// Per-structure geometry (CellArgs, built on the host by fm_capi.hip). A
+// structure object may hold a batch of structures: atoms of structure k are
+// the contiguous range [atom_off[k], atom_off[k+1]) and only see each other.
 // ---------------------------------------------------------------------------
-struct CellArgs {
-  double cell[9], ct[9], cti[9], cd[9], cdi[9];
-  double cutoff;
-  int sweep;
-  int nb[3], K[3];
-};
-
 __device__ __forceinline__ void row3(const double *a, const double *B, double *o) {
   for (int k = 0; k < 3; k++) o[k] = (a[0] * B[k] + a[1] * B[3 + k]) + a[2] * B[6 + k];
 }
 
-// structure.cpp:120-133 per atom, plus the bin of the wrapped position.
-__global__ void k_wrap(int n, const double *pos, CellArgs ca, double *wrapped, int *bin) {
+// structure.cpp:120-133 per atom, plus the global bin of the wrapped position.
+__global__ void k_wrap(int n, const double *pos, const CellArgs *cas, const int *atom_struct,
+                       const int *bin_off, double *wrapped, int *bin) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const CellArgs &ca = cas[atom_struct[i]];
   double p[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
   double r1[3], fr[3], w1[3], w[3];
   row3(p, ca.ct, r1);
@@ -140,7 +137,7 @@ __global__ void k_wrap(int n, const double *pos, CellArgs ca, double *wrapped, i
     int b = (int)floor(fr[k] * ca.nb[k]);
     q[k] = min(max(b, 0), ca.nb[k] - 1);
   }
-  bin[i] = (q[0] * ca.nb[1] + q[1]) * ca.nb[2] + q[2];
+  bin[i] = bin_off[atom_struct[i]] + (q[0] * ca.nb[1] + q[1]) * ca.nb[2] + q[2];
 }
 
 __global__ void k_bin_count(int n, const int *bin, int *bcount) {
@@ -175,6 +172,23 @@ __device__ __forceinline__ double image_dist(const double *wi, const double *wj,
   return sqrt(im[0] * im[0] + im[1] * im[1] + im[2] * im[2]);
 }
 
+__device__ __forceinline__ long long nbr_key(int j, int s1, int s2, int s3, int sw) {
+  int span = 2 * sw + 1;
+  long long key = ((long long)j * span + (s1 + sw)) * span;
+  return (key + (s2 + sw)) * span + (s3 + sw);
+}
+
+__device__ __forceinline__ void nbr_unkey(long long key, int sw, int &j, int &s1, int &s2,
+                                          int &s3) {
+  int span = 2 * sw + 1;
+  s3 = (int)(key % span) - sw;
+  key /= span;
+  s2 = (int)(key % span) - sw;
+  key /= span;
+  s1 = (int)(key % span) - sw;
+  j = (int)(key / span);
+}
+
 // One wavefront per atom. kFill = false counts the neighbours; kFill = true
 // collects the keys in LDS, sorts them and writes the list.
 constexpr int kNbrCap = 2048;  // keys per atom held in LDS
@@ -182,17 +196,20 @@ constexpr int kNbrWaves = 4;
 
 template <bool kFill>
 __global__ __launch_bounds__(64 * kNbrWaves) void k_neighbors(
-    int n, const double *wrapped, const int *bin_of, const int *bstart,
-    const int *order, const int *species, CellArgs ca, int *count, const int *cum,
-    int *sidx, int *nspec, double *rel, int *max_count) {
+    int n, const double *wrapped, const int *bin_of, const int *bstart, const int *order,
+    const int *species, const CellArgs *cas, const int *atom_struct, const int *bin_off,
+    int *count, const int *cum, int *sidx, int *nspec, double *rel, int *max_count) {
   __shared__ long long keys[kNbrWaves][kFill ? kNbrCap : 1];
   int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   int i = blockIdx.x * kNbrWaves + w;
   if (i >= n) return;
+  int st = atom_struct[i];
+  const CellArgs &ca = cas[st];
+  int boff = bin_off[st];
   double wi[3] = {wrapped[3 * i], wrapped[3 * i + 1], wrapped[3 * i + 2]};
-  int b = bin_of[i];
+  int b = bin_of[i] - boff;
   int bx = b / (ca.nb[1] * ca.nb[2]), by = (b / ca.nb[2]) % ca.nb[1], bz = b % ca.nb[2];
-  int sw = ca.sweep, span = 2 * sw + 1;
+  int sw = ca.sweep;
   int found = 0;
   for (int o0 = -ca.K[0]; o0 <= ca.K[0]; o0++) {
     int t0 = bx + o0, s1 = floor_div(t0, ca.nb[0]), b0 = t0 - s1 * ca.nb[0];
@@ -203,7 +220,7 @@ __global__ __launch_bounds__(64 * kNbrWaves) void k_neighbors(
       for (int o2 = -ca.K[2]; o2 <= ca.K[2]; o2++) {
         int t2 = bz + o2, s3 = floor_div(t2, ca.nb[2]), b2 = t2 - s3 * ca.nb[2];
         if (s3 < -sw || s3 > sw) continue;
-        int q = (b0 * ca.nb[1] + b1) * ca.nb[2] + b2;
+        int q = boff + (b0 * ca.nb[1] + b1) * ca.nb[2] + b2;
         int p0 = bstart[q], p1 = bstart[q + 1];
         for (int p = p0; p < p1; p += 64) {
           bool ok = false;
@@ -219,11 +236,7 @@ __global__ __launch_bounds__(64 * kNbrWaves) void k_neighbors(
           if (kFill && ok) {
             int pre = __popcll(mask & ((1ull << lane) - 1ull));
             int slot = found + pre;
-            if (slot < kNbrCap) {
-              long long key = ((long long)j * span + (s1 + sw)) * span;
-              key = (key + (s2 + sw)) * span + (s3 + sw);
-              keys[w][slot] = key;
-            }
+            if (slot < kNbrCap) keys[w][slot] = nbr_key(j, s1, s2, s3, sw);
           }
           found += __popcll(mask);
         }
@@ -241,8 +254,9 @@ __global__ __launch_bounds__(64 * kNbrWaves) void k_neighbors(
   int np2 = 1;
   while (np2 < found) np2 <<= 1;
   for (int t = found + lane; t < np2; t += 64) keys[w][t] = 0x7fffffffffffffffLL;
-  __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   for (int k = 2; k <= np2; k <<= 1) {
     for (int jj = k >> 1; jj > 0; jj >>= 1) {
       for (int t = lane; t < np2; t += 64) {
@@ -263,13 +277,8 @@ __global__ __launch_bounds__(64 * kNbrWaves) void k_neighbors(
   }
   int base = cum[i];
   for (int t = lane; t < found; t += 64) {
-    long long key = keys[w][t];
-    int s3 = (int)(key % span) - sw;
-    key /= span;
-    int s2 = (int)(key % span) - sw;
-    key /= span;
-    int s1 = (int)(key % span) - sw;
-    int j = (int)(key / span);
+    int j, s1, s2, s3;
+    nbr_unkey(keys[w][t], sw, j, s1, s2, s3);
     double wj[3] = {wrapped[3 * j], wrapped[3 * j + 1], wrapped[3 * j + 2]};
     double im[3];
     double dist = image_dist(wi, wj, ca, s1, s2, s3, im);
@@ -287,14 +296,18 @@ __global__ __launch_bounds__(64 * kNbrWaves) void k_neighbors(
 // large sweep): one thread per atom, keys sorted in place in global memory.
 __global__ void k_neighbors_big(int n, const double *wrapped, const int *bin_of,
                                 const int *bstart, const int *order, const int *species,
-                                CellArgs ca, const int *cum, long long *scratch, int *sidx,
-                                int *nspec, double *rel) {
+                                const CellArgs *cas, const int *atom_struct, const int *bin_off,
+                                const int *cum, long long *scratch, int *sidx, int *nspec,
+                                double *rel) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  int st = atom_struct[i];
+  const CellArgs &ca = cas[st];
+  int boff = bin_off[st];
   double wi[3] = {wrapped[3 * i], wrapped[3 * i + 1], wrapped[3 * i + 2]};
-  int b = bin_of[i];
+  int b = bin_of[i] - boff;
   int bx = b / (ca.nb[1] * ca.nb[2]), by = (b / ca.nb[2]) % ca.nb[1], bz = b % ca.nb[2];
-  int sw = ca.sweep, span = 2 * sw + 1;
+  int sw = ca.sweep;
   long long *keys = scratch + cum[i];
   int cnt = cum[i + 1] - cum[i];
   int found = 0;
@@ -307,17 +320,14 @@ __global__ void k_neighbors_big(int n, const double *wrapped, const int *bin_of,
       for (int o2 = -ca.K[2]; o2 <= ca.K[2]; o2++) {
         int t2 = bz + o2, s3 = floor_div(t2, ca.nb[2]), b2 = t2 - s3 * ca.nb[2];
         if (s3 < -sw || s3 > sw) continue;
-        int q = (b0 * ca.nb[1] + b1) * ca.nb[2] + b2;
+        int q = boff + (b0 * ca.nb[1] + b1) * ca.nb[2] + b2;
         for (int p = bstart[q]; p < bstart[q + 1]; p++) {
           int j = order[p];
           double wj[3] = {wrapped[3 * j], wrapped[3 * j + 1], wrapped[3 * j + 2]};
           double im[3];
           double dist = image_dist(wi, wj, ca, s1, s2, s3, im);
-          if ((dist < ca.cutoff) && (dist != 0) && found < cnt) {
-            long long key = ((long long)j * span + (s1 + sw)) * span;
-            key = (key + (s2 + sw)) * span + (s3 + sw);
-            keys[found++] = key;
-          }
+          if ((dist < ca.cutoff) && (dist != 0) && found < cnt)
+            keys[found++] = nbr_key(j, s1, s2, s3, sw);
         }
       }
     }
@@ -332,13 +342,8 @@ __global__ void k_neighbors_big(int n, const double *wrapped, const int *bin_of,
     keys[c + 1] = v;
   }
   for (int t = 0; t < found; t++) {
-    long long key = keys[t];
-    int s3 = (int)(key % span) - sw;
-    key /= span;
-    int s2 = (int)(key % span) - sw;
-    key /= span;
-    int s1 = (int)(key % span) - sw;
-    int j = (int)(key / span);
+    int j, s1, s2, s3;
+    nbr_unkey(keys[t], sw, j, s1, s2, s3);
     double wj[3] = {wrapped[3 * j], wrapped[3 * j + 1], wrapped[3 * j + 2]};
     double im[3];
     double dist = image_dist(wi, wj, ca, s1, s2, s3, im);
@@ -352,44 +357,26 @@ __global__ void k_neighbors_big(int n, const double *wrapped, const int *bin_of,
   }
 }
 
-static void cross3(const double *u, const double *v, double *o) {
-  o[0] = u[1] * v[2] - u[2] * v[1];
-  o[1] = u[2] * v[0] - u[0] * v[2];
-  o[2] = u[0] * v[1] - u[1] * v[0];
-}
-
 void build_neighbors(fm_structure s) {
   fm_context ctx = s->ctx;
   hipStream_t st = ctx->stream;
   int n = s->n;
-  CellArgs ca;
-  for (int k = 0; k < 9; k++) {
-    ca.cell[k] = s->cell[k];
-    ca.cti[k] = s->cell_t_inv[k];
-    ca.cd[k] = s->cell_dot[k];
-    ca.cdi[k] = s->cell_dot_inv[k];
-  }
-  for (int r = 0; r < 3; r++)
-    for (int c = 0; c < 3; c++) ca.ct[3 * r + c] = s->cell[3 * c + r];
-  ca.cutoff = s->cutoff;
-  ca.sweep = s->sweep;
-  double cr[3][3];
-  cross3(s->cell + 3, s->cell + 6, cr[0]);
-  cross3(s->cell + 6, s->cell, cr[1]);
-  cross3(s->cell, s->cell + 3, cr[2]);
-  double rc = s->cutoff * (1 + 1e-9) + 1e-9;
-  long long ncell = 1;
-  for (int ax = 0; ax < 3; ax++) {
-    double h = s->volume / std::sqrt(cr[ax][0] * cr[ax][0] + cr[ax][1] * cr[ax][1] +
-                                     cr[ax][2] * cr[ax][2]);
-    int nb = std::max(1, (int)std::floor(h / rc));
-    nb = std::min(nb, 1024);
-    ca.nb[ax] = nb;
-    ca.K[ax] = (int)std::ceil(rc / (h / nb));
-    ncell *= nb;
-  }
-  FM_CHECK(ncell < (1LL << 30), FM_ERR_CAPACITY, "cell list too large");
   Pool *pool = &ctx->pool;
+  // Global bin offsets of the structures of the batch.
+  std::vector<int> bin_off(s->n_struct + 1, 0);
+  for (int k = 0; k < s->n_struct; k++) {
+    const CellArgs &ca = s->cellargs[k];
+    long long nb = (long long)ca.nb[0] * ca.nb[1] * ca.nb[2];
+    FM_CHECK(bin_off[k] + nb < (1LL << 30), FM_ERR_CAPACITY, "cell list too large");
+    bin_off[k + 1] = bin_off[k] + (int)nb;
+  }
+  int ncell = bin_off[s->n_struct];
+  s->d_bin_off.alloc(pool, bin_off.size() * sizeof(int));
+  FM_HIP(hipMemcpyAsync(s->d_bin_off.as<int>(), bin_off.data(), bin_off.size() * sizeof(int),
+                        hipMemcpyHostToDevice, st));
+  const CellArgs *cas = s->d_cellargs.as<CellArgs>();
+  const int *ast = s->atom_struct.as<int>();
+  const int *boffp = s->d_bin_off.as<int>();
   s->wrapped.alloc(pool, 3 * (size_t)n * sizeof(double));
   s->count.alloc(pool, (size_t)n * sizeof(int));
   s->cumulative.alloc(pool, ((size_t)n + 1) * sizeof(int));
@@ -400,40 +387,40 @@ void build_neighbors(fm_structure s) {
   DevBuf order(pool, (size_t)n * sizeof(int));
   DevBuf maxc(pool, sizeof(int));
   int tpb = 256, nblk = (n + tpb - 1) / tpb;
-  hipLaunchKernelGGL(k_wrap, dim3(nblk), dim3(tpb), 0, st, n, s->positions.as<double>(), ca,
-                     s->wrapped.as<double>(), bin.as<int>());
-  FM_HIP(hipMemsetAsync(bcount.as<int>(), 0, ncell * sizeof(int), st));
-  FM_HIP(hipMemsetAsync(cursor.as<int>(), 0, ncell * sizeof(int), st));
+  hipLaunchKernelGGL(k_wrap, dim3(nblk), dim3(tpb), 0, st, n, s->positions.as<double>(), cas, ast,
+                     boffp, s->wrapped.as<double>(), bin.as<int>());
+  FM_HIP(hipMemsetAsync(bcount.as<int>(), 0, (size_t)ncell * sizeof(int), st));
+  FM_HIP(hipMemsetAsync(cursor.as<int>(), 0, (size_t)ncell * sizeof(int), st));
   FM_HIP(hipMemsetAsync(maxc.as<int>(), 0, sizeof(int), st));
   hipLaunchKernelGGL(k_bin_count, dim3(nblk), dim3(tpb), 0, st, n, bin.as<int>(), bcount.as<int>());
-  scan_exclusive(ctx, bcount.as<int>(), bstart.as<int>(), (int)ncell);
+  scan_exclusive(ctx, bcount.as<int>(), bstart.as<int>(), ncell);
   hipLaunchKernelGGL(k_bin_fill, dim3(nblk), dim3(tpb), 0, st, n, bin.as<int>(), bstart.as<int>(),
                      cursor.as<int>(), order.as<int>());
   int wblk = (n + kNbrWaves - 1) / kNbrWaves;
   hipLaunchKernelGGL(k_neighbors<false>, dim3(wblk), dim3(64 * kNbrWaves), 0, st, n,
                      s->wrapped.as<double>(), bin.as<int>(), bstart.as<int>(), order.as<int>(),
-                     s->species.as<int>(), ca, s->count.as<int>(), nullptr, nullptr, nullptr,
-                     nullptr, maxc.as<int>());
+                     s->species.as<int>(), cas, ast, boffp, s->count.as<int>(), nullptr, nullptr,
+                     nullptr, nullptr, maxc.as<int>());
   FM_HIP(hipGetLastError());
   int total = scan_exclusive(ctx, s->count.as<int>(), s->cumulative.as<int>(), n);
   int max_count = 0;
   FM_HIP(hipMemcpyAsync(&max_count, maxc.as<int>(), sizeof(int), hipMemcpyDeviceToHost, st));
   FM_HIP(hipStreamSynchronize(st));
   s->n_nb = total;
-  s->sidx.alloc(pool, (size_t)total * sizeof(int));
-  s->nspecies.alloc(pool, (size_t)total * sizeof(int));
-  s->rel.alloc(pool, 4 * (size_t)total * sizeof(double));
+  s->sidx.alloc(pool, std::max<size_t>(1, (size_t)total * sizeof(int)));
+  s->nspecies.alloc(pool, std::max<size_t>(1, (size_t)total * sizeof(int)));
+  s->rel.alloc(pool, std::max<size_t>(1, 4 * (size_t)total * sizeof(double)));
   if (max_count <= kNbrCap) {
     hipLaunchKernelGGL(k_neighbors<true>, dim3(wblk), dim3(64 * kNbrWaves), 0, st, n,
                        s->wrapped.as<double>(), bin.as<int>(), bstart.as<int>(), order.as<int>(),
-                       s->species.as<int>(), ca, nullptr, s->cumulative.as<int>(),
+                       s->species.as<int>(), cas, ast, boffp, nullptr, s->cumulative.as<int>(),
                        s->sidx.as<int>(), s->nspecies.as<int>(), s->rel.as<double>(), nullptr);
   } else {
     DevBuf scratch(pool, (size_t)total * sizeof(long long));
     hipLaunchKernelGGL(k_neighbors_big, dim3(nblk), dim3(tpb), 0, st, n, s->wrapped.as<double>(),
-                       bin.as<int>(), bstart.as<int>(), order.as<int>(), s->species.as<int>(), ca,
-                       s->cumulative.as<int>(), scratch.as<long long>(), s->sidx.as<int>(),
-                       s->nspecies.as<int>(), s->rel.as<double>());
+                       bin.as<int>(), bstart.as<int>(), order.as<int>(), s->species.as<int>(), cas,
+                       ast, boffp, s->cumulative.as<int>(), scratch.as<long long>(),
+                       s->sidx.as<int>(), s->nspecies.as<int>(), s->rel.as<double>());
     FM_HIP(hipStreamSynchronize(st));
   }
   FM_HIP(hipGetLastError());

@@ -44,49 +44,58 @@ __global__ void k_species_final(int n_s, int ro, const double *Epart, int n_etil
 
 // Forces: F_i = sum_{bonds b of i} f_b - sum_{bonds b with neighbour i} f_b
 // (normalized_dot_product.cpp:197-199 after K^T alpha), each list in fixed
-// order. Also per-block partial sums of the energies and virials.
+// order, written at the atom's slot of its structure's mean_efs. One block
+// per chunk of <= 256 atoms of one structure; the block also reduces the
+// chunk's energies and virials.
 template <typename TF>
-__global__ void k_gather(int n, const int *bond_off, const int *in_off, const int *in_bond,
-                         const TF *fbond, const int *row_of_atom, const double *E_row,
-                         const double *vir, double *out_F, double *local_energy,
-                         double *blk_part) {
+__global__ void k_gather(const int *chunk_begin, const int *chunk_end, const int *atom_struct,
+                         const int *atom_off, const long long *mean_off, const int *bond_off,
+                         const int *in_off, const int *in_bond, const TF *fbond,
+                         const int *row_of_atom, const double *E_row, const double *vir,
+                         double *out, double *blk_part) {
   __shared__ double sh[7][256];
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int c = blockIdx.x;
+  int i = chunk_begin[c] + threadIdx.x;
   double e = 0, v[6] = {0, 0, 0, 0, 0, 0};
-  if (i < n) {
+  if (i < chunk_end[c]) {
     double f[3] = {0, 0, 0};
     for (int b = bond_off[i]; b < bond_off[i + 1]; b++)
-      for (int c = 0; c < 3; c++) f[c] += (double)fbond[3 * (size_t)b + c];
+      for (int q = 0; q < 3; q++) f[q] += (double)fbond[3 * (size_t)b + q];
     for (int a = in_off[i]; a < in_off[i + 1]; a++) {
       int b = in_bond[a];
-      for (int c = 0; c < 3; c++) f[c] -= (double)fbond[3 * (size_t)b + c];
+      for (int q = 0; q < 3; q++) f[q] -= (double)fbond[3 * (size_t)b + q];
     }
-    for (int c = 0; c < 3; c++) out_F[3 * (size_t)i + c] = f[c];
+    int k = atom_struct[i];
+    double *F = out + mean_off[k] + 1 + 3 * (size_t)(i - atom_off[k]);
+    for (int q = 0; q < 3; q++) F[q] = f[q];
     e = E_row[row_of_atom[i]];
-    if (local_energy) local_energy[i] = e;
-    for (int k = 0; k < 6; k++) v[k] = vir[6 * (size_t)i + k];
+    for (int q = 0; q < 6; q++) v[q] = vir[6 * (size_t)i + q];
   }
   sh[0][threadIdx.x] = e;
-  for (int k = 0; k < 6; k++) sh[1 + k][threadIdx.x] = v[k];
+  for (int q = 0; q < 6; q++) sh[1 + q][threadIdx.x] = v[q];
   __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (threadIdx.x < s)
-      for (int k = 0; k < 7; k++) sh[k][threadIdx.x] += sh[k][threadIdx.x + s];
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st)
+      for (int q = 0; q < 7; q++) sh[q][threadIdx.x] += sh[q][threadIdx.x + st];
     __syncthreads();
   }
-  if (threadIdx.x < 7) blk_part[(size_t)blockIdx.x * 7 + threadIdx.x] = sh[threadIdx.x][0];
+  if (threadIdx.x < 7) blk_part[(size_t)c * 7 + threadIdx.x] = sh[threadIdx.x][0];
 }
 
-// Sum of the block partials in block order: energy and stress
-// (-1/V sum_b f (x) r, normalized_dot_product.cpp:201-206).
-__global__ void k_final_sum(int nblk, const double *blk_part, double vol_inv, double *E,
-                            double *S) {
-  int k = threadIdx.x;
-  if (k >= 7) return;
-  double acc = 0;
-  for (int b = 0; b < nblk; b++) acc += blk_part[(size_t)b * 7 + k];
-  if (k == 0) *E = acc;
-  else S[k - 1] = acc * vol_inv;
+// Per structure, the sum of its chunk partials in chunk order: energy and
+// stress -1/V sum_b f (x) r (normalized_dot_product.cpp:201-206).
+__global__ void k_final_sum(int n_struct, const int *struct_chunk_off, const double *blk_part,
+                            const double *vol_inv, const long long *mean_off,
+                            const int *atom_off, double *out) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n_struct) return;
+  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (int c = struct_chunk_off[k]; c < struct_chunk_off[k + 1]; c++)
+    for (int q = 0; q < 7; q++) acc[q] += blk_part[(size_t)c * 7 + q];
+  double *o = out + mean_off[k];
+  int nk = atom_off[k + 1] - atom_off[k];
+  o[0] = acc[0];
+  for (int q = 0; q < 6; q++) o[1 + 3 * nk + q] = acc[1 + q] * vol_inv[k];
 }
 
 __global__ void k_fill(double *p, long long n, double v) {
@@ -109,7 +118,8 @@ void predict(fm_model m, fm_structure s, int flags, int form, double *mean_efs,
   bool f64 = form == FM_FORM_FP64;
   size_t esz = f64 ? sizeof(double) : sizeof(float);
   int n = s->n, nd = s->nd, nos = s->nos;
-  int n_out = 1 + 3 * n + 6;
+  // mean_efs of every structure of the batch, concatenated.
+  size_t n_out = (size_t)s->mean_off[s->n_struct];
   // Per-species slices of the atom-major w and k matrices (n_s x M_s each)
   // and of the energy / variance partials.
   std::vector<size_t> wo(nos + 1, 0), eo(nos + 1, 0), qo(nos + 1, 0);
@@ -219,35 +229,44 @@ void predict(fm_model m, fm_structure s, int flags, int form, double *mean_efs,
   DevBuf fbond(pool, std::max<size_t>(1, (size_t)3 * s->n_bonds * esz));
   DevBuf vir(pool, std::max<size_t>(1, (size_t)6 * n * sizeof(double)));
   double *o = out.as<double>();
+  FM_HIP(hipMemsetAsync(o, 0, (size_t)n_out * sizeof(double), st));
   if (n > 0) {
     launch_adjoint(s, G.as<void>(), nd, fbond.as<void>(), vir.as<double>(), 1, kEmptyThresh,
                    -1.0);
-    int tpb = 256, nblk = (n + tpb - 1) / tpb;
-    DevBuf blk(pool, (size_t)nblk * 7 * sizeof(double));
+    DevBuf blk(pool, (size_t)s->n_chunks * 7 * sizeof(double));
     if (f64)
-      hipLaunchKernelGGL(k_gather<double>, dim3(nblk), dim3(tpb), 0, st, n, s->bond_off.as<int>(),
-                         s->in_off.as<int>(), s->in_bond.as<int>(), fbond.as<double>(),
-                         s->row_of_atom.as<int>(), E_row.as<double>(), vir.as<double>(), o + 1,
-                         nullptr, blk.as<double>());
+      hipLaunchKernelGGL(k_gather<double>, dim3(s->n_chunks), dim3(256), 0, st,
+                         s->d_chunk_begin.as<int>(), s->d_chunk_end.as<int>(),
+                         s->atom_struct.as<int>(), s->d_atom_off.as<int>(),
+                         s->d_mean_off.as<long long>(), s->bond_off.as<int>(), s->in_off.as<int>(),
+                         s->in_bond.as<int>(), fbond.as<double>(), s->row_of_atom.as<int>(),
+                         E_row.as<double>(), vir.as<double>(), o, blk.as<double>());
     else
-      hipLaunchKernelGGL(k_gather<float>, dim3(nblk), dim3(tpb), 0, st, n, s->bond_off.as<int>(),
-                         s->in_off.as<int>(), s->in_bond.as<int>(), fbond.as<float>(),
-                         s->row_of_atom.as<int>(), E_row.as<double>(), vir.as<double>(), o + 1,
-                         nullptr, blk.as<double>());
-    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(64), 0, st, nblk, blk.as<double>(),
-                       1.0 / s->volume, o, o + 1 + 3 * n);
+      hipLaunchKernelGGL(k_gather<float>, dim3(s->n_chunks), dim3(256), 0, st,
+                         s->d_chunk_begin.as<int>(), s->d_chunk_end.as<int>(),
+                         s->atom_struct.as<int>(), s->d_atom_off.as<int>(),
+                         s->d_mean_off.as<long long>(), s->bond_off.as<int>(), s->in_off.as<int>(),
+                         s->in_bond.as<int>(), fbond.as<float>(), s->row_of_atom.as<int>(),
+                         E_row.as<double>(), vir.as<double>(), o, blk.as<double>());
+    hipLaunchKernelGGL(k_final_sum, dim3((s->n_struct + 63) / 64), dim3(64), 0, st, s->n_struct,
+                       s->d_struct_chunk_off.as<int>(), blk.as<double>(), s->d_vol_inv.as<double>(),
+                       s->d_mean_off.as<long long>(), s->d_atom_off.as<int>(), o);
     ctx->launches[ST_ADJ] += 3;
-  } else {
-    hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, st, o, (long long)n_out, 0.0);
   }
   FM_HIP(hipGetLastError());
   ctx->mark_end(ST_ADJ);
 
   if (mean_efs)
     FM_HIP(hipMemcpyAsync(mean_efs, o, (size_t)n_out * sizeof(double), hipMemcpyDeviceToHost, st));
-  if (local_var && want_var && n > 0)
-    FM_HIP(hipMemcpyAsync(local_var, var_row.as<double>(), (size_t)n * sizeof(double),
+  if (local_var && want_var && n > 0) {
+    // Species-sorted rows of the batch to each structure's species-sorted
+    // order (compute_cluster_uncertainties order).
+    std::vector<double> vr(n);
+    FM_HIP(hipMemcpyAsync(vr.data(), var_row.as<double>(), (size_t)n * sizeof(double),
                           hipMemcpyDeviceToHost, st));
+    FM_HIP(hipStreamSynchronize(st));
+    for (int r = 0; r < n; r++) local_var[s->var_pos[r]] = vr[r];
+  }
   if (local_energy && n > 0) {
     // E_row is in species-sorted order; map to atom order on the host.
     std::vector<double> er(n);

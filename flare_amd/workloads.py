@@ -70,3 +70,37 @@ def lennard_jones(cell, positions, eps=0.4, sig=2.6, rc=7.8):
             fmag = 24 * eps * (2 * x ** 12 - x ** 6) / rr[m] ** 2
             F[i] -= np.sum(fmag[:, None] * dd[m], axis=0)
     return E, F
+
+
+def batch_structures(n_structures=512, seed=5, n_min=16, n_max=2048, nos=3):
+    """Config 3 structures: atom counts log-uniform over [n_min, n_max];
+    species uniform over nos; density 0.05-0.09 atoms/A^3; a 50% coin flip
+    between bulk (random triclinic cell) and slab (10-15 A of vacuum along
+    the third lattice vector). Sites come from an FCC lattice scaled to the
+    density with random vacancies down to N atoms, displaced by 0.1 A."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n_structures):
+        n = int(round(np.exp(rng.uniform(np.log(n_min), np.log(n_max)))))
+        rho = rng.uniform(0.05, 0.09)
+        # FCC supercell with at least n sites (4 per conventional cell).
+        reps = np.ones(3, int)
+        while 4 * reps.prod() < n * rng.uniform(1.0, 1.05) + 1e-9:
+            reps[np.argmin(reps)] += 1
+        n_sites = 4 * int(reps.prod())
+        a = (4.0 / rho) ** (1.0 / 3.0)
+        cell, pos = lattice(FCC_BASIS, a, tuple(reps))
+        keep = np.sort(rng.choice(n_sites, n, replace=False))
+        pos = pos[keep]
+        # Random triclinic strain (shear up to 10%).
+        strain = np.eye(3) + np.triu(rng.uniform(-0.1, 0.1, (3, 3)), 1)
+        cell = cell @ strain
+        pos = pos @ strain
+        if rng.uniform() < 0.5:
+            c = cell[2] / np.linalg.norm(cell[2])
+            cell = cell.copy()
+            cell[2] = cell[2] + c * rng.uniform(10.0, 15.0)
+        pos = pos + rng.normal(scale=0.1, size=pos.shape)
+        species = rng.integers(0, nos, n).astype(np.int32)
+        out.append((cell, species, pos))
+    return out

@@ -72,6 +72,23 @@ int fm_structure_create_device(fm_context ctx, const double *cell, int n_atoms,
                                const int32_t *species_dev,
                                const double *positions_dev, double cutoff,
                                fm_structure *out);
+/* A batch of periodic structures in one object (one launch per stage for
+ * the whole batch): cells is n_structures x 9, n_atoms per structure,
+ * species and positions concatenated in structure order. Every structure
+ * has its own periodic images; atoms never see atoms of another structure.
+ * Structure objects of one structure and of a batch are used alike:
+ * fm_predict writes the mean_efs of each structure one after another and
+ * each structure's local variances in its own species-sorted order. No
+ * reference counterpart (the reference evaluates one Structure per call). */
+int fm_batch_create(fm_context ctx, int n_structures, const double *cells,
+                    const int32_t *n_atoms, const int32_t *species,
+                    const double *positions, double cutoff, fm_structure *out);
+int fm_batch_create_device(fm_context ctx, int n_structures, const double *cells,
+                           const int32_t *n_atoms, const int32_t *species_dev,
+                           const double *positions_dev, double cutoff,
+                           fm_structure *out);
+/* Atom offset, sweep and volume of structure k of a batch. */
+int fm_batch_info(fm_structure s, int k, int *atom_offset, int *sweep, double *volume);
 int fm_structure_destroy(fm_structure s);
 /* sweep (structure.cpp:35), total neighbours, volume, single_sweep_cutoff. */
 int fm_structure_info(fm_structure s, int *sweep, int64_t *n_neighbors,
@@ -140,7 +157,9 @@ int fm_model_destroy(fm_model m);
  * normalized_dot_product.cpp:117-214 contracted with alpha by the adjoint,
  * so dB/dr is never formed). Outputs, any may be NULL:
  *   mean_efs     1 + 3N + 6: [E, F (atom-major xyz), S (xx,xy,xz,yy,yz,zz)]
- *   local_var    N, species-sorted order (compute_cluster_uncertainties)
+ *                (for a batch, one such block per structure, concatenated)
+ *   local_var    N, species-sorted order (compute_cluster_uncertainties),
+ *                per structure for a batch
  *   local_energy N, atom order (this kernel's per-atom energies). */
 int fm_predict(fm_model m, fm_structure s, int flags, int form,
                double *mean_efs, double *local_var, double *local_energy);
@@ -178,9 +197,9 @@ int fm_mapped_info(fm_mapped mp, int *power, int *normalized, int *n_species,
 int fm_mapped_destroy(fm_mapped mp);
 /* PairFLARE::compute (pair_flare.cpp:68-184) on a periodic structure whose
  * FP64 B2 descriptors were computed with the potential's settings
- * (radial_r1 = 0): per-atom energies (atom order), forces N x 3, and the
- * virial sum_bonds -f (x) r (xx, xy, xz, yy, yz, zz). Any output may be
- * NULL. */
+ * (radial_r1 = 0): per-atom energies (atom order), forces N x 3, and for
+ * each structure the virial sum_bonds -f (x) r (xx, xy, xz, yy, yz, zz).
+ * Any output may be NULL. */
 int fm_mapped_compute(fm_mapped mp, fm_structure s, double *atom_energy,
                       double *forces, double *virial);
 
