@@ -99,6 +99,125 @@ def run_form(lib, ctx, model_h, calc, cell, sp_dev, pos_dev, n, form, steps, war
     return dict(total=total, lat=np.array(lat), stage=stage, mean=mean.copy(), var=var.copy())
 
 
+def batched_leg(lib, ctx, hip, steps, warmup):
+    """Config 3: 512 structures (16-2048 atoms, 3 species, bulk/slab cells)
+    in one batch per step; B2 [3, 8, 4] rc 5.5 (d = 1500); 1500 sparse
+    environments from the first structures (seed 6); FP64 vs chosen."""
+    from flare_amd import _C_flare as CF
+    from flare_amd import _lib
+    from flare_amd.workloads import batch_structures
+    structs = batch_structures(512, seed=5)
+    cut = np.full((3, 3), 5.5)
+    calc = CF.B2("chebyshev", "quadratic", [0.0, 5.5], [], [3, 8, 4], cut)
+    sgp = CF.SparseGP([CF.NormalizedDotProduct(2.0, 2.0)], 0.01, 0.05, 0.005)
+    rng = np.random.default_rng(6)
+    need = 1500
+    for st in structs:
+        if need <= 0:
+            break
+        s = CF.Structure(*st, 5.5, [calc])
+        take = min(need, len(st[1]))
+        sgp.add_specific_environments(s, sorted(rng.choice(len(st[1]), take, replace=False)))
+        need -= take
+    sgp.update_L_inverse()
+    sgp.set_alpha(np.random.default_rng(4).normal(scale=0.1, size=sgp.n_sparse))
+    n_atoms = np.array([len(s[1]) for s in structs], np.int32)
+    cells = np.stack([s[0].reshape(9) for s in structs]).astype(np.float64)
+    sp_dev = hip.upload(np.ascontiguousarray(np.concatenate([s[1] for s in structs]).astype(np.int32)))
+    pos_dev = hip.upload(np.ascontiguousarray(np.concatenate([s[2] for s in structs]).reshape(-1)))
+    hip.L.hipDeviceSynchronize()
+    n = int(n_atoms.sum())
+    total = int(np.sum(1 + 3 * n_atoms.astype(np.int64) + 6))
+    settings, keep = calc._settings()
+    model_h = sgp._model(0, ctx).h
+    flags = _lib.PREDICT_MEAN | _lib.PREDICT_LOCAL
+    out = {}
+    res = {}
+    for name, form in (("fp64", _lib.FORM_FP64), ("chosen", _lib.FORM_CHOSEN)):
+        mean = np.zeros(total)
+        var = np.zeros(n)
+
+        def step():
+            h = C.c_void_p()
+            _lib.check(lib.fm_batch_create_device(ctx.h, len(structs), _lib.ptr(cells),
+                                                  _lib.ptr(n_atoms), sp_dev, pos_dev, 5.5,
+                                                  C.byref(h)))
+            _lib.check(lib.fm_structure_compute_b2(h, C.byref(settings), form))
+            _lib.check(lib.fm_predict(model_h, h, flags, form, _lib.ptr(mean), _lib.ptr(var),
+                                      None))
+            lib.fm_structure_destroy(h)
+
+        for _ in range(warmup):
+            step()
+        lat = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            a = time.perf_counter()
+            step()
+            lat.append(time.perf_counter() - a)
+        dt = time.perf_counter() - t0
+        res[name] = (mean.copy(), var.copy())
+        out[name] = {"atom_evals_per_s": n * steps / dt, "ms_per_step": dt / steps * 1e3,
+                     "p50_ms": float(np.percentile(lat, 50) * 1e3),
+                     "p99_ms": float(np.percentile(lat, 99) * 1e3),
+                     "stage_ms": {k: round(v[0], 4) for k, v in ctx.timings().items()}}
+    del keep
+    # Errors of the chosen form against FP64 over all forces / variances.
+    o = 0
+    dF = dE = 0.0
+    for na in n_atoms:
+        m64, mch = res["fp64"][0][o:o + 1 + 3 * na + 6], res["chosen"][0][o:o + 1 + 3 * na + 6]
+        dE = max(dE, abs(m64[0] - mch[0]) / na)
+        dF = max(dF, float(np.abs(m64[1:1 + 3 * na] - mch[1:1 + 3 * na]).max()))
+        o += 1 + 3 * na + 6
+    s64, sch = np.sqrt(np.abs(res["fp64"][1])), np.sqrt(np.abs(res["chosen"][1]))
+    out["workload"] = (f"512 structures, {n} atoms (16-2048 each), 3 species, bulk/slab; "
+                       f"B2 [3,8,4] rc 5.5 (d=1500); {sgp.n_sparse} sparse; one batch per step")
+    out["speedup_vs_fp64"] = out["chosen"]["atom_evals_per_s"] / out["fp64"]["atom_evals_per_s"]
+    out["error_vs_fp64"] = {"energy_meV_per_atom": dE * 1e3, "force_max_meV_per_A": dF * 1e3,
+                            "sigma_rel_max": float(np.max(np.abs(sch - s64) / np.maximum(s64, 1e-300)))}
+    return out, (sgp, calc)
+
+
+def serving_leg(sgp, calc, batched, n_requests=2000, n_fp64=200):
+    """Config 4: Poisson stream of requests (sizes log-uniform 32-8192
+    atoms, the batched model), served with continuous batching in the
+    chosen form at 0.7x the batched throughput; FP64 drop-in evaluated
+    request by request on a prefix of the same stream."""
+    from flare_amd import _C_flare as CF
+    from flare_amd import _lib
+    from flare_amd.serving import Server, latency_stats, poisson_run
+    from flare_amd.workloads import batch_structures
+    reqs_s = batch_structures(n_requests, seed=41, n_min=32, n_max=8192)
+    mean_n = float(np.mean([len(s[1]) for s in reqs_s]))
+    rate = 0.7 * batched["chosen"]["atom_evals_per_s"] / mean_n
+    srv = Server(sgp, calc, 5.5, form=_lib.FORM_CHOSEN, hbm_budget=24 << 30, n_workers=2)
+    try:
+        reqs, wall = poisson_run(srv, reqs_s, rate=rate, seed=3)
+    finally:
+        srv.close()
+    st = latency_stats(reqs)
+    atoms = sum(len(s[1]) for s in reqs_s)
+    # FP64 drop-in, request by request, on the first n_fp64 requests.
+    lat64 = []
+    t0 = time.perf_counter()
+    for cell, sp, pos in reqs_s[:n_fp64]:
+        a = time.perf_counter()
+        s = CF.Structure(cell, sp, pos, 5.5, [calc], form=_lib.FORM_FP64)
+        sgp.predict_local_uncertainties(s)
+        lat64.append(time.perf_counter() - a)
+    dt64 = time.perf_counter() - t0
+    atoms64 = sum(len(s[1]) for s in reqs_s[:n_fp64])
+    return {"workload": f"{n_requests} requests, 32-8192 atoms (mean {mean_n:.0f}), Poisson "
+                        f"rate {rate:.1f}/s (0.7x batched chosen throughput)",
+            "atom_evals_per_s": atoms / wall, "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
+            "batches": len(srv.batches),
+            "fp64_request_by_request": {"atom_evals_per_s": atoms64 / dt64,
+                                        "p50_ms": float(np.percentile(lat64, 50) * 1e3),
+                                        "p99_ms": float(np.percentile(lat64, 99) * 1e3),
+                                        "requests": n_fp64}}
+
+
 def cpu_baseline(sgp, threads):
     """The oracle's scalable mode (cell list + per-atom adjoint with K.alpha)
     on a 4,000-atom sample of the same alloy and model, FP64, OpenMP."""
@@ -125,6 +244,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--sparse", type=int, default=2000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workloads", default="all", choices=["single", "all"],
+                    help="single: the 32k-atom structure only; all: also the batched and "
+                         "served workloads")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -213,6 +335,12 @@ def main():
                           "sigma_rel_max": float(srel.max()),
                           "sigma_rel_median": float(np.median(srel))},
     }
+    if args.workloads == "all":
+        log("batched workload ...")
+        out["batched"], (bsgp, bcalc) = batched_leg(lib, ctx, hip, max(3, args.steps // 2),
+                                                    1)
+        log("serving workload ...")
+        out["serving"] = serving_leg(bsgp, bcalc, out["batched"])
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
         log(f"cpu baseline on {threads} threads ...")

@@ -123,12 +123,15 @@ void d2h(T *dst, const fm::DevBuf &src, size_t count, hipStream_t st) {
   if (dst && count) FM_HIP(hipMemcpyAsync(dst, src.as<T>(), count * sizeof(T), hipMemcpyDeviceToHost, st));
 }
 
-__global__ void k_check_positions(int n, const double *pos, const int *species, int *bad) {
+// flags[0]: bit 1 non-finite position, bit 2 negative species;
+// flags[1]: largest species.
+__global__ void k_check_positions(int n, const double *pos, const int *species, int *flags) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   for (int k = 0; k < 3; k++)
-    if (!isfinite(pos[3 * i + k])) atomicOr(bad, 1);
-  if (species[i] < 0) atomicOr(bad, 2);
+    if (!isfinite(pos[3 * i + k])) atomicOr(&flags[0], 1);
+  if (species[i] < 0) atomicOr(&flags[0], 2);
+  atomicMax(&flags[1], species[i]);
 }
 
 }  // namespace
@@ -301,17 +304,17 @@ static int structure_create(fm_context ctx, int n_struct, const double *cells,
         FM_HIP(hipMemcpyAsync(s->species.as<int>(), species, (size_t)n * sizeof(int), kind, st));
         FM_HIP(hipMemcpyAsync(s->positions.as<double>(), positions, 3 * (size_t)n * sizeof(double),
                               kind, st));
-        fm::DevBuf bad(pool, sizeof(int));
-        FM_HIP(hipMemsetAsync(bad.as<int>(), 0, sizeof(int), st));
+        fm::DevBuf flags(pool, 2 * sizeof(int));
+        FM_HIP(hipMemsetAsync(flags.as<int>(), 0, 2 * sizeof(int), st));
         hipLaunchKernelGGL(k_check_positions, dim3((n + 255) / 256), dim3(256), 0, st, n,
-                           s->positions.as<double>(), s->species.as<int>(), bad.as<int>());
-        int hb = 0;
-        FM_HIP(hipMemcpyAsync(&hb, bad.as<int>(), sizeof(int), hipMemcpyDeviceToHost, st));
+                           s->positions.as<double>(), s->species.as<int>(), flags.as<int>());
+        int hf[2] = {0, 0};
+        FM_HIP(hipMemcpyAsync(hf, flags.as<int>(), 2 * sizeof(int), hipMemcpyDeviceToHost, st));
         FM_HIP(hipStreamSynchronize(st));
-        FM_CHECK(!(hb & 1), FM_ERR_INVALID, "positions contain NaN or Inf");
-        FM_CHECK(!(hb & 2), FM_ERR_INVALID, "species must be non-negative");
+        FM_CHECK(!(hf[0] & 1), FM_ERR_INVALID, "positions contain NaN or Inf");
+        FM_CHECK(!(hf[0] & 2), FM_ERR_INVALID, "species must be non-negative");
+        s->max_species = hf[1];
       }
-      FM_HIP(hipStreamSynchronize(st));
       ctx->mark_begin(fm::ST_NEIGH);
       if (n > 0) {
         fm::build_neighbors(s);
@@ -426,11 +429,7 @@ int fm_structure_compute_b2(fm_structure s, const fm_b2_settings *st_in, int for
     s->nd = nr * (nr + 1) / 2 * (s->lmax + 1);
     if (s->n > 0) {
       // Species must lie in [0, n_species) for this descriptor.
-      std::vector<int> sp(s->n);
-      FM_HIP(hipMemcpyAsync(sp.data(), s->species.as<int>(), (size_t)s->n * sizeof(int),
-                            hipMemcpyDeviceToHost, stream));
-      FM_HIP(hipStreamSynchronize(stream));
-      for (int v : sp) FM_CHECK(v >= 0 && v < s->nos, FM_ERR_INVALID, "species outside [0, n_species)");
+      FM_CHECK(s->max_species < s->nos, FM_ERR_INVALID, "species outside [0, n_species)");
     }
     std::vector<int> pairs;
     for (int a = 0; a < nr; a++)
@@ -454,9 +453,11 @@ int fm_structure_compute_b2(fm_structure s, const fm_b2_settings *st_in, int for
     s->nn.alloc(pool, std::max<size_t>(1, (size_t)s->n * sizeof(double)));
     if (s->n > 0) fm::launch_b2(s);
     ctx->mark_end(fm::ST_DESC);
-    // Per-structure species-sorted position of every global row.
-    s->var_pos.assign(s->n, 0);
-    if (s->n > 0) {
+    // Per-structure species-sorted position of every global row (the
+    // identity for a single structure).
+    s->var_pos.resize(s->n);
+    for (int r = 0; r < s->n; r++) s->var_pos[r] = r;
+    if (s->n > 0 && s->n_struct > 1) {
       std::vector<int> sp(s->n);
       FM_HIP(hipMemcpyAsync(sp.data(), s->species.as<int>(), s->n * sizeof(int),
                             hipMemcpyDeviceToHost, stream));
@@ -473,25 +474,31 @@ int fm_structure_compute_b2(fm_structure s, const fm_b2_settings *st_in, int for
         }
       }
     }
-    // Neighbour counts by species (b2.cpp:106-112).
-    s->nb_by_type.assign(s->nos, 0);
-    if (s->n > 0) {
-      std::vector<int> bo(s->n + 1), sp(s->n);
-      FM_HIP(hipMemcpyAsync(bo.data(), s->bond_off.as<int>(), (s->n + 1) * sizeof(int),
-                            hipMemcpyDeviceToHost, stream));
-      FM_HIP(hipMemcpyAsync(sp.data(), s->species.as<int>(), s->n * sizeof(int),
-                            hipMemcpyDeviceToHost, stream));
-      FM_HIP(hipStreamSynchronize(stream));
-      for (int i = 0; i < s->n; i++) s->nb_by_type[sp[i]] += bo[i + 1] - bo[i];
-    }
+    s->nb_by_type.clear();  // filled on first export (nb_by_type_of)
     s->has_b2 = true;
   });
+}
+
+// Neighbour counts by species (b2.cpp:106-112), computed on first use.
+static void nb_by_type_of(fm_structure s) {
+  if (!s->nb_by_type.empty()) return;
+  s->nb_by_type.assign(s->nos, 0);
+  if (s->n == 0) return;
+  hipStream_t st = s->ctx->stream;
+  std::vector<int> bo(s->n + 1), sp(s->n);
+  FM_HIP(hipMemcpyAsync(bo.data(), s->bond_off.as<int>(), (s->n + 1) * sizeof(int),
+                        hipMemcpyDeviceToHost, st));
+  FM_HIP(hipMemcpyAsync(sp.data(), s->species.as<int>(), s->n * sizeof(int),
+                        hipMemcpyDeviceToHost, st));
+  FM_HIP(hipStreamSynchronize(st));
+  for (int i = 0; i < s->n; i++) s->nb_by_type[sp[i]] += bo[i + 1] - bo[i];
 }
 
 int fm_structure_b2_sizes(fm_structure s, int *n_descriptors, int32_t *n_clusters_by_type,
                           int32_t *n_neighbors_by_type) {
   return guarded([&] {
     FM_CHECK(s && s->has_b2, FM_ERR_STATE, "descriptors have not been computed");
+    nb_by_type_of(s);
     if (n_descriptors) *n_descriptors = s->nd;
     for (int t = 0; t < s->nos; t++) {
       if (n_clusters_by_type) n_clusters_by_type[t] = s->n_by_type[t];

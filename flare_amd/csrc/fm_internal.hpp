@@ -124,6 +124,7 @@ struct fm_structure_s {
   // of structure k are [atom_off[k], atom_off[k+1]). Single structures have
   // n_struct = 1 and the fields below (cell ... sweep) describe structure 0.
   int n_struct = 1;
+  int max_species = -1;
   std::vector<int> atom_off;
   std::vector<fm::CellArgs> cellargs;
   std::vector<double> volumes, sweep_cutoffs;
@@ -193,5 +194,5 @@ void launch_adjoint(fm_structure s, const void *G, int64_t g_ld, void *fbond,
 void predict(fm_model m, fm_structure s, int flags, int form, double *mean_efs,
              double *local_var, double *local_energy);
 // scans (fm_neighbors.hip)
-int scan_exclusive(fm_context ctx, const int *in, int *out, int n);
+int scan_exclusive(fm_context ctx, const int *in, int *out, int n, bool want_total);
 }  // namespace fm

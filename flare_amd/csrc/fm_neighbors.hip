@@ -85,9 +85,9 @@ __global__ void k_scan_apply(const int *in, int n, const int *bsum, int *out) {
   if (i == n - 1) out[n] = bsum[blockIdx.x] + e + v;
 }
 
-// out[0..n] = exclusive scan of in[0..n); out[n] = total. Returns the total
-// (synchronises the context stream).
-int scan_exclusive(fm_context ctx, const int *in, int *out, int n) {
+// out[0..n] = exclusive scan of in[0..n); out[n] = total. With want_total
+// it returns the total (synchronising the context stream), otherwise 0.
+int scan_exclusive(fm_context ctx, const int *in, int *out, int n, bool want_total) {
   if (n == 0) {
     FM_HIP(hipMemsetAsync(out, 0, sizeof(int), ctx->stream));
     return 0;
@@ -100,6 +100,7 @@ int scan_exclusive(fm_context ctx, const int *in, int *out, int n) {
   hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(kScanBlock), 0, ctx->stream, in, n,
                      bsum.as<int>(), out);
   FM_HIP(hipGetLastError());
+  if (!want_total) return 0;
   int total = 0;
   FM_HIP(hipMemcpyAsync(&total, out + n, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   FM_HIP(hipStreamSynchronize(ctx->stream));
@@ -393,7 +394,7 @@ void build_neighbors(fm_structure s) {
   FM_HIP(hipMemsetAsync(cursor.as<int>(), 0, (size_t)ncell * sizeof(int), st));
   FM_HIP(hipMemsetAsync(maxc.as<int>(), 0, sizeof(int), st));
   hipLaunchKernelGGL(k_bin_count, dim3(nblk), dim3(tpb), 0, st, n, bin.as<int>(), bcount.as<int>());
-  scan_exclusive(ctx, bcount.as<int>(), bstart.as<int>(), ncell);
+  scan_exclusive(ctx, bcount.as<int>(), bstart.as<int>(), ncell, false);
   hipLaunchKernelGGL(k_bin_fill, dim3(nblk), dim3(tpb), 0, st, n, bin.as<int>(), bstart.as<int>(),
                      cursor.as<int>(), order.as<int>());
   int wblk = (n + kNbrWaves - 1) / kNbrWaves;
@@ -402,7 +403,7 @@ void build_neighbors(fm_structure s) {
                      s->species.as<int>(), cas, ast, boffp, s->count.as<int>(), nullptr, nullptr,
                      nullptr, nullptr, maxc.as<int>());
   FM_HIP(hipGetLastError());
-  int total = scan_exclusive(ctx, s->count.as<int>(), s->cumulative.as<int>(), n);
+  int total = scan_exclusive(ctx, s->count.as<int>(), s->cumulative.as<int>(), n, true);
   int max_count = 0;
   FM_HIP(hipMemcpyAsync(&max_count, maxc.as<int>(), sizeof(int), hipMemcpyDeviceToHost, st));
   FM_HIP(hipStreamSynchronize(st));
@@ -505,7 +506,7 @@ void build_bonds(fm_structure s) {
   hipLaunchKernelGGL(k_bond_count, dim3(nblk), dim3(tpb), 0, st, n, s->species.as<int>(),
                      s->cumulative.as<int>(), s->nspecies.as<int>(), s->rel.as<double>(),
                      s->d_cutoffs.as<double>(), s->nos, bcount.as<int>());
-  int nbonds = scan_exclusive(ctx, bcount.as<int>(), s->bond_off.as<int>(), n);
+  int nbonds = scan_exclusive(ctx, bcount.as<int>(), s->bond_off.as<int>(), n, true);
   s->n_bonds = nbonds;
   s->bond_q.alloc(pool, (size_t)nbonds * sizeof(int));
   DevBuf in_count(pool, (size_t)n * sizeof(int));
@@ -515,7 +516,7 @@ void build_bonds(fm_structure s) {
                      s->d_cutoffs.as<double>(), s->nos, s->bond_off.as<int>(), s->sidx.as<int>(),
                      s->bond_q.as<int>(), in_count.as<int>());
   s->in_off.alloc(pool, ((size_t)n + 1) * sizeof(int));
-  scan_exclusive(ctx, in_count.as<int>(), s->in_off.as<int>(), n);
+  scan_exclusive(ctx, in_count.as<int>(), s->in_off.as<int>(), n, false);
   s->in_bond.alloc(pool, (size_t)nbonds * sizeof(int));
   DevBuf cursor(pool, (size_t)n * sizeof(int));
   FM_HIP(hipMemsetAsync(cursor.as<int>(), 0, (size_t)n * sizeof(int), st));
@@ -525,21 +526,30 @@ void build_bonds(fm_structure s) {
                        cursor.as<int>(), s->in_bond.as<int>());
   hipLaunchKernelGGL(k_in_sort, dim3(nblk), dim3(tpb), 0, st, n, s->in_off.as<int>(),
                      s->in_bond.as<int>());
-  // Species-sorted rows.
+  // Species-sorted rows: one scan of species flags per species, the
+  // per-species totals fetched with a single copy.
   s->row_of_atom.alloc(pool, (size_t)n * sizeof(int));
   s->atom_of_row.alloc(pool, (size_t)n * sizeof(int));
-  DevBuf flag(pool, (size_t)n * sizeof(int)), rank(pool, ((size_t)n + 1) * sizeof(int));
-  s->n_by_type.assign(s->nos, 0);
-  s->type_off.assign(s->nos + 1, 0);
+  DevBuf flag(pool, (size_t)n * sizeof(int));
+  DevBuf rank(pool, (size_t)s->nos * ((size_t)n + 1) * sizeof(int));
   for (int t = 0; t < s->nos; t++) {
     hipLaunchKernelGGL(k_species_flag, dim3(nblk), dim3(tpb), 0, st, n, s->species.as<int>(), t,
                        flag.as<int>());
-    int cnt = scan_exclusive(ctx, flag.as<int>(), rank.as<int>(), n);
+    scan_exclusive(ctx, flag.as<int>(), rank.as<int>() + (size_t)t * (n + 1), n, false);
+  }
+  std::vector<int> counts(s->nos);
+  for (int t = 0; t < s->nos; t++)
+    FM_HIP(hipMemcpyAsync(&counts[t], rank.as<int>() + (size_t)t * (n + 1) + n, sizeof(int),
+                          hipMemcpyDeviceToHost, st));
+  FM_HIP(hipStreamSynchronize(st));
+  s->n_by_type.assign(s->nos, 0);
+  s->type_off.assign(s->nos + 1, 0);
+  for (int t = 0; t < s->nos; t++) {
+    s->n_by_type[t] = counts[t];
+    s->type_off[t + 1] = s->type_off[t] + counts[t];
     hipLaunchKernelGGL(k_species_rows, dim3(nblk), dim3(tpb), 0, st, n, s->species.as<int>(), t,
-                       rank.as<int>(), s->type_off[t], s->row_of_atom.as<int>(),
-                       s->atom_of_row.as<int>());
-    s->n_by_type[t] = cnt;
-    s->type_off[t + 1] = s->type_off[t] + cnt;
+                       rank.as<int>() + (size_t)t * (n + 1), s->type_off[t],
+                       s->row_of_atom.as<int>(), s->atom_of_row.as<int>());
   }
   FM_HIP(hipGetLastError());
 }

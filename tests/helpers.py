@@ -63,3 +63,10 @@ def rel_err(a, b):
     if den == 0:
         return float(np.max(np.abs(a - b))) if a.size else 0.0
     return float(np.max(np.abs(a - b)) / den)
+
+
+def var_err(a, b, sig2):
+    """Error of local variances sigma^2 - |L_inv k|^2 relative to sigma^2,
+    the size of the cancelling terms (a conditioning-aware bound: the
+    variances themselves can be many orders of magnitude smaller)."""
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b)))) / sig2 if np.size(a) else 0.0
