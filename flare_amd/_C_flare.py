@@ -147,11 +147,32 @@ class DescriptorValues:
                     "neighbor_coordinates", "cutoff_values"):
             self._load()
             return self._cache[name]
-        if name in ("descriptor_force_dervs", "descriptor_force_dots", "cutoff_dervs"):
-            raise AttributeError(
-                f"{name}: the GPU path contracts forces by the adjoint and never forms dense "
-                "descriptor gradients")
+        if name in ("descriptor_force_dervs", "descriptor_force_dots"):
+            self._load_dervs()
+            return self._cache[name]
+        if name == "cutoff_dervs":  # zero for B2 (b2.cpp:138)
+            return [np.zeros(3 * n) for n in self.n_neighbors_by_type]
         raise AttributeError(name)
+
+    def _load_dervs(self):
+        """Dense dB/dr (b2.cpp:181-242), built on the device on first use.
+        Prediction never needs them (forces come from the adjoint); they
+        serve the DTC/SOR kernels and callers that read them."""
+        if "descriptor_force_dervs" in self._cache or self._structure is None:
+            return
+        self._load()
+        L = _lib.load()
+        dervs, dots = [], []
+        for s in range(self.n_types):
+            nb = self.n_neighbors_by_type[s]
+            d = np.zeros((3 * nb, self.n_descriptors))
+            f = np.zeros(3 * nb)
+            _lib.check(L.fm_structure_get_b2_dervs(self._structure._h, s, _lib.ptr(d),
+                                                   _lib.ptr(f)))
+            dervs.append(d)
+            dots.append(f)
+        self._cache["descriptor_force_dervs"] = dervs
+        self._cache["descriptor_force_dots"] = dots
 
 
 class ClusterDescriptor:
@@ -236,6 +257,27 @@ class NormalizedDotProduct(Kernel):
             v[:, nb < _EMPTY] = 0.0
             K[c1[s]:c1[s] + a.shape[0], c2[s]:c2[s] + b.shape[0]] = v
         return K
+
+    def envs_struc(self, envs, struc, hyps=None):
+        """normalized_dot_product.cpp:117-214 on the GPU: the
+        n_sparse x (1 + 3N + 6) kernel matrix between a sparse set and a
+        structure's descriptors (DescriptorValues of a Structure)."""
+        sig = float(self._sigma if hyps is None else np.asarray(hyps)[0])
+        st = struc._structure
+        m = _DeviceModel.from_sparse_set(st._ctx, sig, self.power, envs)
+        K = np.zeros((envs.n_clusters, 1 + 3 * st.noa + 6))
+        _lib.check(_lib.load().fm_envs_struc(m.h, st._h, _lib.ptr(K)))
+        return K
+
+    def self_kernel_struc(self, struc, hyps=None):
+        """normalized_dot_product.cpp:450-586 on the GPU."""
+        sig = float(self._sigma if hyps is None else np.asarray(hyps)[0])
+        st = struc._structure
+        empty = ClusterDescriptor(struc.n_types, struc.n_descriptors)
+        m = _DeviceModel.from_sparse_set(st._ctx, sig, self.power, empty)
+        out = np.zeros(1 + 3 * st.noa + 6)
+        _lib.check(_lib.load().fm_self_kernel_struc(m.h, st._h, _lib.ptr(out)))
+        return out
 
 
 class Structure:
@@ -372,6 +414,7 @@ class SparseGP:
         self.Kuu = np.zeros((0, 0))
         self.Kuu_inverse = np.zeros((0, 0))
         self.Sigma = np.zeros((0, 0))
+        self.R_inv = np.zeros((0, 0))
         self._models = {}
 
     @property
@@ -454,14 +497,22 @@ class SparseGP:
         self.alpha = alpha.copy()
         self._invalidate()
 
-    def set_state(self, alpha, L_inv=None, Sigma=None, Kuu_inverse=None):
+    def set_state(self, alpha, L_inv=None, Sigma=None, Kuu_inverse=None, R_inv=None):
+        """Sets the trained state. R_inv (the inverse of the QR factor R,
+        sparse_gp.cpp:635-640) defines Sigma = R_inv R_inv^T; a Sigma given
+        alone is factored as Sigma = R_inv R_inv^T by Cholesky."""
         self.set_alpha(alpha)
         if L_inv is not None:
             self.L_inv = np.array(L_inv, dtype=np.float64)
             self.Kuu_inverse = (self.L_inv.T @ self.L_inv if Kuu_inverse is None
                                 else np.array(Kuu_inverse, dtype=np.float64))
-        if Sigma is not None:
+        if R_inv is not None:
+            self.R_inv = np.array(R_inv, dtype=np.float64)
+            self.Sigma = self.R_inv @ self.R_inv.T if Sigma is None else np.array(
+                Sigma, dtype=np.float64)
+        elif Sigma is not None:
             self.Sigma = np.array(Sigma, dtype=np.float64)
+            self.R_inv = np.linalg.cholesky(self.Sigma)
         self._invalidate()
 
     def _model(self, k, ctx):
@@ -504,11 +555,29 @@ class SparseGP:
         structure.mean_efs = mean
         structure.local_uncertainties = local
 
+    def _dense(self, structure, kind):
+        self._check_state()
+        if self.n_kernels != 1:
+            raise ValueError("DTC and SOR predictions take a model with one kernel")
+        if self.R_inv.shape != (self.n_sparse, self.n_sparse):
+            raise RuntimeError("R_inv (or Sigma) must be set for DTC and SOR predictions")
+        n = structure.noa
+        mean = np.zeros(1 + 3 * n + 6)
+        var = np.zeros(1 + 3 * n + 6)
+        m = self._model(0, structure._ctx)
+        _lib.check(_lib.load().fm_predict_dense(m.h, structure._h, kind, _lib.ptr(mean),
+                                                _lib.ptr(var)))
+        structure.mean_efs = mean
+        structure.variance_efs = var
+
     def predict_SOR(self, structure):
-        raise NotImplementedError("predict_SOR is not on the GPU path yet")
+        """sparse_gp.cpp:667-692: variance diag(K^T R_inv R_inv^T K)."""
+        self._dense(structure, _lib.DENSE_SOR)
 
     def predict_DTC(self, structure):
-        raise NotImplementedError("predict_DTC is not on the GPU path yet")
+        """sparse_gp.cpp:694-729: variance K_self - diag(K^T Kuu^-1 K) +
+        diag(K^T Sigma K)."""
+        self._dense(structure, _lib.DENSE_DTC)
 
 
 class _DeviceModel:
@@ -521,23 +590,38 @@ class _DeviceModel:
         cd = sgp.sparse_descriptors[k]
         off = sum(c.n_clusters for c in sgp.sparse_descriptors[:k])
         M = cd.n_clusters
-        nd = cd.n_descriptors
+        Linv = Rinv = None
+        if sgp.L_inv.shape[0] == sgp.n_sparse and M:
+            Linv = sgp.L_inv[off:off + M, off:off + M]
+        if sgp.n_kernels == 1 and sgp.R_inv.shape == (M, M) and M:
+            Rinv = sgp.R_inv
+        self._create(ctx, kern.sigma, kern.power, cd, sgp.alpha[off:off + M], Linv, Rinv)
+
+    def _create(self, ctx, sigma, power, cd, alpha, Linv=None, Rinv=None):
+        M, nd = cd.n_clusters, cd.n_descriptors
         desc = _lib.f64(np.concatenate(cd.descriptors, axis=0) if M else np.zeros((0, nd)))
         norms = _lib.f64(np.concatenate(cd.descriptor_norms) if M else np.zeros(0))
-        alpha = _lib.f64(sgp.alpha[off:off + M])
-        Linv = None
-        if sgp.L_inv.shape[0] == sgp.n_sparse and M:
-            Linv = _lib.f64(sgp.L_inv[off:off + M, off:off + M])
+        alpha = _lib.f64(alpha)
+        Linv = _lib.f64(Linv) if Linv is not None else None
+        Rinv = _lib.f64(Rinv) if Rinv is not None else None
+        self._keep = (desc, norms, alpha, Linv, Rinv)
         nbt = _lib.i32(cd.n_clusters_by_type)
         P = _lib.C.POINTER(_lib.C.c_double)
-        d = _lib.ModelDesc(kern.sigma, kern.power, cd.n_types, nd,
+        d = _lib.ModelDesc(float(sigma), float(power), cd.n_types, nd,
                            nbt.ctypes.data_as(_lib.C.POINTER(_lib.C.c_int32)),
                            desc.ctypes.data_as(P), norms.ctypes.data_as(P),
                            alpha.ctypes.data_as(P),
-                           Linv.ctypes.data_as(P) if Linv is not None else None)
+                           Linv.ctypes.data_as(P) if Linv is not None else None,
+                           Rinv.ctypes.data_as(P) if Rinv is not None else None)
         h = _lib.C.c_void_p()
         _lib.check(_lib.load().fm_model_create(ctx.h, _lib.C.byref(d), _lib.C.byref(h)))
         self.h = h
+
+    @classmethod
+    def from_sparse_set(cls, ctx, sigma, power, cd):
+        self = cls.__new__(cls)
+        self._create(ctx, sigma, power, cd, np.zeros(cd.n_clusters))
+        return self
 
     def __del__(self):
         try:

@@ -22,6 +22,7 @@ SOURCES = {
     "fm_gemm.hip": [],
     "fm_predict.hip": [],
     "fm_mapped.hip": [],
+    "fm_dtc.hip": [],
 }
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

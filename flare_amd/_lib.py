@@ -27,6 +27,8 @@ CUTOFF_COSINE = 1
 
 PREDICT_MEAN = 1
 PREDICT_LOCAL = 2
+DENSE_DTC = 1
+DENSE_SOR = 2
 
 # Every symbol include/flare_mi.h declares.
 EXPORTS = [
@@ -38,6 +40,7 @@ EXPORTS = [
     "fm_model_mapping_coefficients", "fm_mapped_create", "fm_mapped_read_file",
     "fm_mapped_info", "fm_mapped_destroy", "fm_mapped_compute",
     "fm_batch_create", "fm_batch_create_device", "fm_batch_info",
+    "fm_structure_get_b2_dervs", "fm_envs_struc", "fm_self_kernel_struc", "fm_predict_dense",
 ]
 
 
@@ -64,7 +67,7 @@ class ModelDesc(C.Structure):
                 ("n_descriptors", C.c_int), ("n_sparse_by_type", C.POINTER(C.c_int32)),
                 ("sparse_descriptors", C.POINTER(C.c_double)),
                 ("sparse_norms", C.POINTER(C.c_double)), ("alpha", C.POINTER(C.c_double)),
-                ("L_inv", C.POINTER(C.c_double))]
+                ("L_inv", C.POINTER(C.c_double)), ("R_inv", C.POINTER(C.c_double))]
 
 
 _lib = None

@@ -9,6 +9,7 @@
 #include <string>
 #include <vector>
 
+#include "fm_gemm.hpp"
 #include "fm_internal.hpp"
 
 namespace {
@@ -449,6 +450,14 @@ int fm_structure_compute_b2(fm_structure s, const fm_b2_settings *st_in, int for
     }
     size_t esz = form == FM_FORM_FP64 ? sizeof(double) : sizeof(float);
     s->B.alloc(pool, std::max<size_t>(1, (size_t)s->n * s->nd * esz));
+    if (form == FM_FORM_CHOSEN) {
+      size_t split_bytes = (size_t)s->n * fm::split_ld(s->nd) * sizeof(uint16_t);
+      s->Bh.alloc(pool, std::max<size_t>(1, split_bytes));
+      s->Bl.alloc(pool, std::max<size_t>(1, split_bytes));
+    } else {
+      s->Bh.release();
+      s->Bl.release();
+    }
     s->norm.alloc(pool, std::max<size_t>(1, (size_t)s->n * sizeof(double)));
     s->nn.alloc(pool, std::max<size_t>(1, (size_t)s->n * sizeof(double)));
     if (s->n > 0) fm::launch_b2(s);
@@ -475,6 +484,8 @@ int fm_structure_compute_b2(fm_structure s, const fm_b2_settings *st_in, int for
       }
     }
     s->nb_by_type.clear();  // filled on first export (nb_by_type_of)
+    s->has_dervs = false;    // dense derivatives are rebuilt on demand
+    s->DB.release();
     s->has_b2 = true;
   });
 }
@@ -599,7 +610,8 @@ int fm_model_create(fm_context ctx, const fm_model_desc *d, fm_model *out) {
     for (int t = 0; t < m->nos; t++) {
       int Ms = m->M_by_type[t], c0 = m->off[t];
       std::vector<double> X((size_t)Ms * nd, 0.0), XT((size_t)nd * Ms, 0.0), a(Ms, 0.0);
-      std::vector<float> X32((size_t)Ms * nd), XT32((size_t)nd * Ms);
+      size_t ldd = fm::split_ld(nd), ldm = fm::split_ld(Ms);
+      std::vector<float> X32((size_t)Ms * ldd, 0.0f), XT32((size_t)nd * ldm, 0.0f);
       for (int j = 0; j < Ms; j++) {
         double nrm = d->sparse_norms[c0 + j];
         // Empty sparse environments contribute nothing
@@ -612,22 +624,31 @@ int fm_model_create(fm_context ctx, const fm_model_desc *d, fm_model *out) {
         }
         a[j] = d->alpha[c0 + j];
       }
-      for (size_t k = 0; k < X.size(); k++) X32[k] = (float)X[k];
-      for (size_t k = 0; k < XT.size(); k++) XT32[k] = (float)XT[k];
+      for (int j = 0; j < Ms; j++)
+        for (int k = 0; k < nd; k++) X32[j * ldd + k] = (float)X[(size_t)j * nd + k];
+      for (int k = 0; k < nd; k++)
+        for (int j = 0; j < Ms; j++) XT32[k * ldm + j] = (float)XT[(size_t)k * Ms + j];
+      std::vector<uint16_t> xh(X32.size()), xl(X32.size()), xth(XT32.size()), xtl(XT32.size());
+      fm::split_bf16_host(X32.data(), X32.size(), xh.data(), xl.data());
+      fm::split_bf16_host(XT32.data(), XT32.size(), xth.data(), xtl.data());
       auto up = [&](fm::DevBuf &buf, const void *src, size_t bytes) {
         buf.alloc(pool, std::max<size_t>(1, bytes));
         if (bytes) FM_HIP(hipMemcpyAsync(buf.as<void>(), src, bytes, hipMemcpyHostToDevice, st));
       };
       m->X.emplace_back();
       m->XT.emplace_back();
-      m->X32.emplace_back();
-      m->XT32.emplace_back();
+      m->Xh.emplace_back();
+      m->Xl.emplace_back();
+      m->XTh.emplace_back();
+      m->XTl.emplace_back();
       m->alpha.emplace_back();
       m->L.emplace_back();
       up(m->X.back(), X.data(), X.size() * sizeof(double));
       up(m->XT.back(), XT.data(), XT.size() * sizeof(double));
-      up(m->X32.back(), X32.data(), X32.size() * sizeof(float));
-      up(m->XT32.back(), XT32.data(), XT32.size() * sizeof(float));
+      up(m->Xh.back(), xh.data(), xh.size() * sizeof(uint16_t));
+      up(m->Xl.back(), xl.data(), xl.size() * sizeof(uint16_t));
+      up(m->XTh.back(), xth.data(), xth.size() * sizeof(uint16_t));
+      up(m->XTl.back(), xtl.data(), xtl.size() * sizeof(uint16_t));
       up(m->alpha.back(), a.data(), a.size() * sizeof(double));
       int rows = 0, lower = 1;
       if (m->has_L && Ms > 0) {
@@ -655,8 +676,33 @@ int fm_model_create(fm_context ctx, const fm_model_desc *d, fm_model *out) {
       }
       m->L_rows.push_back(rows);
       m->L_lower.push_back(lower);
+      if (rows != Ms) m->L_block_diag = false;
+      m->xnorm.emplace_back(d->sparse_norms ? d->sparse_norms + c0 : nullptr,
+                            d->sparse_norms ? d->sparse_norms + c0 + Ms : nullptr);
       FM_HIP(hipStreamSynchronize(st));
     }
+    m->alpha_full.alloc(pool, std::max<size_t>(1, (size_t)M * sizeof(double)));
+    if (M)
+      FM_HIP(hipMemcpyAsync(m->alpha_full.as<double>(), m->host_alpha.data(), M * sizeof(double),
+                            hipMemcpyHostToDevice, st));
+    m->has_R = d->R_inv != nullptr && M > 0;
+    if (m->has_R) {
+      // R_inv^T, the left operand of |R_inv^T k|^2; lower triangular when
+      // R_inv is upper triangular (the QR factor's inverse).
+      std::vector<double> RT((size_t)M * M);
+      int upper = 1;
+      for (int r = 0; r < M; r++)
+        for (int c = 0; c < M; c++) {
+          double v = d->R_inv[(size_t)r * M + c];
+          RT[(size_t)c * M + r] = v;
+          if (c < r && v != 0.0) upper = 0;
+        }
+      m->R_lower = upper;
+      m->RinvT.alloc(pool, RT.size() * sizeof(double));
+      FM_HIP(hipMemcpyAsync(m->RinvT.as<double>(), RT.data(), RT.size() * sizeof(double),
+                            hipMemcpyHostToDevice, st));
+    }
+    FM_HIP(hipStreamSynchronize(st));
     *out = m;
   });
 }
@@ -676,6 +722,62 @@ int fm_predict(fm_model m, fm_structure s, int flags, int form, double *mean_efs
     FM_CHECK(m->ctx == s->ctx, FM_ERR_INVALID, "model and structure belong to different contexts");
     FM_HIP(hipSetDevice(s->ctx->device));
     fm::predict(m, s, flags, form, mean_efs, local_var, local_energy);
+  });
+}
+
+int fm_structure_get_b2_dervs(fm_structure s, int t, double *force_dervs, double *force_dots) {
+  return guarded([&] {
+    FM_CHECK(s && s->has_b2, FM_ERR_STATE, "descriptors have not been computed");
+    FM_CHECK(s->n_struct == 1, FM_ERR_INVALID, "descriptor export is per single structure");
+    FM_CHECK(t >= 0 && t < s->nos, FM_ERR_INVALID, "species out of range");
+    FM_HIP(hipSetDevice(s->ctx->device));
+    fm::compute_dervs(s);
+    hipStream_t st = s->ctx->stream;
+    size_t f0 = s->nbt_off[t], nb = s->nbt_off[t + 1] - f0;
+    if (nb == 0) return;
+    if (force_dervs)
+      FM_HIP(hipMemcpyAsync(force_dervs, s->DB.as<double>() + 3 * f0 * s->nd,
+                            3 * nb * s->nd * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (force_dots)
+      FM_HIP(hipMemcpyAsync(force_dots, s->fdots.as<double>() + 3 * f0, 3 * nb * sizeof(double),
+                            hipMemcpyDeviceToHost, st));
+    FM_HIP(hipStreamSynchronize(st));
+  });
+}
+
+static void dense_check(fm_model m, fm_structure s) {
+  FM_CHECK(m && s, FM_ERR_INVALID, "null argument");
+  FM_CHECK(m->ctx == s->ctx, FM_ERR_INVALID, "model and structure belong to different contexts");
+  FM_CHECK(s->has_b2, FM_ERR_STATE, "descriptors have not been computed");
+  FM_CHECK(m->nos == s->nos && m->nd == s->nd, FM_ERR_INVALID,
+           "model and descriptors have different species counts or sizes");
+  FM_HIP(hipSetDevice(s->ctx->device));
+}
+
+int fm_envs_struc(fm_model m, fm_structure s, double *K) {
+  return guarded([&] {
+    dense_check(m, s);
+    FM_CHECK(K, FM_ERR_INVALID, "null output");
+    fm::envs_struc(m, s, K);
+  });
+}
+
+int fm_self_kernel_struc(fm_model m, fm_structure s, double *kself) {
+  return guarded([&] {
+    dense_check(m, s);
+    FM_CHECK(kself, FM_ERR_INVALID, "null output");
+    fm::self_kernel_struc(m, s, kself);
+  });
+}
+
+int fm_predict_dense(fm_model m, fm_structure s, int kind, double *mean_efs, double *var_efs) {
+  return guarded([&] {
+    dense_check(m, s);
+    FM_CHECK(kind == FM_DENSE_DTC || kind == FM_DENSE_SOR, FM_ERR_INVALID,
+             "kind must be FM_DENSE_DTC or FM_DENSE_SOR");
+    FM_CHECK(kind != FM_DENSE_DTC || m->L_block_diag, FM_ERR_INVALID,
+             "the DTC variance needs an L_inv that is block diagonal over species");
+    fm::predict_dense(m, s, kind, mean_efs, var_efs);
   });
 }
 

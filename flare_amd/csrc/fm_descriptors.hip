@@ -14,6 +14,7 @@
 #include <cmath>
 
 #include "fm_device.hpp"
+#include "fm_gemm.hpp"
 #include "fm_internal.hpp"
 
 namespace fm {
@@ -92,10 +93,14 @@ __device__ void accumulate_c(const B2Args &a, int i, int lane, T *c) {
 }
 
 // B2 (b2.cpp:209-231) per atom: B[k] = sum_m c[n1, lm] c[n2, lm], written at
-// the atom's species-sorted row, with |B| and B.B.
+// the atom's species-sorted row, with |B| and B.B. The FP32 form also writes
+// each row as BF16 pairs (hi = bf16(B), lo = bf16(B - hi)) into Bh / Bl with
+// leading dimension ldp and zero padding, the operand of the kernel
+// contraction.
 template <typename T>
 __global__ __launch_bounds__(64 * kAtomsPerBlock) void k_b2(B2Args a, T *B, double *norm,
-                                                              double *nn) {
+                                                              double *nn, __bf16 *Bh, __bf16 *Bl,
+                                                              int ldp) {
   extern __shared__ __align__(16) unsigned char smem[];
   int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   int i = blockIdx.x * (blockDim.x >> 6) + w;
@@ -116,8 +121,18 @@ __global__ __launch_bounds__(64 * kAtomsPerBlock) void k_b2(B2Args a, T *B, doub
     T acc = 0;
     for (int m = 0; m < 2 * l + 1; m++) acc += c1[m] * c2[m];
     Brow[k] = acc;
+    if constexpr (sizeof(T) == 4) {
+      __bf16 h = (__bf16)acc;
+      Bh[(size_t)row * ldp + k] = h;
+      Bl[(size_t)row * ldp + k] = (__bf16)(acc - (float)h);
+    }
     s2 += acc * acc;
   }
+  if constexpr (sizeof(T) == 4)
+    for (int k = a.nd + lane; k < ldp; k += 64) {
+      Bh[(size_t)row * ldp + k] = (__bf16)0.0f;
+      Bl[(size_t)row * ldp + k] = (__bf16)0.0f;
+    }
   s2 = wave_sum(s2);
   if (lane == 0) {
     norm[row] = (double)sqrt(s2);
@@ -297,8 +312,7 @@ static B2Args make_args(fm_structure s) {
   return a;
 }
 
-static void upload_knorm(hipStream_t st, int lmax) {
-  static thread_local double host[kMaxH];
+void knorm_values(int lmax, double *out) {
   for (int l = 0; l <= lmax; l++)
     for (int m = -l; m <= l; m++) {
       int mm = m < 0 ? -m : m;
@@ -306,8 +320,13 @@ static void upload_knorm(hipStream_t st, int lmax) {
       for (int k = l - mm + 1; k <= l + mm; k++) ratio /= k;
       double K = std::sqrt((2 * l + 1) / (4 * 3.14159265358979323846) * ratio);
       if (mm > 0) K *= std::sqrt(2.0);
-      host[l * l + m + l] = K;
+      out[l * l + m + l] = K;
     }
+}
+
+static void upload_knorm(hipStream_t st, int lmax) {
+  static thread_local double host[kMaxH];
+  knorm_values(lmax, host);
   FM_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_knorm), host, sizeof(double) * (lmax + 1) * (lmax + 1),
                                 0, hipMemcpyHostToDevice, st));
 }
@@ -327,10 +346,12 @@ void launch_b2(fm_structure s) {
   if (nblk == 0) return;
   if (s->form == FM_FORM_FP64)
     hipLaunchKernelGGL(k_b2<double>, dim3(nblk), dim3(64 * apb), lds, st, a,
-                       s->B.as<double>(), s->norm.as<double>(), s->nn.as<double>());
+                       s->B.as<double>(), s->norm.as<double>(), s->nn.as<double>(),
+                       (__bf16 *)nullptr, (__bf16 *)nullptr, 0);
   else
     hipLaunchKernelGGL(k_b2<float>, dim3(nblk), dim3(64 * apb), lds, st, a,
-                       s->B.as<float>(), s->norm.as<double>(), s->nn.as<double>());
+                       s->B.as<float>(), s->norm.as<double>(), s->nn.as<double>(),
+                       s->Bh.as<__bf16>(), s->Bl.as<__bf16>(), (int)split_ld(s->nd));
   FM_HIP(hipGetLastError());
 }
 

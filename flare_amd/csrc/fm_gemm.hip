@@ -15,10 +15,13 @@
 //
 // FP64 form: v_mfma_f64_16x16x4f64, 64 x 64 tiles. Chosen form: split-BF16
 // x3 on v_mfma_f32_16x16x32_bf16 (a = a_hi + a_lo, ab ~ a_hi b_hi +
-// a_hi b_lo + a_lo b_hi, FP32 accumulation), 128 x 128 tiles; the three
-// products of a k-step run over all 16 accumulators of a wave before the
-// next, so consecutive MFMAs are independent.
+// a_hi b_lo + a_lo b_hi, FP32 accumulation), 128 x 128 tiles, on operands
+// split once where they are produced; the three products of a k-step run
+// over all 16 accumulators of a wave before the next, so consecutive MFMAs
+// are independent.
 #include <hip/hip_runtime.h>
+
+#include <cstring>
 
 #include "fm_device.hpp"
 #include "fm_gemm.hpp"
@@ -57,7 +60,7 @@ __device__ __forceinline__ float fpow(float x, float p, int pi) {
 }
 
 template <typename Acc, typename TW, int TM, int TN>
-__device__ void ep_kernel(const EpArgs &e, const TileOut<Acc, TM, TN> &t, int row0, int col0,
+__device__ __forceinline__ void ep_kernel(const EpArgs &e, const TileOut<Acc, TM, TN> &t, int row0, int col0,
                           int M, int N, int lane, int wr, int wc, double *red) {
   constexpr int BMc = 32 * TM;
   // rows: atoms of this species block, cols: sparse environments.
@@ -85,11 +88,37 @@ __device__ void ep_kernel(const EpArgs &e, const TileOut<Acc, TM, TN> &t, int ro
         Acc kv = empty ? Acc(0) : Acc(e.sig2) * fpow(s, (Acc)e.power, pi);
         Acc pm1 = pi == 2 ? s : (pi == 1 ? Acc(1) : fpow(s, (Acc)(e.power - 1), 0));
         Acc wv = empty ? Acc(0) : Acc(e.sig2) * a * (Acc)e.power * pm1;
-        e.W_as<TW>()[(size_t)r * e.ldw + c] = (TW)wv;
+        if constexpr (sizeof(Acc) == 4) {  // split-BF16 form: w as BF16 pairs
+          float wf = (float)wv;
+          __bf16 h = (__bf16)wf;
+          reinterpret_cast<__bf16 *>(e.Wh)[(size_t)r * e.ldw + c] = h;
+          reinterpret_cast<__bf16 *>(e.Wl)[(size_t)r * e.ldw + c] = (__bf16)(wf - (float)h);
+        } else {
+          e.W_as<TW>()[(size_t)r * e.ldw + c] = (TW)wv;
+        }
         if (e.K) e.K[(size_t)r * e.ldk + c] = (double)kv;
         rs[ti][i] += (double)(kv * a);
       }
     }
+  if constexpr (sizeof(Acc) == 4) {
+    // Padding columns [N, ldw) of the split output are zero: they are the
+    // reduction tail of the next contraction.
+#pragma unroll
+    for (int ti = 0; ti < TM; ti++)
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        int r = row0 + wr * 16 * TM + ti * 16 + tile_row<Acc>(lane, i);
+        if (r >= M) continue;
+#pragma unroll
+        for (int tj = 0; tj < TN; tj++) {
+          int c = col0 + wc * 16 * TN + tj * 16 + (lane & 15);
+          if (c >= N && c < e.ldw) {
+            reinterpret_cast<__bf16 *>(e.Wh)[(size_t)r * e.ldw + c] = (__bf16)0.0f;
+            reinterpret_cast<__bf16 *>(e.Wl)[(size_t)r * e.ldw + c] = (__bf16)0.0f;
+          }
+        }
+      }
+  }
   // Sum over the 16 columns of each lane group, then over the two wave
   // columns through LDS; one partial per (column tile, row).
 #pragma unroll
@@ -116,7 +145,7 @@ __device__ void ep_kernel(const EpArgs &e, const TileOut<Acc, TM, TN> &t, int ro
 }
 
 template <typename Acc, typename TW, int TM, int TN>
-__device__ void ep_store(const EpArgs &e, const TileOut<Acc, TM, TN> &t, int row0, int col0,
+__device__ __forceinline__ void ep_store(const EpArgs &e, const TileOut<Acc, TM, TN> &t, int row0, int col0,
                          int M, int N, int lane, int wr, int wc) {
 #pragma unroll
   for (int ti = 0; ti < TM; ti++)
@@ -133,7 +162,7 @@ __device__ void ep_store(const EpArgs &e, const TileOut<Acc, TM, TN> &t, int row
 }
 
 template <typename Acc, int TM, int TN>
-__device__ void ep_sumsq(const EpArgs &e, const TileOut<Acc, TM, TN> &t, int row0, int col0,
+__device__ __forceinline__ void ep_sumsq(const EpArgs &e, const TileOut<Acc, TM, TN> &t, int row0, int col0,
                          int M, int N, int lane, int wr, int wc, double *red) {
   constexpr int BNc = 32 * TN;
   // rows: L rows, cols: atoms. Column sums of squares over this row tile.
@@ -249,30 +278,25 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K, const dou
 }
 
 // ---------------------------------------------------------------------------
-// Split-BF16 x3 GEMM on FP32 operands (chosen form), 128 x 128 tiles. Each
-// thread loads 16 consecutive k of one row of A and of B per k-step, splits
-// them into hi/lo BF16 and stores them to LDS as 16-byte vectors; LDS rows
-// are k-contiguous so lane l reads 8 consecutive k at row l % 16, k offset
-// 8 (l / 16).
+// Split-BF16 x3 GEMM (chosen form), 128 x 128 tiles, on operands stored as
+// BF16 pairs: x = hi + lo with hi = bf16(x), lo = bf16(x - hi) of the FP32
+// value, split once where the operand is produced (the sparse set at model
+// upload, descriptors in the descriptor kernel, w in the kernel-contraction
+// epilogue). Each thread copies 16 consecutive k of one row of A and of B
+// per k-step as 16-byte vectors into k-contiguous LDS rows; lane l then
+// reads 8 consecutive k at row l % 16, k offset 8 (l / 16).
 // ---------------------------------------------------------------------------
 constexpr int B16_TM = 4, B16_TN = 4, B16_BM = 32 * B16_TM, B16_BN = 32 * B16_TN;
 constexpr int BK16 = 32;
 constexpr int PADB = 8;
 constexpr int LDK = BK16 + PADB;
-
-__device__ __forceinline__ void split8(const float *x, v8bf &hi, v8bf &lo) {
-#pragma unroll
-  for (int q = 0; q < 8; q++) {
-    __bf16 h = (__bf16)x[q];
-    hi[q] = h;
-    lo[q] = (__bf16)(x[q] - (float)h);
-  }
-}
+static_assert(BK16 == kSplitPad, "k-step equals the operand padding");
 
 template <int EP>
-__global__ __launch_bounds__(256) void k_gemm_bf16x3(int M, int N, int K, const float *A,
-                                                     long long lda, const float *B, long long ldb,
-                                                     int lower_tri, EpArgs e) {
+__global__ __launch_bounds__(256) void k_gemm_bf16x3(int M, int N, int Kpad, const __bf16 *Ahg,
+                                                     const __bf16 *Alg, long long lda,
+                                                     const __bf16 *Bhg, const __bf16 *Blg,
+                                                     long long ldb, EpArgs e) {
   __shared__ __align__(16) __bf16 Ah[B16_BM * LDK];
   __shared__ __align__(16) __bf16 Al[B16_BM * LDK];
   __shared__ __align__(16) __bf16 Bh[B16_BN * LDK];
@@ -281,8 +305,6 @@ __global__ __launch_bounds__(256) void k_gemm_bf16x3(int M, int N, int K, const 
   int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   int wr = wid >> 1, wc = wid & 1;
   int row0 = blockIdx.y * B16_BM, col0 = blockIdx.x * B16_BN;
-  int kend = K;
-  if (lower_tri) kend = min(K, row0 + B16_BM);
   v4f acc[B16_TM][B16_TN];
 #pragma unroll
   for (int a = 0; a < B16_TM; a++)
@@ -290,31 +312,37 @@ __global__ __launch_bounds__(256) void k_gemm_bf16x3(int M, int N, int K, const 
     for (int b = 0; b < B16_TN; b++) acc[a][b] = v4f{0, 0, 0, 0};
   // 128 rows x 32 k per operand: thread -> row tid / 2, k half (tid & 1) * 16.
   int lr = tid >> 1, lk = (tid & 1) * 16;
-  float ra[16], rb[16];
+  bool va = row0 + lr < M, vb = col0 + lr < N;
+  const v8bf *pah = reinterpret_cast<const v8bf *>(Ahg + (size_t)(va ? row0 + lr : 0) * lda + lk);
+  const v8bf *pal = reinterpret_cast<const v8bf *>(Alg + (size_t)(va ? row0 + lr : 0) * lda + lk);
+  const v8bf *pbh = reinterpret_cast<const v8bf *>(Bhg + (size_t)(vb ? col0 + lr : 0) * ldb + lk);
+  const v8bf *pbl = reinterpret_cast<const v8bf *>(Blg + (size_t)(vb ? col0 + lr : 0) * ldb + lk);
+  const v8bf zero = {};
+  v8bf ra[4], rb[4];
   auto gload = [&](int k0) {
-    int r = row0 + lr, c = col0 + lr;
-#pragma unroll
-    for (int q = 0; q < 16; q++) {
-      int k = k0 + lk + q;
-      ra[q] = (r < M && k < kend) ? A[(size_t)r * lda + k] : 0.0f;
-      rb[q] = (c < N && k < kend) ? B[(size_t)c * ldb + k] : 0.0f;
-    }
+    int q = k0 / 8;
+    ra[0] = va ? pah[q] : zero;
+    ra[1] = va ? pah[q + 1] : zero;
+    ra[2] = va ? pal[q] : zero;
+    ra[3] = va ? pal[q + 1] : zero;
+    rb[0] = vb ? pbh[q] : zero;
+    rb[1] = vb ? pbh[q + 1] : zero;
+    rb[2] = vb ? pbl[q] : zero;
+    rb[3] = vb ? pbl[q + 1] : zero;
   };
   gload(0);
-  for (int k0 = 0; k0 < kend; k0 += BK16) {
+  for (int k0 = 0; k0 < Kpad; k0 += BK16) {
     __syncthreads();
-#pragma unroll
-    for (int h = 0; h < 2; h++) {
-      v8bf hi, lo;
-      split8(ra + 8 * h, hi, lo);
-      *reinterpret_cast<v8bf *>(&Ah[lr * LDK + lk + 8 * h]) = hi;
-      *reinterpret_cast<v8bf *>(&Al[lr * LDK + lk + 8 * h]) = lo;
-      split8(rb + 8 * h, hi, lo);
-      *reinterpret_cast<v8bf *>(&Bh[lr * LDK + lk + 8 * h]) = hi;
-      *reinterpret_cast<v8bf *>(&Bl[lr * LDK + lk + 8 * h]) = lo;
-    }
+    *reinterpret_cast<v8bf *>(&Ah[lr * LDK + lk]) = ra[0];
+    *reinterpret_cast<v8bf *>(&Ah[lr * LDK + lk + 8]) = ra[1];
+    *reinterpret_cast<v8bf *>(&Al[lr * LDK + lk]) = ra[2];
+    *reinterpret_cast<v8bf *>(&Al[lr * LDK + lk + 8]) = ra[3];
+    *reinterpret_cast<v8bf *>(&Bh[lr * LDK + lk]) = rb[0];
+    *reinterpret_cast<v8bf *>(&Bh[lr * LDK + lk + 8]) = rb[1];
+    *reinterpret_cast<v8bf *>(&Bl[lr * LDK + lk]) = rb[2];
+    *reinterpret_cast<v8bf *>(&Bl[lr * LDK + lk + 8]) = rb[3];
     __syncthreads();
-    if (k0 + BK16 < kend) gload(k0 + BK16);
+    if (k0 + BK16 < Kpad) gload(k0 + BK16);
     v8bf ah[B16_TM], al[B16_TM], bh[B16_TN], bl[B16_TN];
     int kk = 8 * (lane >> 4);
 #pragma unroll
@@ -381,20 +409,46 @@ void gemm_f64(hipStream_t st, int ep, int M, int N, int K, const double *A, long
   FM_HIP(hipGetLastError());
 }
 
-void gemm_bf16x3(hipStream_t st, int ep, int M, int N, int K, const float *A, long long lda,
-                 const float *B, long long ldb, int lower_tri, const EpArgs &e) {
+void gemm_bf16x3(hipStream_t st, int ep, int M, int N, int Kpad, const uint16_t *Ah,
+                 const uint16_t *Al, long long lda, const uint16_t *Bh, const uint16_t *Bl,
+                 long long ldb, const EpArgs &e) {
   if (M <= 0 || N <= 0) return;
+  FM_CHECK(Kpad % kSplitPad == 0 && lda % kSplitPad == 0 && ldb % kSplitPad == 0 &&
+               Kpad <= lda && Kpad <= ldb,
+           FM_ERR_INVALID, "split operands must be padded to the k-step");
+  FM_CHECK(((uintptr_t)Ah | (uintptr_t)Al | (uintptr_t)Bh | (uintptr_t)Bl) % 16 == 0,
+           FM_ERR_INVALID, "split operands must be 16-byte aligned");
   dim3 grid((N + B16_BN - 1) / B16_BN, (M + B16_BM - 1) / B16_BM);
+  auto A1 = reinterpret_cast<const __bf16 *>(Ah), A2 = reinterpret_cast<const __bf16 *>(Al);
+  auto B1 = reinterpret_cast<const __bf16 *>(Bh), B2 = reinterpret_cast<const __bf16 *>(Bl);
   if (ep == EP_KERNEL)
-    hipLaunchKernelGGL(k_gemm_bf16x3<EP_KERNEL>, grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb,
-                       lower_tri, e);
+    hipLaunchKernelGGL(k_gemm_bf16x3<EP_KERNEL>, grid, dim3(256), 0, st, M, N, Kpad, A1, A2, lda,
+                       B1, B2, ldb, e);
   else if (ep == EP_STORE)
-    hipLaunchKernelGGL(k_gemm_bf16x3<EP_STORE>, grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb,
-                       lower_tri, e);
+    hipLaunchKernelGGL(k_gemm_bf16x3<EP_STORE>, grid, dim3(256), 0, st, M, N, Kpad, A1, A2, lda,
+                       B1, B2, ldb, e);
   else
-    hipLaunchKernelGGL(k_gemm_bf16x3<EP_SUMSQ>, grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb,
-                       lower_tri, e);
+    hipLaunchKernelGGL(k_gemm_bf16x3<EP_SUMSQ>, grid, dim3(256), 0, st, M, N, Kpad, A1, A2, lda,
+                       B1, B2, ldb, e);
   FM_HIP(hipGetLastError());
+}
+
+static uint16_t bf16_rne(float x) {
+  uint32_t u;
+  std::memcpy(&u, &x, 4);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+void split_bf16_host(const float *x, size_t n, uint16_t *hi, uint16_t *lo) {
+  for (size_t k = 0; k < n; k++) {
+    uint16_t h = bf16_rne(x[k]);
+    uint32_t hb = (uint32_t)h << 16;
+    float hf;
+    std::memcpy(&hf, &hb, 4);
+    hi[k] = h;
+    lo[k] = bf16_rne(x[k] - hf);
+  }
 }
 
 }  // namespace fm

@@ -2,6 +2,9 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
+#include <cstdint>
+
 namespace fm {
 
 enum { EP_KERNEL = 0, EP_STORE = 1, EP_SUMSQ = 2 };
@@ -13,8 +16,12 @@ struct EpArgs {
   const double *alpha = nullptr;     // alpha_j per column (sparse env)
   double *K = nullptr;               // optional k matrix (rows x ldk), FP64
   long long ldk = 0;
-  // EP_KERNEL writes w, EP_STORE writes C, into W (rows x ldw).
+  // EP_KERNEL writes w, EP_STORE writes C, into W (rows x ldw). In the
+  // split-BF16 contraction EP_KERNEL instead writes w as the BF16 pair
+  // (hi, lo) of its FP32 value into Wh / Wl (rows x ldw), zero in the
+  // padding columns [N, ldw).
   void *W = nullptr;
+  void *Wh = nullptr, *Wl = nullptr;
   long long ldw = 0;
   // EP_KERNEL: energy partials [column tile][row]; EP_SUMSQ: [row tile][col].
   double *part = nullptr;
@@ -33,7 +40,17 @@ int gemm_row_tiles(int form, int M);
 
 void gemm_f64(hipStream_t st, int ep, int M, int N, int K, const double *A, long long lda,
               const double *B, long long ldb, int lower_tri, const EpArgs &e);
-void gemm_bf16x3(hipStream_t st, int ep, int M, int N, int K, const float *A, long long lda,
-                 const float *B, long long ldb, int lower_tri, const EpArgs &e);
+// Split-BF16 x3 on operands stored as BF16 pairs (hi, lo) of FP32 values:
+// rows of A and B are K-contiguous with leading dimensions lda / ldb that are
+// multiples of kSplitPad, padded with zeros up to Kpad (a multiple of
+// kSplitPad), and 16-byte aligned.
+constexpr int kSplitPad = 32;
+inline long long split_ld(long long k) { return (k + kSplitPad - 1) / kSplitPad * kSplitPad; }
+void gemm_bf16x3(hipStream_t st, int ep, int M, int N, int Kpad, const uint16_t *Ah,
+                 const uint16_t *Al, long long lda, const uint16_t *Bh, const uint16_t *Bl,
+                 long long ldb, const EpArgs &e);
+// Host-side split of FP32 values into BF16 pairs, rounding to nearest even
+// as the device conversion does: hi = bf16(x), lo = bf16(x - hi).
+void split_bf16_host(const float *x, size_t n, uint16_t *hi, uint16_t *lo);
 
 }  // namespace fm

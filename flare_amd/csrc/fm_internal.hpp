@@ -152,6 +152,15 @@ struct fm_structure_s {
   std::vector<int> n_by_type, nb_by_type, type_off;
   fm::DevBuf d_cutoffs, bond_off, bond_q, in_off, in_bond, row_of_atom,
       atom_of_row, pair_table, B, norm, nn;
+  // Chosen form: the descriptors also as BF16 pairs (n x split_ld(d)).
+  fm::DevBuf Bh, Bl;
+  // Dense descriptor derivatives (FP64, single structure): bonds in the
+  // species-sorted order of DescriptorValues, f in [sb_off[row], sb_off[row+1])
+  // for species-sorted row `row`; bonds of species t are [nbt_off[t],
+  // nbt_off[t+1]). DB is 3 n_bonds x d (rows 3 f + component), fdots 3 n_bonds.
+  bool has_dervs = false;
+  std::vector<int> nbt_off;
+  fm::DevBuf sb_off, sb_pos, sb_row, sb_bond, DB, fdots;
 };
 
 struct fm_model_s {
@@ -162,11 +171,21 @@ struct fm_model_s {
   bool has_L = false;
   std::vector<int> L_rows;   // rows of the L_inv column block of each species
   std::vector<int> L_lower;  // 1 when that block is lower triangular
-  // Per species: X-hat (M_s x d) and X-hat^T (d x M_s) in FP64 and FP32,
-  // alpha (zero for empty sparse environments) and the L_inv block
-  // (L_rows x M_s, starting at the species' first row).
-  std::vector<fm::DevBuf> X, XT, X32, XT32, alpha, L;
+  // Per species: X-hat (M_s x d) and X-hat^T (d x M_s) in FP64 and as
+  // BF16 pairs of their FP32 values (Xh/Xl: M_s x split_ld(d), XTh/XTl:
+  // d x split_ld(M_s), zero padded), alpha (zero for empty sparse
+  // environments) and the L_inv block (L_rows x M_s, starting at the
+  // species' first row).
+  std::vector<fm::DevBuf> X, XT, Xh, Xl, XTh, XTl, alpha, L;
   std::vector<double> host_alpha;
+  // Dense (DTC / SOR) predictions: sparse norms per species, alpha in one
+  // vector, R_inv^T (M x M) and whether it is lower triangular, and whether
+  // L_inv is block diagonal over species.
+  std::vector<std::vector<double>> xnorm;
+  fm::DevBuf alpha_full, RinvT;
+  bool has_R = false;
+  int R_lower = 0;
+  bool L_block_diag = true;
 };
 
 struct fm_mapped_s {
@@ -193,6 +212,13 @@ void launch_adjoint(fm_structure s, const void *G, int64_t g_ld, void *fbond,
 // fm_predict.cpp
 void predict(fm_model m, fm_structure s, int flags, int form, double *mean_efs,
              double *local_var, double *local_energy);
+// fm_dtc.hip
+void compute_dervs(fm_structure s);
+void envs_struc(fm_model m, fm_structure s, double *K_out);
+void self_kernel_struc(fm_model m, fm_structure s, double *kself_out);
+void predict_dense(fm_model m, fm_structure s, int kind, double *mean_efs, double *var_efs);
+// fm_descriptors.hip: normalisation constants of the real spherical harmonics
+void knorm_values(int lmax, double *out);
 // scans (fm_neighbors.hip)
 int scan_exclusive(fm_context ctx, const int *in, int *out, int n, bool want_total);
 }  // namespace fm

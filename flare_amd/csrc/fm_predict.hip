@@ -122,15 +122,23 @@ void predict(fm_model m, fm_structure s, int flags, int form, double *mean_efs,
   size_t n_out = (size_t)s->mean_off[s->n_struct];
   // Per-species slices of the atom-major w and k matrices (n_s x M_s each)
   // and of the energy / variance partials.
-  std::vector<size_t> wo(nos + 1, 0), eo(nos + 1, 0), qo(nos + 1, 0);
+  // The chosen form keeps w as BF16 pairs with padded rows (ws = split_ld).
+  std::vector<size_t> wo(nos + 1, 0), eo(nos + 1, 0), qo(nos + 1, 0), so(nos + 1, 0);
   for (int t = 0; t < nos; t++) {
     size_t ns = s->n_by_type[t], Ms = m->M_by_type[t];
     wo[t + 1] = wo[t] + ns * Ms;
+    so[t + 1] = so[t] + ns * (size_t)split_ld((long long)Ms);
     eo[t + 1] = eo[t] + ns * (size_t)std::max(1, gemm_col_tiles(form, (int)Ms));
     int Lr = want_var ? m->L_rows[t] : 0;
     qo[t + 1] = qo[t] + ns * (size_t)gemm_row_tiles(FM_FORM_FP64, Lr);
   }
-  DevBuf W(pool, std::max<size_t>(1, wo[nos] * esz));
+  DevBuf W, Wh, Wl;
+  if (f64) {
+    W.alloc(pool, std::max<size_t>(1, wo[nos] * esz));
+  } else {
+    Wh.alloc(pool, std::max<size_t>(1, so[nos] * sizeof(uint16_t)));
+    Wl.alloc(pool, std::max<size_t>(1, so[nos] * sizeof(uint16_t)));
+  }
   DevBuf Kb;
   if (want_var) Kb.alloc(pool, std::max<size_t>(1, wo[nos] * sizeof(double)));
   DevBuf G(pool, std::max<size_t>(1, (size_t)n * nd * esz));
@@ -156,8 +164,14 @@ void predict(fm_model m, fm_structure s, int flags, int form, double *mean_efs,
     ek.power = m->power;
     ek.row_norm = s->norm.as<double>() + ro;
     ek.alpha = m->alpha[t].as<double>();
-    ek.W = (char *)W.as<char>() + wo[t] * esz;
-    ek.ldw = Ms;
+    if (f64) {
+      ek.W = W.as<double>() + wo[t];
+      ek.ldw = Ms;
+    } else {
+      ek.Wh = Wh.as<uint16_t>() + so[t];
+      ek.Wl = Wl.as<uint16_t>() + so[t];
+      ek.ldw = split_ld(Ms);
+    }
     if (want_var) {
       ek.K = Kb.as<double>() + wo[t];
       ek.ldk = Ms;
@@ -167,9 +181,12 @@ void predict(fm_model m, fm_structure s, int flags, int form, double *mean_efs,
     if (f64)
       gemm_f64(st, EP_KERNEL, ns, Ms, nd, s->B.as<double>() + (size_t)ro * nd, nd,
                m->X[t].as<double>(), nd, 0, ek);
-    else
-      gemm_bf16x3(st, EP_KERNEL, ns, Ms, nd, s->B.as<float>() + (size_t)ro * nd, nd,
-                  m->X32[t].as<float>(), nd, 0, ek);
+    else {
+      long long ldd = split_ld(nd);
+      gemm_bf16x3(st, EP_KERNEL, ns, Ms, (int)ldd, s->Bh.as<uint16_t>() + (size_t)ro * ldd,
+                  s->Bl.as<uint16_t>() + (size_t)ro * ldd, ldd, m->Xh[t].as<uint16_t>(),
+                  m->Xl[t].as<uint16_t>(), ldd, ek);
+    }
     ctx->launches[ST_SGEMM]++;
   }
   ctx->mark_end(ST_SGEMM);
@@ -190,9 +207,12 @@ void predict(fm_model m, fm_structure s, int flags, int form, double *mean_efs,
     if (f64)
       gemm_f64(st, EP_STORE, ns, nd, Ms, W.as<double>() + wo[t], Ms, m->XT[t].as<double>(), Ms,
                0, eg);
-    else
-      gemm_bf16x3(st, EP_STORE, ns, nd, Ms, W.as<float>() + wo[t], Ms, m->XT32[t].as<float>(),
-                  Ms, 0, eg);
+    else {
+      long long ldm = split_ld(Ms);
+      gemm_bf16x3(st, EP_STORE, ns, nd, (int)ldm, Wh.as<uint16_t>() + so[t],
+                  Wl.as<uint16_t>() + so[t], ldm, m->XTh[t].as<uint16_t>(),
+                  m->XTl[t].as<uint16_t>(), ldm, eg);
+    }
     ctx->launches[ST_GGEMM]++;
   }
   ctx->mark_end(ST_GGEMM);

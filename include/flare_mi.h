@@ -147,6 +147,7 @@ typedef struct {
   const double *sparse_norms;       /* M */
   const double *alpha;              /* M */
   const double *L_inv;              /* M x M or NULL */
+  const double *R_inv;              /* M x M or NULL (DTC / SOR variances) */
 } fm_model_desc;
 
 int fm_model_create(fm_context ctx, const fm_model_desc *desc, fm_model *out);
@@ -163,6 +164,34 @@ int fm_model_destroy(fm_model m);
  *   local_energy N, atom order (this kernel's per-atom energies). */
 int fm_predict(fm_model m, fm_structure s, int flags, int form,
                double *mean_efs, double *local_var, double *local_energy);
+
+/* Dense descriptor derivatives of one species (DescriptorValues
+ * descriptor_force_dervs and descriptor_force_dots, descriptor.h:37-43,
+ * b2.cpp:181-242): dB_i/dr_bond for every bond of the species' atoms in the
+ * order of fm_structure_get_b2's neighbour lists, rows 3 bond + component
+ * (3 nb_s x d), and their dots with B_i (3 nb_s). FP64 descriptors of a
+ * single structure only; built on the first call and kept with the
+ * structure. Either pointer may be NULL. */
+int fm_structure_get_b2_dervs(fm_structure s, int species, double *force_dervs,
+                              double *force_dots);
+
+/* NormalizedDotProduct::envs_struc (normalized_dot_product.cpp:117-214):
+ * the M x (1 + 3N + 6) kernel matrix between the sparse set and the
+ * energy, forces and stress of a single structure (FP64 descriptors). */
+int fm_envs_struc(fm_model m, fm_structure s, double *K);
+/* NormalizedDotProduct::self_kernel_struc (normalized_dot_product.cpp:
+ * 450-586): the 1 + 3N + 6 diagonal of the structure's own kernel. */
+int fm_self_kernel_struc(fm_model m, fm_structure s, double *kself);
+
+/* SparseGP::predict_DTC / predict_SOR (sparse_gp.cpp:667-729) for a model
+ * with one kernel: mean_efs = K^T alpha and var_efs (1 + 3N + 6) =
+ * K_self - diag(K^T Kuu^-1 K) + diag(K^T Sigma K) (DTC) or
+ * diag(K^T R_inv R_inv^T K) (SOR), with Kuu^-1 = L_inv^T L_inv and
+ * Sigma = R_inv R_inv^T. Needs R_inv in the model (and L_inv for DTC). */
+#define FM_DENSE_DTC 1
+#define FM_DENSE_SOR 2
+int fm_predict_dense(fm_model m, fm_structure s, int kind, double *mean_efs,
+                     double *var_efs);
 
 /* Mapping coefficients of a power-1 or power-2 model
  * (NormalizedDotProduct::compute_mapping_coefficients,
