@@ -450,13 +450,12 @@ int fm_structure_compute_b2(fm_structure s, const fm_b2_settings *st_in, int for
     }
     size_t esz = form == FM_FORM_FP64 ? sizeof(double) : sizeof(float);
     s->B.alloc(pool, std::max<size_t>(1, (size_t)s->n * s->nd * esz));
-    if (form == FM_FORM_CHOSEN) {
-      size_t split_bytes = (size_t)s->n * fm::split_ld(s->nd) * sizeof(uint16_t);
-      s->Bh.alloc(pool, std::max<size_t>(1, split_bytes));
-      s->Bl.alloc(pool, std::max<size_t>(1, split_bytes));
-    } else {
-      s->Bh.release();
-      s->Bl.release();
+    size_t split_bytes = (size_t)s->n * fm::split_ld(s->nd) * sizeof(uint16_t);
+    for (auto &piece : s->Bs) {
+      if (form == FM_FORM_CHOSEN)
+        piece.alloc(pool, std::max<size_t>(1, split_bytes));
+      else
+        piece.release();
     }
     s->norm.alloc(pool, std::max<size_t>(1, (size_t)s->n * sizeof(double)));
     s->nn.alloc(pool, std::max<size_t>(1, (size_t)s->n * sizeof(double)));
@@ -628,27 +627,31 @@ int fm_model_create(fm_context ctx, const fm_model_desc *d, fm_model *out) {
         for (int k = 0; k < nd; k++) X32[j * ldd + k] = (float)X[(size_t)j * nd + k];
       for (int k = 0; k < nd; k++)
         for (int j = 0; j < Ms; j++) XT32[k * ldm + j] = (float)XT[(size_t)k * Ms + j];
-      std::vector<uint16_t> xh(X32.size()), xl(X32.size()), xth(XT32.size()), xtl(XT32.size());
-      fm::split_bf16_host(X32.data(), X32.size(), xh.data(), xl.data());
-      fm::split_bf16_host(XT32.data(), XT32.size(), xth.data(), xtl.data());
+      std::vector<uint16_t> xs[2], xts[2];
+      for (auto &v : xs) v.resize(X32.size());
+      for (auto &v : xts) v.resize(XT32.size());
+      uint16_t *xsp[2] = {xs[0].data(), xs[1].data()};
+      uint16_t *xtsp[2] = {xts[0].data(), xts[1].data()};
+      fm::split_bf16_host(X32.data(), X32.size(), 2, xsp);
+      fm::split_bf16_host(XT32.data(), XT32.size(), 2, xtsp);
       auto up = [&](fm::DevBuf &buf, const void *src, size_t bytes) {
         buf.alloc(pool, std::max<size_t>(1, bytes));
         if (bytes) FM_HIP(hipMemcpyAsync(buf.as<void>(), src, bytes, hipMemcpyHostToDevice, st));
       };
       m->X.emplace_back();
       m->XT.emplace_back();
-      m->Xh.emplace_back();
-      m->Xl.emplace_back();
-      m->XTh.emplace_back();
-      m->XTl.emplace_back();
+      m->Xs0.emplace_back();
+      m->Xs1.emplace_back();
+      m->XTs0.emplace_back();
+      m->XTs1.emplace_back();
       m->alpha.emplace_back();
       m->L.emplace_back();
       up(m->X.back(), X.data(), X.size() * sizeof(double));
       up(m->XT.back(), XT.data(), XT.size() * sizeof(double));
-      up(m->Xh.back(), xh.data(), xh.size() * sizeof(uint16_t));
-      up(m->Xl.back(), xl.data(), xl.size() * sizeof(uint16_t));
-      up(m->XTh.back(), xth.data(), xth.size() * sizeof(uint16_t));
-      up(m->XTl.back(), xtl.data(), xtl.size() * sizeof(uint16_t));
+      up(m->Xs0.back(), xs[0].data(), xs[0].size() * sizeof(uint16_t));
+      up(m->Xs1.back(), xs[1].data(), xs[1].size() * sizeof(uint16_t));
+      up(m->XTs0.back(), xts[0].data(), xts[0].size() * sizeof(uint16_t));
+      up(m->XTs1.back(), xts[1].data(), xts[1].size() * sizeof(uint16_t));
       up(m->alpha.back(), a.data(), a.size() * sizeof(double));
       int rows = 0, lower = 1;
       if (m->has_L && Ms > 0) {

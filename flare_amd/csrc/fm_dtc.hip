@@ -378,11 +378,10 @@ __global__ void k_var_efs(int n_out, int dtc, const double *kself, const double 
 // ---------------------------------------------------------------------------
 // Host orchestration
 // ---------------------------------------------------------------------------
-static void upload_knorm_d(hipStream_t st, int lmax) {
+static void upload_knorm_d(hipStream_t st, int) {
   static thread_local double host[kMaxH];
-  knorm_values(lmax, host);
-  FM_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_knorm_d), host,
-                                sizeof(double) * (lmax + 1) * (lmax + 1), 0,
+  knorm_values(kMaxL, host);
+  FM_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_knorm_d), host, sizeof(host), 0,
                                 hipMemcpyHostToDevice, st));
 }
 

@@ -279,12 +279,13 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K, const dou
 
 // ---------------------------------------------------------------------------
 // Split-BF16 x3 GEMM (chosen form), 128 x 128 tiles, on operands stored as
-// BF16 pairs: x = hi + lo with hi = bf16(x), lo = bf16(x - hi) of the FP32
-// value, split once where the operand is produced (the sparse set at model
-// upload, descriptors in the descriptor kernel, w in the kernel-contraction
-// epilogue). Each thread copies 16 consecutive k of one row of A and of B
-// per k-step as 16-byte vectors into k-contiguous LDS rows; lane l then
-// reads 8 consecutive k at row l % 16, k offset 8 (l / 16).
+// two BF16 pieces of FP32 values (x = x_0 + x_1, split once where the operand
+// is produced: the sparse set at model upload, descriptors in the descriptor
+// kernel, w in the kernel-contraction epilogue). The products x_1 y_0,
+// x_0 y_1 and x_0 y_0 are summed in that order (smallest first). Each thread copies
+// 16 consecutive k of one row of A and of B per piece and k-step as 16-byte
+// vectors into k-contiguous LDS rows; lane l then reads 8 consecutive k at
+// row l % 16, k offset 8 (l / 16).
 // ---------------------------------------------------------------------------
 constexpr int B16_TM = 4, B16_TN = 4, B16_BM = 32 * B16_TM, B16_BN = 32 * B16_TN;
 constexpr int BK16 = 32;
@@ -292,15 +293,16 @@ constexpr int PADB = 8;
 constexpr int LDK = BK16 + PADB;
 static_assert(BK16 == kSplitPad, "k-step equals the operand padding");
 
-template <int EP>
-__global__ __launch_bounds__(256) void k_gemm_bf16x3(int M, int N, int Kpad, const __bf16 *Ahg,
-                                                     const __bf16 *Alg, long long lda,
-                                                     const __bf16 *Bhg, const __bf16 *Blg,
-                                                     long long ldb, EpArgs e) {
-  __shared__ __align__(16) __bf16 Ah[B16_BM * LDK];
-  __shared__ __align__(16) __bf16 Al[B16_BM * LDK];
-  __shared__ __align__(16) __bf16 Bh[B16_BN * LDK];
-  __shared__ __align__(16) __bf16 Bl[B16_BN * LDK];
+struct SplitPtrs {
+  const __bf16 *p[3];
+};
+
+template <int EP, int NP>
+__global__ __launch_bounds__(256) void k_gemm_split(int M, int N, int Kpad, SplitPtrs Ag,
+                                                    long long lda, SplitPtrs Bg, long long ldb,
+                                                    EpArgs e) {
+  __shared__ __align__(16) __bf16 As[NP][B16_BM * LDK];
+  __shared__ __align__(16) __bf16 Bs[NP][B16_BN * LDK];
   __shared__ double red[2 * (B16_BM > B16_BN ? B16_BM : B16_BN)];
   int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   int wr = wid >> 1, wc = wid & 1;
@@ -310,68 +312,65 @@ __global__ __launch_bounds__(256) void k_gemm_bf16x3(int M, int N, int Kpad, con
   for (int a = 0; a < B16_TM; a++)
 #pragma unroll
     for (int b = 0; b < B16_TN; b++) acc[a][b] = v4f{0, 0, 0, 0};
-  // 128 rows x 32 k per operand: thread -> row tid / 2, k half (tid & 1) * 16.
+  // 128 rows x 32 k per operand and piece: thread -> row tid / 2, k half
+  // (tid & 1) * 16. Rows past M or N read row 0 (in bounds) and load zeros.
   int lr = tid >> 1, lk = (tid & 1) * 16;
   bool va = row0 + lr < M, vb = col0 + lr < N;
-  const v8bf *pah = reinterpret_cast<const v8bf *>(Ahg + (size_t)(va ? row0 + lr : 0) * lda + lk);
-  const v8bf *pal = reinterpret_cast<const v8bf *>(Alg + (size_t)(va ? row0 + lr : 0) * lda + lk);
-  const v8bf *pbh = reinterpret_cast<const v8bf *>(Bhg + (size_t)(vb ? col0 + lr : 0) * ldb + lk);
-  const v8bf *pbl = reinterpret_cast<const v8bf *>(Blg + (size_t)(vb ? col0 + lr : 0) * ldb + lk);
+  size_t oa = (size_t)(va ? row0 + lr : 0) * lda + lk;
+  size_t ob = (size_t)(vb ? col0 + lr : 0) * ldb + lk;
   const v8bf zero = {};
-  v8bf ra[4], rb[4];
+  v8bf ra[NP][2], rb[NP][2];
   auto gload = [&](int k0) {
     int q = k0 / 8;
-    ra[0] = va ? pah[q] : zero;
-    ra[1] = va ? pah[q + 1] : zero;
-    ra[2] = va ? pal[q] : zero;
-    ra[3] = va ? pal[q + 1] : zero;
-    rb[0] = vb ? pbh[q] : zero;
-    rb[1] = vb ? pbh[q + 1] : zero;
-    rb[2] = vb ? pbl[q] : zero;
-    rb[3] = vb ? pbl[q + 1] : zero;
+#pragma unroll
+    for (int pc = 0; pc < NP; pc++) {
+      const v8bf *pa = reinterpret_cast<const v8bf *>(Ag.p[pc] + oa);
+      const v8bf *pb = reinterpret_cast<const v8bf *>(Bg.p[pc] + ob);
+      ra[pc][0] = va ? pa[q] : zero;
+      ra[pc][1] = va ? pa[q + 1] : zero;
+      rb[pc][0] = vb ? pb[q] : zero;
+      rb[pc][1] = vb ? pb[q + 1] : zero;
+    }
   };
   gload(0);
   for (int k0 = 0; k0 < Kpad; k0 += BK16) {
     __syncthreads();
-    *reinterpret_cast<v8bf *>(&Ah[lr * LDK + lk]) = ra[0];
-    *reinterpret_cast<v8bf *>(&Ah[lr * LDK + lk + 8]) = ra[1];
-    *reinterpret_cast<v8bf *>(&Al[lr * LDK + lk]) = ra[2];
-    *reinterpret_cast<v8bf *>(&Al[lr * LDK + lk + 8]) = ra[3];
-    *reinterpret_cast<v8bf *>(&Bh[lr * LDK + lk]) = rb[0];
-    *reinterpret_cast<v8bf *>(&Bh[lr * LDK + lk + 8]) = rb[1];
-    *reinterpret_cast<v8bf *>(&Bl[lr * LDK + lk]) = rb[2];
-    *reinterpret_cast<v8bf *>(&Bl[lr * LDK + lk + 8]) = rb[3];
+#pragma unroll
+    for (int pc = 0; pc < NP; pc++) {
+      *reinterpret_cast<v8bf *>(&As[pc][lr * LDK + lk]) = ra[pc][0];
+      *reinterpret_cast<v8bf *>(&As[pc][lr * LDK + lk + 8]) = ra[pc][1];
+      *reinterpret_cast<v8bf *>(&Bs[pc][lr * LDK + lk]) = rb[pc][0];
+      *reinterpret_cast<v8bf *>(&Bs[pc][lr * LDK + lk + 8]) = rb[pc][1];
+    }
     __syncthreads();
     if (k0 + BK16 < Kpad) gload(k0 + BK16);
-    v8bf ah[B16_TM], al[B16_TM], bh[B16_TN], bl[B16_TN];
+    v8bf af[NP][B16_TM], bf[NP][B16_TN];
     int kk = 8 * (lane >> 4);
 #pragma unroll
-    for (int t = 0; t < B16_TM; t++) {
-      int ar = wr * 16 * B16_TM + t * 16 + (lane & 15);
-      ah[t] = *reinterpret_cast<const v8bf *>(&Ah[ar * LDK + kk]);
-      al[t] = *reinterpret_cast<const v8bf *>(&Al[ar * LDK + kk]);
+    for (int pc = 0; pc < NP; pc++) {
+#pragma unroll
+      for (int t = 0; t < B16_TM; t++) {
+        int ar = wr * 16 * B16_TM + t * 16 + (lane & 15);
+        af[pc][t] = *reinterpret_cast<const v8bf *>(&As[pc][ar * LDK + kk]);
+      }
+#pragma unroll
+      for (int t = 0; t < B16_TN; t++) {
+        int br = wc * 16 * B16_TN + t * 16 + (lane & 15);
+        bf[pc][t] = *reinterpret_cast<const v8bf *>(&Bs[pc][br * LDK + kk]);
+      }
     }
+    // Products of piece orders i + j <= NP - 1, smallest terms first.
+    auto prod = [&](const v8bf (&x)[B16_TM], const v8bf (&y)[B16_TN]) {
 #pragma unroll
-    for (int t = 0; t < B16_TN; t++) {
-      int br = wc * 16 * B16_TN + t * 16 + (lane & 15);
-      bh[t] = *reinterpret_cast<const v8bf *>(&Bh[br * LDK + kk]);
-      bl[t] = *reinterpret_cast<const v8bf *>(&Bl[br * LDK + kk]);
-    }
+      for (int a = 0; a < B16_TM; a++)
 #pragma unroll
-    for (int a = 0; a < B16_TM; a++)
-#pragma unroll
-      for (int b = 0; b < B16_TN; b++)
-        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
-#pragma unroll
-    for (int a = 0; a < B16_TM; a++)
-#pragma unroll
-      for (int b = 0; b < B16_TN; b++)
-        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
-#pragma unroll
-    for (int a = 0; a < B16_TM; a++)
-#pragma unroll
-      for (int b = 0; b < B16_TN; b++)
-        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
+        for (int b = 0; b < B16_TN; b++)
+          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(x[a], y[b], acc[a][b], 0, 0, 0);
+    };
+    static_assert(NP == 2, "two pieces: products lo.hi, hi.lo, hi.hi");
+    prod(af[1], bf[0]);
+    prod(af[0], bf[1]);
+    prod(af[0], bf[0]);
   }
   TileOut<float, B16_TM, B16_TN> t;
 #pragma unroll
@@ -409,27 +408,38 @@ void gemm_f64(hipStream_t st, int ep, int M, int N, int K, const double *A, long
   FM_HIP(hipGetLastError());
 }
 
-void gemm_bf16x3(hipStream_t st, int ep, int M, int N, int Kpad, const uint16_t *Ah,
-                 const uint16_t *Al, long long lda, const uint16_t *Bh, const uint16_t *Bl,
-                 long long ldb, const EpArgs &e) {
-  if (M <= 0 || N <= 0) return;
-  FM_CHECK(Kpad % kSplitPad == 0 && lda % kSplitPad == 0 && ldb % kSplitPad == 0 &&
-               Kpad <= lda && Kpad <= ldb,
-           FM_ERR_INVALID, "split operands must be padded to the k-step");
-  FM_CHECK(((uintptr_t)Ah | (uintptr_t)Al | (uintptr_t)Bh | (uintptr_t)Bl) % 16 == 0,
-           FM_ERR_INVALID, "split operands must be 16-byte aligned");
-  dim3 grid((N + B16_BN - 1) / B16_BN, (M + B16_BM - 1) / B16_BM);
-  auto A1 = reinterpret_cast<const __bf16 *>(Ah), A2 = reinterpret_cast<const __bf16 *>(Al);
-  auto B1 = reinterpret_cast<const __bf16 *>(Bh), B2 = reinterpret_cast<const __bf16 *>(Bl);
+template <int NP>
+static void launch_split(hipStream_t st, int ep, dim3 grid, int M, int N, int Kpad,
+                         const SplitPtrs &a, long long lda, const SplitPtrs &b, long long ldb,
+                         const EpArgs &e) {
   if (ep == EP_KERNEL)
-    hipLaunchKernelGGL(k_gemm_bf16x3<EP_KERNEL>, grid, dim3(256), 0, st, M, N, Kpad, A1, A2, lda,
-                       B1, B2, ldb, e);
+    hipLaunchKernelGGL((k_gemm_split<EP_KERNEL, NP>), grid, dim3(256), 0, st, M, N, Kpad, a, lda,
+                       b, ldb, e);
   else if (ep == EP_STORE)
-    hipLaunchKernelGGL(k_gemm_bf16x3<EP_STORE>, grid, dim3(256), 0, st, M, N, Kpad, A1, A2, lda,
-                       B1, B2, ldb, e);
+    hipLaunchKernelGGL((k_gemm_split<EP_STORE, NP>), grid, dim3(256), 0, st, M, N, Kpad, a, lda,
+                       b, ldb, e);
   else
-    hipLaunchKernelGGL(k_gemm_bf16x3<EP_SUMSQ>, grid, dim3(256), 0, st, M, N, Kpad, A1, A2, lda,
-                       B1, B2, ldb, e);
+    hipLaunchKernelGGL((k_gemm_split<EP_SUMSQ, NP>), grid, dim3(256), 0, st, M, N, Kpad, a, lda,
+                       b, ldb, e);
+}
+
+void gemm_split(hipStream_t st, int ep, int np, int M, int N, int Kpad, const SplitOperand &A,
+                const SplitOperand &B, const EpArgs &e) {
+  if (M <= 0 || N <= 0) return;
+  FM_CHECK(np == 2, FM_ERR_INVALID, "split operands have 2 pieces");
+  FM_CHECK(Kpad % kSplitPad == 0 && A.ld % kSplitPad == 0 && B.ld % kSplitPad == 0 &&
+               Kpad <= A.ld && Kpad <= B.ld,
+           FM_ERR_INVALID, "split operands must be padded to the k-step");
+  SplitPtrs a{}, b{};
+  for (int pc = 0; pc < np; pc++) {
+    FM_CHECK(A.piece[pc] && B.piece[pc], FM_ERR_INVALID, "missing split operand piece");
+    FM_CHECK(((uintptr_t)A.piece[pc] | (uintptr_t)B.piece[pc]) % 16 == 0, FM_ERR_INVALID,
+             "split operands must be 16-byte aligned");
+    a.p[pc] = reinterpret_cast<const __bf16 *>(A.piece[pc]);
+    b.p[pc] = reinterpret_cast<const __bf16 *>(B.piece[pc]);
+  }
+  dim3 grid((N + B16_BN - 1) / B16_BN, (M + B16_BM - 1) / B16_BM);
+  launch_split<2>(st, ep, grid, M, N, Kpad, a, A.ld, b, B.ld, e);
   FM_HIP(hipGetLastError());
 }
 
@@ -440,14 +450,17 @@ static uint16_t bf16_rne(float x) {
   return (uint16_t)(u >> 16);
 }
 
-void split_bf16_host(const float *x, size_t n, uint16_t *hi, uint16_t *lo) {
+void split_bf16_host(const float *x, size_t n, int np, uint16_t *const *pieces) {
   for (size_t k = 0; k < n; k++) {
-    uint16_t h = bf16_rne(x[k]);
-    uint32_t hb = (uint32_t)h << 16;
-    float hf;
-    std::memcpy(&hf, &hb, 4);
-    hi[k] = h;
-    lo[k] = bf16_rne(x[k] - hf);
+    float r = x[k];
+    for (int pc = 0; pc < np; pc++) {
+      uint16_t h = bf16_rne(r);
+      uint32_t hb = (uint32_t)h << 16;
+      float hf;
+      std::memcpy(&hf, &hb, 4);
+      pieces[pc][k] = h;
+      r = r - hf;
+    }
   }
 }
 

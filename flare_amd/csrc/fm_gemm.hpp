@@ -40,17 +40,23 @@ int gemm_row_tiles(int form, int M);
 
 void gemm_f64(hipStream_t st, int ep, int M, int N, int K, const double *A, long long lda,
               const double *B, long long ldb, int lower_tri, const EpArgs &e);
-// Split-BF16 x3 on operands stored as BF16 pairs (hi, lo) of FP32 values:
-// rows of A and B are K-contiguous with leading dimensions lda / ldb that are
-// multiples of kSplitPad, padded with zeros up to Kpad (a multiple of
-// kSplitPad), and 16-byte aligned.
+// Split-BF16 x3 contraction on operands stored as two BF16 pieces of FP32
+// values, x = x_0 + x_1 with x_0 = bf16(x), x_1 = bf16(x - x_0); it sums
+// x_1 y_0 + x_0 y_1 + x_0 y_0 (3 MFMAs per product, about 16 significant
+// bits; FP32 accumulation). Rows of A and B are K-contiguous with leading
+// dimensions that are multiples of kSplitPad, padded with zeros up to Kpad (a
+// multiple of kSplitPad), and 16-byte aligned. np must be 2 (the number of
+// pieces each operand carries).
 constexpr int kSplitPad = 32;
 inline long long split_ld(long long k) { return (k + kSplitPad - 1) / kSplitPad * kSplitPad; }
-void gemm_bf16x3(hipStream_t st, int ep, int M, int N, int Kpad, const uint16_t *Ah,
-                 const uint16_t *Al, long long lda, const uint16_t *Bh, const uint16_t *Bl,
-                 long long ldb, const EpArgs &e);
-// Host-side split of FP32 values into BF16 pairs, rounding to nearest even
-// as the device conversion does: hi = bf16(x), lo = bf16(x - hi).
-void split_bf16_host(const float *x, size_t n, uint16_t *hi, uint16_t *lo);
+struct SplitOperand {
+  const uint16_t *piece[3] = {nullptr, nullptr, nullptr};
+  long long ld = 0;
+};
+void gemm_split(hipStream_t st, int ep, int np, int M, int N, int Kpad, const SplitOperand &A,
+                const SplitOperand &B, const EpArgs &e);
+// Host-side split of FP32 values into np BF16 pieces, rounding to nearest
+// even as the device conversion does.
+void split_bf16_host(const float *x, size_t n, int np, uint16_t *const *pieces);
 
 }  // namespace fm

@@ -152,8 +152,12 @@ struct fm_structure_s {
   std::vector<int> n_by_type, nb_by_type, type_off;
   fm::DevBuf d_cutoffs, bond_off, bond_q, in_off, in_bond, row_of_atom,
       atom_of_row, pair_table, B, norm, nn;
-  // Chosen form: the descriptors also as BF16 pairs (n x split_ld(d)).
-  fm::DevBuf Bh, Bl;
+  // Chosen form: the descriptors also as two BF16 pieces (n x split_ld(d)),
+  // the operand of the kernel contraction.
+  fm::DevBuf Bs[2];
+  // Density coefficients c (atom-major, n x nrad H, descriptor precision),
+  // kept for the adjoint.
+  fm::DevBuf C;
   // Dense descriptor derivatives (FP64, single structure): bonds in the
   // species-sorted order of DescriptorValues, f in [sb_off[row], sb_off[row+1])
   // for species-sorted row `row`; bonds of species t are [nbt_off[t],
@@ -171,12 +175,12 @@ struct fm_model_s {
   bool has_L = false;
   std::vector<int> L_rows;   // rows of the L_inv column block of each species
   std::vector<int> L_lower;  // 1 when that block is lower triangular
-  // Per species: X-hat (M_s x d) and X-hat^T (d x M_s) in FP64 and as
-  // BF16 pairs of their FP32 values (Xh/Xl: M_s x split_ld(d), XTh/XTl:
-  // d x split_ld(M_s), zero padded), alpha (zero for empty sparse
-  // environments) and the L_inv block (L_rows x M_s, starting at the
-  // species' first row).
-  std::vector<fm::DevBuf> X, XT, Xh, Xl, XTh, XTl, alpha, L;
+  // Per species: X-hat (M_s x d) and X-hat^T (d x M_s) in FP64 and as two
+  // BF16 pieces of their FP32 values (Xs: M_s x split_ld(d), for the kernel
+  // contraction; XTs: d x split_ld(M_s), for the gradient contraction; zero
+  // padded), alpha (zero for empty sparse environments) and the L_inv block
+  // (L_rows x M_s, starting at the species' first row).
+  std::vector<fm::DevBuf> X, XT, Xs0, Xs1, XTs0, XTs1, alpha, L;
   std::vector<double> host_alpha;
   // Dense (DTC / SOR) predictions: sparse norms per species, alpha in one
   // vector, R_inv^T (M x M) and whether it is lower triangular, and whether

@@ -191,9 +191,13 @@ __device__ __forceinline__ void nbr_unkey(long long key, int sw, int &j, int &s1
 }
 
 // One wavefront per atom. kFill = false counts the neighbours; kFill = true
-// collects the keys in LDS, sorts them and writes the list.
+// collects the keys in LDS, sorts them and writes the list. The stencil's
+// bins (up to kStencilCap per pass) are laid out in LDS with a prefix sum of
+// their sizes, so the wave's 64 lanes take consecutive candidates across
+// bins instead of one bin at a time.
 constexpr int kNbrCap = 2048;  // keys per atom held in LDS
 constexpr int kNbrWaves = 4;
+constexpr int kStencilCap = 128;
 
 template <bool kFill>
 __global__ __launch_bounds__(64 * kNbrWaves) void k_neighbors(
@@ -201,6 +205,9 @@ __global__ __launch_bounds__(64 * kNbrWaves) void k_neighbors(
     const int *species, const CellArgs *cas, const int *atom_struct, const int *bin_off,
     int *count, const int *cum, int *sidx, int *nspec, double *rel, int *max_count) {
   __shared__ long long keys[kNbrWaves][kFill ? kNbrCap : 1];
+  __shared__ int s_off[kNbrWaves][kStencilCap + 1];
+  __shared__ int s_p0[kNbrWaves][kStencilCap];
+  __shared__ int s_img[kNbrWaves][kStencilCap];
   int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   int i = blockIdx.x * kNbrWaves + w;
   if (i >= n) return;
@@ -210,44 +217,92 @@ __global__ __launch_bounds__(64 * kNbrWaves) void k_neighbors(
   double wi[3] = {wrapped[3 * i], wrapped[3 * i + 1], wrapped[3 * i + 2]};
   int b = bin_of[i] - boff;
   int bx = b / (ca.nb[1] * ca.nb[2]), by = (b / ca.nb[2]) % ca.nb[1], bz = b % ca.nb[2];
-  int sw = ca.sweep;
+  int sw = ca.sweep, span = 2 * sw + 1;
+  int e0 = 2 * ca.K[0] + 1, e1 = 2 * ca.K[1] + 1, e2 = 2 * ca.K[2] + 1;
+  int nst = e0 * e1 * e2;
   int found = 0;
-  for (int o0 = -ca.K[0]; o0 <= ca.K[0]; o0++) {
-    int t0 = bx + o0, s1 = floor_div(t0, ca.nb[0]), b0 = t0 - s1 * ca.nb[0];
-    if (s1 < -sw || s1 > sw) continue;
-    for (int o1 = -ca.K[1]; o1 <= ca.K[1]; o1++) {
+  for (int sbase = 0; sbase < nst; sbase += kStencilCap) {
+    int scount = min(kStencilCap, nst - sbase);
+    if (sbase > 0) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    for (int e = lane; e < scount; e += 64) {
+      int sidx_ = sbase + e;
+      int o2 = sidx_ % e2 - ca.K[2], o1 = (sidx_ / e2) % e1 - ca.K[1],
+          o0 = sidx_ / (e1 * e2) - ca.K[0];
+      int t0 = bx + o0, s1 = floor_div(t0, ca.nb[0]), b0 = t0 - s1 * ca.nb[0];
       int t1 = by + o1, s2 = floor_div(t1, ca.nb[1]), b1 = t1 - s2 * ca.nb[1];
-      if (s2 < -sw || s2 > sw) continue;
-      for (int o2 = -ca.K[2]; o2 <= ca.K[2]; o2++) {
-        int t2 = bz + o2, s3 = floor_div(t2, ca.nb[2]), b2 = t2 - s3 * ca.nb[2];
-        if (s3 < -sw || s3 > sw) continue;
+      int t2 = bz + o2, s3 = floor_div(t2, ca.nb[2]), b2 = t2 - s3 * ca.nb[2];
+      bool valid = s1 >= -sw && s1 <= sw && s2 >= -sw && s2 <= sw && s3 >= -sw && s3 <= sw;
+      int size = 0, p0 = 0, img = 0;
+      if (valid) {
         int q = boff + (b0 * ca.nb[1] + b1) * ca.nb[2] + b2;
-        int p0 = bstart[q], p1 = bstart[q + 1];
-        for (int p = p0; p < p1; p += 64) {
-          bool ok = false;
-          int j = 0;
-          if (p + lane < p1) {
-            j = order[p + lane];
-            double wj[3] = {wrapped[3 * j], wrapped[3 * j + 1], wrapped[3 * j + 2]};
-            double im[3];
-            double dist = image_dist(wi, wj, ca, s1, s2, s3, im);
-            ok = (dist < ca.cutoff) && (dist != 0);
-          }
-          unsigned long long mask = __ballot(ok);
-          if (kFill && ok) {
-            int pre = __popcll(mask & ((1ull << lane) - 1ull));
-            int slot = found + pre;
-            if (slot < kNbrCap) keys[w][slot] = nbr_key(j, s1, s2, s3, sw);
-          }
-          found += __popcll(mask);
-        }
+        p0 = bstart[q];
+        size = bstart[q + 1] - p0;
+        img = ((s1 + sw) * span + (s2 + sw)) * span + (s3 + sw);
       }
+      s_p0[w][e] = p0;
+      s_img[w][e] = img;
+      s_off[w][e + 1] = size;
+    }
+    if (lane == 0) s_off[w][0] = 0;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // Inclusive prefix sum of the sizes, 64 entries at a time.
+    int carry = 0;
+    for (int cb = 0; cb < scount; cb += 64) {
+      int e = cb + lane;
+      int v = e < scount ? s_off[w][e + 1] : 0;
+      for (int d = 1; d < 64; d <<= 1) {
+        int u = __shfl_up(v, d, 64);
+        if (lane >= d) v += u;
+      }
+      if (e < scount) s_off[w][e + 1] = v + carry;
+      carry += __shfl(v, 63, 64);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    int total = s_off[w][scount];
+    for (int c0 = 0; c0 < total; c0 += 64) {
+      int t = c0 + lane;
+      bool ok = false;
+      int j = 0, s1 = 0, s2 = 0, s3 = 0;
+      if (t < total) {
+        // The bin holding candidate t: s_off[lo] <= t < s_off[lo + 1].
+        int lo = 0, hi = scount;
+        while (hi - lo > 1) {
+          int mid = (lo + hi) >> 1;
+          if (s_off[w][mid] <= t) lo = mid;
+          else hi = mid;
+        }
+        j = order[s_p0[w][lo] + (t - s_off[w][lo])];
+        int img = s_img[w][lo];
+        s3 = img % span - sw;
+        s2 = (img / span) % span - sw;
+        s1 = img / (span * span) - sw;
+        double wj[3] = {wrapped[3 * j], wrapped[3 * j + 1], wrapped[3 * j + 2]};
+        double im[3];
+        double dist = image_dist(wi, wj, ca, s1, s2, s3, im);
+        ok = (dist < ca.cutoff) && (dist != 0);
+      }
+      unsigned long long mask = __ballot(ok);
+      if (kFill && ok) {
+        int pre = __popcll(mask & ((1ull << lane) - 1ull));
+        int slot = found + pre;
+        if (slot < kNbrCap) keys[w][slot] = nbr_key(j, s1, s2, s3, sw);
+      }
+      found += __popcll(mask);
     }
   }
   if (!kFill) {
     if (lane == 0) {
       count[i] = found;
-      atomicMax(max_count, found);
+      // Only lists longer than the LDS capacity need the fallback.
+      if (found > kNbrCap) atomicMax(max_count, found);
     }
     return;
   }

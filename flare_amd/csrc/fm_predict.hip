@@ -183,9 +183,13 @@ void predict(fm_model m, fm_structure s, int flags, int form, double *mean_efs,
                m->X[t].as<double>(), nd, 0, ek);
     else {
       long long ldd = split_ld(nd);
-      gemm_bf16x3(st, EP_KERNEL, ns, Ms, (int)ldd, s->Bh.as<uint16_t>() + (size_t)ro * ldd,
-                  s->Bl.as<uint16_t>() + (size_t)ro * ldd, ldd, m->Xh[t].as<uint16_t>(),
-                  m->Xl[t].as<uint16_t>(), ldd, ek);
+      SplitOperand A, Bx;
+      for (int pc = 0; pc < 2; pc++) A.piece[pc] = s->Bs[pc].as<uint16_t>() + (size_t)ro * ldd;
+      A.ld = ldd;
+      Bx.piece[0] = m->Xs0[t].as<uint16_t>();
+      Bx.piece[1] = m->Xs1[t].as<uint16_t>();
+      Bx.ld = ldd;
+      gemm_split(st, EP_KERNEL, 2, ns, Ms, (int)ldd, A, Bx, ek);
     }
     ctx->launches[ST_SGEMM]++;
   }
@@ -209,9 +213,14 @@ void predict(fm_model m, fm_structure s, int flags, int form, double *mean_efs,
                0, eg);
     else {
       long long ldm = split_ld(Ms);
-      gemm_bf16x3(st, EP_STORE, ns, nd, (int)ldm, Wh.as<uint16_t>() + so[t],
-                  Wl.as<uint16_t>() + so[t], ldm, m->XTh[t].as<uint16_t>(),
-                  m->XTl[t].as<uint16_t>(), ldm, eg);
+      SplitOperand A, Bx;
+      A.piece[0] = Wh.as<uint16_t>() + so[t];
+      A.piece[1] = Wl.as<uint16_t>() + so[t];
+      A.ld = ldm;
+      Bx.piece[0] = m->XTs0[t].as<uint16_t>();
+      Bx.piece[1] = m->XTs1[t].as<uint16_t>();
+      Bx.ld = ldm;
+      gemm_split(st, EP_STORE, 2, ns, nd, (int)ldm, A, Bx, eg);
     }
     ctx->launches[ST_GGEMM]++;
   }
