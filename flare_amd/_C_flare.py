@@ -88,6 +88,27 @@ class B2(Descriptor):
         return structure._compute_b2(self)
 
 
+class B3(B2):
+    """B3 bispectrum (descriptors/b3.h; b3.cpp:13-50), same arguments as B2;
+    l_max <= 4 (the reference's Wigner 3j table, wigner3j.cpp:1983-1985).
+    Unlike the reference's constructor with a cutoff matrix (b3.cpp:37-56,
+    which leaves the Wigner coefficients unset), both constructors work."""
+
+    descriptor_name = "B3"
+    _N_LS = [1, 5, 15, 34, 65]  # triangle-valid (l1, l2, l3), b3.cpp:200-211
+
+    @property
+    def n_descriptors(self):
+        nos, N, lmax = self._descriptor_settings
+        nr = nos * N
+        return nr * (nr + 1) * (nr + 2) // 6 * self._N_LS[lmax]
+
+    def _settings(self):
+        if self._descriptor_settings[2] > 4:
+            raise ValueError("B3 supports l_max <= 4")
+        return super()._settings()
+
+
 class DescriptorValues:
     """Per-structure descriptor container (descriptor.h:31-57), read lazily
     from the device. Fields are per species lists, in species-sorted order
@@ -386,8 +407,9 @@ class Structure:
             raise ValueError("one descriptor calculator per Structure is supported on the GPU "
                              "path")
         settings, keep = calc._settings()
-        _lib.check(_lib.load().fm_structure_compute_b2(self._h, _lib.C.byref(settings),
-                                                       int(self.form)))
+        fn = (_lib.load().fm_structure_compute_b3 if isinstance(calc, B3)
+              else _lib.load().fm_structure_compute_b2)
+        _lib.check(fn(self._h, _lib.C.byref(settings), int(self.form)))
         del keep
         return DescriptorValues(self, calc)
 

@@ -23,6 +23,7 @@ SOURCES = {
     "fm_predict.hip": [],
     "fm_mapped.hip": [],
     "fm_dtc.hip": [],
+    "fm_b3.hip": [],
 }
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -41,6 +41,7 @@ EXPORTS = [
     "fm_mapped_info", "fm_mapped_destroy", "fm_mapped_compute",
     "fm_batch_create", "fm_batch_create_device", "fm_batch_info",
     "fm_structure_get_b2_dervs", "fm_envs_struc", "fm_self_kernel_struc", "fm_predict_dense",
+    "fm_structure_compute_b3",
 ]
 
 
@@ -97,6 +98,7 @@ def load():
                                         C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.fm_structure_get_neighbors.argtypes = [vp] * 7
         L.fm_structure_compute_b2.argtypes = [vp, C.POINTER(B2Settings), ip]
+        L.fm_structure_compute_b3.argtypes = [vp, C.POINTER(B2Settings), ip]
         L.fm_structure_b2_sizes.argtypes = [vp, C.POINTER(C.c_int), vp, vp]
         L.fm_structure_get_b2.argtypes = [vp, ip] + [vp] * 7
         L.fm_model_create.argtypes = [vp, C.POINTER(ModelDesc), pvp]

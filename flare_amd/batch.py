@@ -32,7 +32,8 @@ class Batch:
                                      float(cutoff), C.byref(h)))
         self._h = h
         settings, keep = calc._settings()
-        _lib.check(L.fm_structure_compute_b2(h, C.byref(settings), int(form)))
+        fn = L.fm_structure_compute_b3 if calc.descriptor_name == "B3" else L.fm_structure_compute_b2
+        _lib.check(fn(h, C.byref(settings), int(form)))
         del keep
 
     def __len__(self):

@@ -1,0 +1,419 @@
+// B3 descriptor (descriptors/b3.cpp:80-443) on top of the B2 bond machinery.
+//
+// The complex single-bond coefficients of b3.cpp:291-443 (complex Y_l^m with
+// the Condon-Shortley phase) are a fixed linear map of the real ones the
+// descriptor kernel already accumulates: for m > 0,
+//   c_l^m  = (-1)^m (c^r_lm + i c^r_l,-m) / sqrt 2,
+//   c_l^-m = (c^r_lm - i c^r_l,-m) / sqrt 2,   c_l^0 = c^r_l0.
+// So B3 runs the bond pass of k_b2 (c only, kept in HBM), then per atom
+//   B3_k = Re sum_{m1 + m2 + m3 = 0} w3j(l1 l2 l3; m1 m2 m3) c[n1, l1 m1]
+//          c[n2, l2 m2] c[n3, l3 m3]
+// over n1 <= n2 <= n3 and the triangle-valid (l1, l2, l3) in the
+// reference's order (b3.cpp:185-289). The adjoint maps dE/dB3 to the complex
+// gradient dE/dRe c + i dE/dIm c = sum conj(w3j w_k c c) over the two other
+// factors, then through the transpose of the linear map to dE/dc^r, so the
+// bond-force kernel of B2 produces the forces unchanged.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "fm_device.hpp"
+#include "fm_gemm.hpp"
+#include "fm_internal.hpp"
+
+namespace fm {
+
+namespace {
+
+double fact(int n) {
+  double f = 1;
+  for (int k = 2; k <= n; k++) f *= k;
+  return f;
+}
+
+// Wigner 3j symbol by the Racah formula (the values wigner3j.cpp tabulates).
+double w3j(int j1, int j2, int j3, int m1, int m2, int m3) {
+  if (m1 + m2 + m3 != 0) return 0;
+  if (j3 < std::abs(j1 - j2) || j3 > j1 + j2) return 0;
+  if (std::abs(m1) > j1 || std::abs(m2) > j2 || std::abs(m3) > j3) return 0;
+  double tri = fact(j1 + j2 - j3) * fact(j1 - j2 + j3) * fact(-j1 + j2 + j3) /
+               fact(j1 + j2 + j3 + 1);
+  double pre = std::sqrt(tri * fact(j1 + m1) * fact(j1 - m1) * fact(j2 + m2) * fact(j2 - m2) *
+                         fact(j3 + m3) * fact(j3 - m3));
+  double sum = 0;
+  for (int k = 0; k <= j1 + j2 + j3; k++) {
+    int a[6] = {k, j3 - j2 + k + m1, j3 - j1 + k - m2, j1 + j2 - j3 - k, j1 - k - m1,
+                j2 - k + m2};
+    bool ok = true;
+    for (int v : a) ok = ok && v >= 0;
+    if (!ok) continue;
+    double den = 1;
+    for (int v : a) den *= fact(v);
+    sum += ((k % 2) ? -1.0 : 1.0) / den;
+  }
+  int ph = ((j1 - j2 - m3) % 2 + 2) % 2;
+  return (ph ? -1.0 : 1.0) * pre * sum;
+}
+
+}  // namespace
+
+// Host tables of a B3 setting (shared by the forward and adjoint kernels).
+struct B3Tables {
+  int nr = 0, H = 0, n_ls = 0, n_tri = 0, nd = 0;
+  std::vector<int> tri;        // n_tri x 3 (n1, n2, n3), n1 <= n2 <= n3
+  std::vector<int> tri_base;   // nr: index of the first triple with this n1
+  std::vector<int> ls_l;       // n_ls x 3
+  std::vector<int> term_off;   // n_ls + 1
+  std::vector<int> term_lm;    // n_terms x 3 (lm of each factor)
+  std::vector<double> term_w;  // n_terms
+  // Gradient lists: for each (lm, position p): the (ls, term) pairs whose
+  // factor p has that lm. Entries grouped by lm * 3 + p.
+  std::vector<int> grad_off;   // 3 H + 1
+  std::vector<int> grad_ls, grad_term;
+};
+
+static void build_b3_tables(int nr, int lmax, B3Tables &t) {
+  t.nr = nr;
+  t.H = (lmax + 1) * (lmax + 1);
+  t.term_off.push_back(0);
+  for (int l1 = 0; l1 <= lmax; l1++)
+    for (int l2 = 0; l2 <= lmax; l2++)
+      for (int l3 = 0; l3 <= lmax; l3++) {
+        if (std::abs(l1 - l2) > l3 || l3 > l1 + l2) continue;
+        t.ls_l.insert(t.ls_l.end(), {l1, l2, l3});
+        for (int m1 = 0; m1 < 2 * l1 + 1; m1++)
+          for (int m2 = 0; m2 < 2 * l2 + 1; m2++) {
+            int m3 = l1 + l2 + l3 - m1 - m2;
+            if (m3 < 0 || m3 > 2 * l3) continue;
+            double w = w3j(l1, l2, l3, m1 - l1, m2 - l2, m3 - l3);
+            if (w == 0.0) continue;
+            t.term_lm.insert(t.term_lm.end(), {l1 * l1 + m1, l2 * l2 + m2, l3 * l3 + m3});
+            t.term_w.push_back(w);
+          }
+        t.term_off.push_back((int)t.term_w.size());
+      }
+  t.n_ls = (int)t.ls_l.size() / 3;
+  for (int n1 = 0; n1 < nr; n1++) {
+    t.tri_base.push_back((int)t.tri.size() / 3);
+    for (int n2 = n1; n2 < nr; n2++)
+      for (int n3 = n2; n3 < nr; n3++) t.tri.insert(t.tri.end(), {n1, n2, n3});
+  }
+  t.n_tri = (int)t.tri.size() / 3;
+  t.nd = t.n_tri * t.n_ls;
+  std::vector<std::vector<std::pair<int, int>>> lists(3 * t.H);
+  for (int ls = 0; ls < t.n_ls; ls++)
+    for (int q = t.term_off[ls]; q < t.term_off[ls + 1]; q++)
+      for (int p = 0; p < 3; p++) lists[t.term_lm[3 * q + p] * 3 + p].push_back({ls, q});
+  t.grad_off.push_back(0);
+  for (auto &v : lists) {
+    for (auto &e : v) {
+      t.grad_ls.push_back(e.first);
+      t.grad_term.push_back(e.second);
+    }
+    t.grad_off.push_back((int)t.grad_ls.size());
+  }
+}
+
+int b3_size(int nos, int N, int lmax) {
+  B3Tables t;
+  build_b3_tables(nos * N, lmax, t);
+  return t.nd;
+}
+
+// Device copies of the tables.
+struct B3Dev {
+  const int *tri, *ls_l, *term_off, *term_lm, *grad_off, *grad_ls, *grad_term, *tri_base;
+  const double *term_w;
+  int nr, H, n_ls, n_tri, nd;
+};
+
+// Complex c of one atom in LDS from the real coefficients (interleaved
+// re, im; index n H + lm with lm = l^2 + m + l).
+template <typename T>
+__device__ void complex_c(const T *cr, T *cc, int nr, int H, int lane) {
+  const T s2 = T(0.70710678118654752440);
+  for (int k = lane; k < nr * H; k += 64) {
+    int n = k / H, lm = k - n * H;
+    int l = 0;
+    while ((l + 1) * (l + 1) <= lm) l++;
+    int m = lm - l * l - l;
+    const T *row = cr + n * H + l * l + l;
+    T re, im;
+    if (m == 0) {
+      re = row[0];
+      im = 0;
+    } else if (m > 0) {
+      T sg = (m & 1) ? T(-1) : T(1);
+      re = sg * s2 * row[m];
+      im = sg * s2 * row[-m];
+    } else {
+      re = s2 * row[-m];
+      im = -s2 * row[m];
+    }
+    cc[2 * k] = re;
+    cc[2 * k + 1] = im;
+  }
+}
+
+// B3 per atom: one wavefront, complex c in LDS, lanes over outputs.
+template <typename T>
+__global__ __launch_bounds__(256) void k_b3(int n, B3Dev t, const T *Cg, const int *row_of_atom,
+                                             T *B, double *norm, double *nnv, __bf16 *B0,
+                                             __bf16 *B1, int ldp) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int i = blockIdx.x * (blockDim.x >> 6) + w;
+  if (i >= n) return;
+  int nc = t.nr * t.H;
+  T *cc = reinterpret_cast<T *>(smem) + (size_t)w * 2 * nc;
+  complex_c<T>(Cg + (size_t)i * nc, cc, t.nr, t.H, lane);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  int row = row_of_atom[i];
+  T *Brow = B + (size_t)row * t.nd;
+  T s2 = 0;
+  for (int k = lane; k < t.nd; k += 64) {
+    int tr = k / t.n_ls, ls = k - tr * t.n_ls;
+    int n1 = t.tri[3 * tr], n2 = t.tri[3 * tr + 1], n3 = t.tri[3 * tr + 2];
+    const T *c1 = cc + 2 * n1 * t.H, *c2 = cc + 2 * n2 * t.H, *c3 = cc + 2 * n3 * t.H;
+    T acc = 0;
+    for (int q = t.term_off[ls]; q < t.term_off[ls + 1]; q++) {
+      int a1 = t.term_lm[3 * q], a2 = t.term_lm[3 * q + 1], a3 = t.term_lm[3 * q + 2];
+      T xr = c1[2 * a1], xi = c1[2 * a1 + 1], yr = c2[2 * a2], yi = c2[2 * a2 + 1];
+      T zr = c3[2 * a3], zi = c3[2 * a3 + 1];
+      T pr = xr * yr - xi * yi, pi = xr * yi + xi * yr;
+      acc += (T)t.term_w[q] * (pr * zr - pi * zi);
+    }
+    Brow[k] = acc;
+    if (B0) {
+      float r = (float)acc;
+      __bf16 h = (__bf16)r;
+      B0[(size_t)row * ldp + k] = h;
+      B1[(size_t)row * ldp + k] = (__bf16)(r - (float)h);
+    }
+    s2 += acc * acc;
+  }
+  if (B0)
+    for (int k = t.nd + lane; k < ldp; k += 64) {
+      B0[(size_t)row * ldp + k] = (__bf16)0.0f;
+      B1[(size_t)row * ldp + k] = (__bf16)0.0f;
+    }
+  s2 = wave_sum(s2);
+  if (lane == 0) {
+    norm[row] = (double)sqrt(s2);
+    nnv[row] = (double)s2;
+  }
+}
+
+__device__ __forceinline__ int tri_index(const B3Dev &t, int n1, int n2, int n3) {
+  int nr = t.nr;
+  return t.tri_base[n1] + (n2 - n1) * nr - (n2 * (n2 - 1) - n1 * (n1 - 1)) / 2 + (n3 - n2);
+}
+
+// dE/dc for B3 per atom. With `project`, w_k = (G_k - (G . B_hat) B_k /|B|)
+// / |B| (as for B2; read from HBM, the row is too long for LDS). Lanes over
+// the targets (n, lm): the complex gradient sums, over each factor position
+// p holding (n, lm), w_k w3j conj(product of the other two factors); the
+// real gradient u = dE/dc^r follows by the transpose of the complex map.
+// Each target's sum runs in a fixed order, so results are deterministic.
+template <typename T>
+__global__ __launch_bounds__(256) void k_dEdc_b3(int n, B3Dev t, const T *Cg,
+                                                  const int *row_of_atom, const T *B,
+                                                  const double *norm, const double *nnv,
+                                                  const T *G, long long g_ld, int project,
+                                                  double empty_norm, double empty_nn, T *Ug) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int i = blockIdx.x * (blockDim.x >> 6) + w;
+  if (i >= n) return;
+  int nc = t.nr * t.H;
+  T *cc = reinterpret_cast<T *>(smem) + (size_t)w * 4 * nc;
+  T *gc = cc + 2 * nc;  // complex gradient
+  int row = row_of_atom[i];
+  double nrm = norm[row];
+  if (nrm < empty_norm || nnv[row] < empty_nn) return;
+  complex_c<T>(Cg + (size_t)i * nc, cc, t.nr, t.H, lane);
+  const T *Brow = B + (size_t)row * t.nd;
+  const T *Grow = G + (size_t)row * g_ld;
+  T ninv = T(1) / (T)nrm, gb = 0;
+  if (project) {
+    for (int k = lane; k < t.nd; k += 64) gb += Grow[k] * Brow[k];
+    gb = wave_sum(gb) * ninv;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  auto wk = [&](int k) -> T {
+    return project ? (Grow[k] - gb * Brow[k] * ninv) * ninv : Grow[k];
+  };
+  int nr = t.nr, H = t.H;
+  for (int g = lane; g < nc; g += 64) {
+    int nn = g / H, lm = g - nn * H;
+    T ar = 0, ai = 0;
+    for (int p = 0; p < 3; p++) {
+      int e0 = t.grad_off[lm * 3 + p], e1 = t.grad_off[lm * 3 + p + 1];
+      for (int e = e0; e < e1; e++) {
+        int ls = t.grad_ls[e], q = t.grad_term[e];
+        T wq = (T)t.term_w[q];
+        int a[3] = {t.term_lm[3 * q], t.term_lm[3 * q + 1], t.term_lm[3 * q + 2]};
+        // The two other factors' lm.
+        int oa = p == 0 ? a[1] : a[0], ob = p == 2 ? a[1] : a[2];
+        // Triples with factor p = nn: (nn, x, y), (x, nn, y) or (x, y, nn).
+        int lo_x = p == 0 ? nn : 0, hi_x = p == 0 ? nr - 1 : nn;
+        for (int x = lo_x; x <= hi_x; x++) {
+          int lo_y = p == 0 ? x : (p == 1 ? nn : x);
+          int hi_y = p == 2 ? nn : nr - 1;
+          for (int y = lo_y; y <= hi_y; y++) {
+            int n1, n2, n3, fa, fb;  // fa, fb: radial index of the other factors
+            if (p == 0) {
+              n1 = nn; n2 = x; n3 = y; fa = x; fb = y;
+            } else if (p == 1) {
+              n1 = x; n2 = nn; n3 = y; fa = x; fb = y;
+            } else {
+              n1 = x; n2 = y; n3 = nn; fa = x; fb = y;
+            }
+            T wkv = wk(tri_index(t, n1, n2, n3) * t.n_ls + ls) * wq;
+            const T *u = cc + 2 * (fa * H + oa), *v = cc + 2 * (fb * H + ob);
+            T pr = u[0] * v[0] - u[1] * v[1], pi = u[0] * v[1] + u[1] * v[0];
+            ar += wkv * pr;
+            ai -= wkv * pi;
+          }
+        }
+      }
+    }
+    gc[2 * g] = ar;
+    gc[2 * g + 1] = ai;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  // u[n, l, m] = d E / d c^r: transpose of the complex map.
+  const T s2 = T(0.70710678118654752440);
+  T *urow = Ug + (size_t)i * nc;
+  for (int g = lane; g < nc; g += 64) {
+    int nn = g / H, lm = g - nn * H;
+    int l = 0;
+    while ((l + 1) * (l + 1) <= lm) l++;
+    int m = lm - l * l - l;
+    const T *gr = gc + 2 * (nn * H + l * l + l);
+    T u;
+    if (m == 0) {
+      u = gr[0];
+    } else {
+      int mm = m > 0 ? m : -m;
+      T sg = (mm & 1) ? T(-1) : T(1);
+      if (m > 0)  // c^r_lm enters Re c_l^m and Re c_l^-m
+        u = s2 * (sg * gr[2 * mm] + gr[-2 * mm]);
+      else        // c^r_l,-m enters Im c_l^m and Im c_l^-m
+        u = s2 * (sg * gr[2 * mm + 1] - gr[-2 * mm + 1]);
+    }
+    urow[g] = u;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+struct B3State {
+  B3Tables t;
+  DevBuf tri, tri_base, ls_l, term_off, term_lm, term_w, grad_off, grad_ls, grad_term;
+  B3Dev dev() const {
+    return B3Dev{tri.as<int>(),      ls_l.as<int>(),    term_off.as<int>(),
+                 term_lm.as<int>(),  grad_off.as<int>(), grad_ls.as<int>(),
+                 grad_term.as<int>(), tri_base.as<int>(), term_w.as<double>(),
+                 t.nr, t.H, t.n_ls, t.n_tri, t.nd};
+  }
+};
+
+static B3State *b3_state(fm_structure s) {
+  if (!s->b3) {
+    auto *st = new B3State();
+    build_b3_tables(s->nos * s->N, s->lmax, st->t);
+    hipStream_t stream = s->ctx->stream;
+    Pool *pool = &s->ctx->pool;
+    auto up = [&](DevBuf &b, const void *src, size_t bytes) {
+      b.alloc(pool, std::max<size_t>(1, bytes));
+      if (bytes) FM_HIP(hipMemcpyAsync(b.as<void>(), src, bytes, hipMemcpyHostToDevice, stream));
+    };
+    const B3Tables &t = st->t;
+    up(st->tri, t.tri.data(), t.tri.size() * 4);
+    up(st->tri_base, t.tri_base.data(), t.tri_base.size() * 4);
+    up(st->ls_l, t.ls_l.data(), t.ls_l.size() * 4);
+    up(st->term_off, t.term_off.data(), t.term_off.size() * 4);
+    up(st->term_lm, t.term_lm.data(), t.term_lm.size() * 4);
+    up(st->term_w, t.term_w.data(), t.term_w.size() * 8);
+    up(st->grad_off, t.grad_off.data(), t.grad_off.size() * 4);
+    up(st->grad_ls, t.grad_ls.data(), t.grad_ls.size() * 4);
+    up(st->grad_term, t.grad_term.data(), t.grad_term.size() * 4);
+    FM_HIP(hipStreamSynchronize(stream));
+    s->b3 = st;
+  }
+  return s->b3;
+}
+
+void free_b3_state(fm_structure s) {
+  delete s->b3;
+  s->b3 = nullptr;
+}
+
+static int waves_per_block(size_t lds_per_atom) {
+  const size_t cap = 160 * 1024;
+  FM_CHECK(lds_per_atom <= cap, FM_ERR_CAPACITY, "per-atom B3 workspace exceeds LDS");
+  int apb = 4;
+  while (apb > 1 && apb * lds_per_atom > cap) apb >>= 1;
+  return apb;
+}
+
+void launch_b3(fm_structure s) {
+  hipStream_t st = s->ctx->stream;
+  B3State *b = b3_state(s);
+  B3Dev d = b->dev();
+  int n = s->n;
+  if (n == 0) return;
+  bool f64 = s->form == FM_FORM_FP64;
+  size_t esz = f64 ? 8 : 4;
+  size_t per = 2 * (size_t)d.nr * d.H * esz;
+  int apb = waves_per_block(per);
+  int nblk = (n + apb - 1) / apb;
+  if (f64)
+    hipLaunchKernelGGL(k_b3<double>, dim3(nblk), dim3(64 * apb), apb * per, st, n, d,
+                       s->C.as<double>(), s->row_of_atom.as<int>(), s->B.as<double>(),
+                       s->norm.as<double>(), s->nn.as<double>(), (__bf16 *)nullptr,
+                       (__bf16 *)nullptr, 0);
+  else
+    hipLaunchKernelGGL(k_b3<float>, dim3(nblk), dim3(64 * apb), apb * per, st, n, d,
+                       s->C.as<float>(), s->row_of_atom.as<int>(), s->B.as<float>(),
+                       s->norm.as<double>(), s->nn.as<double>(), s->Bs[0].as<__bf16>(),
+                       s->Bs[1].as<__bf16>(), (int)split_ld(s->nd));
+  FM_HIP(hipGetLastError());
+}
+
+void launch_dEdc_b3(fm_structure s, const void *G, int64_t g_ld, int project,
+                    double empty_norm, double empty_nn, void *U) {
+  hipStream_t st = s->ctx->stream;
+  B3State *b = b3_state(s);
+  B3Dev d = b->dev();
+  int n = s->n;
+  if (n == 0) return;
+  bool f64 = s->form == FM_FORM_FP64;
+  size_t esz = f64 ? 8 : 4;
+  size_t per = 4 * (size_t)d.nr * d.H * esz;
+  int apb = waves_per_block(per);
+  int nblk = (n + apb - 1) / apb;
+  if (f64)
+    hipLaunchKernelGGL(k_dEdc_b3<double>, dim3(nblk), dim3(64 * apb), apb * per, st, n, d,
+                       s->C.as<double>(), s->row_of_atom.as<int>(), s->B.as<double>(),
+                       s->norm.as<double>(), s->nn.as<double>(), (const double *)G,
+                       (long long)g_ld, project, empty_norm, empty_nn, (double *)U);
+  else
+    hipLaunchKernelGGL(k_dEdc_b3<float>, dim3(nblk), dim3(64 * apb), apb * per, st, n, d,
+                       s->C.as<float>(), s->row_of_atom.as<int>(), s->B.as<float>(),
+                       s->norm.as<double>(), s->nn.as<double>(), (const float *)G,
+                       (long long)g_ld, project, empty_norm, empty_nn, (float *)U);
+  FM_HIP(hipGetLastError());
+}
+
+}  // namespace fm

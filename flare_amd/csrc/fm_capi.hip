@@ -371,6 +371,7 @@ int fm_structure_destroy(fm_structure s) {
   return guarded([&] {
     if (!s) return;
     (void)hipStreamSynchronize(s->ctx->stream);
+    fm::free_b3_state(s);
     delete s;
   });
 }
@@ -404,8 +405,10 @@ int fm_structure_get_neighbors(fm_structure s, double *wrapped, int32_t *count, 
   });
 }
 
-int fm_structure_compute_b2(fm_structure s, const fm_b2_settings *st_in, int form) {
-  return guarded([&] {
+// B2 (kind 2) or B3 (kind 3) descriptors of a structure; B3 keeps the B2
+// bond pass (c only) and adds the triple products (fm_b3.hip).
+static void compute_descriptor(fm_structure s, const fm_b2_settings *st_in, int form, int kind) {
+  {
     FM_CHECK(s && st_in && st_in->cutoffs, FM_ERR_INVALID, "null argument");
     FM_CHECK(form == FM_FORM_FP64 || form == FM_FORM_CHOSEN, FM_ERR_INVALID, "unknown form");
     FM_CHECK(st_in->n_species >= 1 && st_in->n_max >= 1 && st_in->l_max >= 0, FM_ERR_INVALID,
@@ -427,7 +430,15 @@ int fm_structure_compute_b2(fm_structure s, const fm_b2_settings *st_in, int for
     for (double c : s->cutoffs)
       FM_CHECK(c > s->r1 && std::isfinite(c), FM_ERR_INVALID, "pair cutoffs must exceed r1");
     int nr = s->nos * s->N;
-    s->nd = nr * (nr + 1) / 2 * (s->lmax + 1);
+    fm::free_b3_state(s);
+    s->desc_kind = kind;
+    if (kind == 3) {
+      FM_CHECK(s->lmax <= 4, FM_ERR_INVALID,
+               "B3 needs l_max <= 4 (the Wigner 3j table of wigner3j.cpp)");
+      s->nd = fm::b3_size(s->nos, s->N, s->lmax);
+    } else {
+      s->nd = nr * (nr + 1) / 2 * (s->lmax + 1);
+    }
     if (s->n > 0) {
       // Species must lie in [0, n_species) for this descriptor.
       FM_CHECK(s->max_species < s->nos, FM_ERR_INVALID, "species outside [0, n_species)");
@@ -459,7 +470,10 @@ int fm_structure_compute_b2(fm_structure s, const fm_b2_settings *st_in, int for
     }
     s->norm.alloc(pool, std::max<size_t>(1, (size_t)s->n * sizeof(double)));
     s->nn.alloc(pool, std::max<size_t>(1, (size_t)s->n * sizeof(double)));
-    if (s->n > 0) fm::launch_b2(s);
+    if (s->n > 0) {
+      fm::launch_b2(s, kind == 2);
+      if (kind == 3) fm::launch_b3(s);
+    }
     ctx->mark_end(fm::ST_DESC);
     // Per-structure species-sorted position of every global row (the
     // identity for a single structure).
@@ -486,7 +500,15 @@ int fm_structure_compute_b2(fm_structure s, const fm_b2_settings *st_in, int for
     s->has_dervs = false;    // dense derivatives are rebuilt on demand
     s->DB.release();
     s->has_b2 = true;
-  });
+  }
+}
+
+int fm_structure_compute_b2(fm_structure s, const fm_b2_settings *st_in, int form) {
+  return guarded([&] { compute_descriptor(s, st_in, form, 2); });
+}
+
+int fm_structure_compute_b3(fm_structure s, const fm_b2_settings *st_in, int form) {
+  return guarded([&] { compute_descriptor(s, st_in, form, 3); });
 }
 
 // Neighbour counts by species (b2.cpp:106-112), computed on first use.

@@ -22,7 +22,6 @@
 
 namespace fm {
 
-constexpr int kMaxHC = 2;  // harmonic chunks of 64 lanes (lmax <= 10)
 constexpr int kAtomsPerBlock = 4;
 
 struct B2Args {
@@ -34,14 +33,6 @@ struct B2Args {
 };
 
 __constant__ double c_knorm[kMaxH];
-
-template <typename T>
-__device__ __forceinline__ void lane_lm(int lm, int &l, int &m) {
-  l = (int)sqrtf((float)lm);
-  while (l * l > lm) l--;
-  while ((l + 1) * (l + 1) <= lm) l++;
-  m = lm - l * l - l;
-}
 
 // 1 / k for the associated-Legendre recursions (k <= lmax + 1).
 __constant__ double c_rcp[kMaxL + 2];
@@ -110,99 +101,107 @@ __device__ __forceinline__ void ylm_sweep(int L, T ux, T uy, T uz, F &&f) {
   }
 }
 
-__device__ __forceinline__ float readlane(float v, int l) {
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+
+// 16 x 16 x 4 MFMA in the descriptor precision (FP32 accumulators hold
+// rows 4 (lane / 16) + q, FP64 ones rows lane / 16 + 4 q; both take
+// A[lane % 16][lane / 16] and B[lane / 16][lane % 16]).
+template <typename T>
+struct MfmaAcc;
+template <>
+struct MfmaAcc<float> {
+  typedef float type __attribute__((ext_vector_type(4)));
+};
+template <>
+struct MfmaAcc<double> {
+  typedef double type __attribute__((ext_vector_type(4)));
+};
+__device__ __forceinline__ MfmaAcc<float>::type mfma_4(float a, float b,
+                                                       MfmaAcc<float>::type c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
-__device__ __forceinline__ double readlane(double v, int l) {
-  long long x = __double_as_longlong(v);
-  int lo = __builtin_amdgcn_readlane((int)x, l);
-  int hi = __builtin_amdgcn_readlane((int)(x >> 32), l);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+__device__ __forceinline__ MfmaAcc<double>::type mfma_4(double a, double b,
+                                                        MfmaAcc<double>::type c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+template <typename T>
+__device__ __forceinline__ int mfma_row(int lane, int q) {
+  return sizeof(T) == 8 ? (lane >> 4) + 4 * q : 4 * (lane >> 4) + q;
+}
+
+// Workspace (elements of T) of one atom of the descriptor kernel: c, then a
+// region holding the bond tables (Y 64 x H, g 64 x N, neighbour species 64)
+// and later the B row.
+__host__ __device__ inline size_t b2_work(int nc, int H, int N, int nd) {
+  size_t tables = 64 * (size_t)H + 64 * (size_t)N + 64;
+  return (size_t)nc + (tables > (size_t)nd ? tables : (size_t)nd);
 }
 
 // c[(t N + n) H + lm] of atom i for all neighbour species t into LDS
-// (b2.cpp:367-395): for each chunk of up to 64 bonds, lane k takes bond k,
-// evaluates its radial basis g_n (registers) and every Y_lm (into the LDS
-// table Yt[k][lm]); then lanes over lm add g_n(k) Y_lm(k) over the chunk's
-// bonds of species t in list order, reading g_n(k) from lane k.
-template <typename T, int NG>
-__device__ void accumulate_c(const B2Args &a, int i, int lane, T *c, T *Yt) {
+// (b2.cpp:367-395). For each chunk of up to 64 bonds, lane k takes bond k and
+// writes its radial basis g_n and every Y_lm to LDS tables; then, per species
+// t and 16 harmonics, c_t += G_t^T Y on the matrix cores (A = g_n of the
+// chunk's bonds of species t, B = their Y_lm).
+template <typename T>
+__device__ void accumulate_c(const B2Args &a, int i, int lane, T *c, T *tab) {
   int si = a.species[i];
   int b0 = a.bond_off[i], b1 = a.bond_off[i + 1];
-  int nchunks = (b1 - b0 + 63) / 64;
-  T g[kMaxN], dg[kMaxN];
-  int sj = -1;
-  // NG lane groups of 64 / NG lanes each (NG > 1 when H <= 64 / NG): group
-  // grp accumulates the radial functions n = grp, grp + NG, ... (slot nn of
-  // acc holds n = nn NG + grp).
-  constexpr int hp = 64 / NG;
-  int grp = lane / hp, lmb = lane % hp;
-  for (int t = 0; t < a.nos; t++) {
-    T acc[kMaxHC][kMaxN];
+  int nc = a.nrad * a.H, H = a.H, N = a.N;
+  T *Yt = tab, *gt = tab + 64 * (size_t)H;
+  int *sjt = reinterpret_cast<int *>(gt + 64 * (size_t)N);
+  for (int k = lane; k < nc; k += 64) c[k] = 0;
+  int r16 = lane & 15, k4 = lane >> 4;
+  for (int bb = b0; bb < b1; bb += 64) {
+    int nk = min(64, b1 - bb);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (lane < nk) {
+      int q = a.bond_q[bb + lane];
+      int sj = a.nspec[q];
+      const double *rp = a.rel + 4 * (size_t)q;
+      T g[kMaxN], dg[kMaxN];
+      radial_basis<T>((T)rp[0], (T)a.cutoffs[si * a.nos + sj], (T)a.r1, N, a.cutkind, g, dg);
 #pragma unroll
-    for (int ch = 0; ch < kMaxHC; ch++)
-#pragma unroll
-      for (int nn = 0; nn < kMaxN; nn++) acc[ch][nn] = 0;
-    for (int ck = 0; ck < nchunks; ck++) {
-      int bb = b0 + 64 * ck;
-      int nk = min(64, b1 - bb);
-      if (nchunks > 1 || t == 0) {
-        if (ck > 0 || t > 0) {
-          // Lanes still read the previous chunk's table.
-          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-          __builtin_amdgcn_wave_barrier();
-          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        }
-        if (lane < nk) {
-          int q = a.bond_q[bb + lane];
-          sj = a.nspec[q];
-          const double *rp = a.rel + 4 * (size_t)q;
-          T r = (T)rp[0];
-          T rc = (T)a.cutoffs[si * a.nos + sj];
-          radial_basis<T>(r, rc, (T)a.r1, a.N, a.cutkind, g, dg);
-          T rinv = T(1) / r;
-          T *yrow = Yt + (size_t)lane * a.H;
-          ylm_sweep<T, false>(a.lmax, (T)rp[1] * rinv, (T)rp[2] * rinv, (T)rp[3] * rinv,
-                              [&](int lm, T Y, T, T, T) { yrow[lm] = Y; });
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      }
-      for (int k = 0; k < nk; k++) {
-        if (__builtin_amdgcn_readlane(sj, k) != t) continue;
-        T y[kMaxHC];
-#pragma unroll
-        for (int ch = 0; ch < kMaxHC; ch++) {
-          int lm = lmb + 64 * ch;
-          y[ch] = lm < a.H ? Yt[(size_t)k * a.H + lm] : T(0);
-        }
-#pragma unroll
-        for (int nn = 0; nn < kMaxN / NG; nn++) {
-          if (nn * NG < a.N) {
-            T gk = 0;
-#pragma unroll
-            for (int gg = 0; gg < NG; gg++) {
-              T v = readlane(g[nn * NG + gg], k);
-              if (gg == grp) gk = v;
-            }
-#pragma unroll
-            for (int ch = 0; ch < kMaxHC; ch++) acc[ch][nn] += gk * y[ch];
-          }
-        }
-      }
+      for (int nn = 0; nn < kMaxN; nn++)
+        if (nn < N) gt[lane * N + nn] = g[nn];
+      sjt[lane] = sj;
+      T rinv = T(1) / (T)rp[0];
+      T *yrow = Yt + (size_t)lane * H;
+      ylm_sweep<T, false>(a.lmax, (T)rp[1] * rinv, (T)rp[2] * rinv, (T)rp[3] * rinv,
+                          [&](int lm, T Y, T, T, T) { yrow[lm] = Y; });
     }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int t = 0; t < a.nos; t++) {
+      for (int ct = 0; ct < H; ct += 16) {
+        int lm = ct + r16;
+        typename MfmaAcc<T>::type acc;
 #pragma unroll
-    for (int ch = 0; ch < kMaxHC; ch++) {
-      int lm = lmb + 64 * ch;
-      if (lm >= a.H) continue;
+        for (int q = 0; q < 4; q++) {
+          int n = mfma_row<T>(lane, q);
+          acc[q] = (n < N && lm < H) ? c[(t * N + n) * H + lm] : T(0);
+        }
+        for (int k0 = 0; k0 < nk; k0 += 4) {
+          int b = k0 + k4;
+          T av = 0, bv = 0;
+          if (b < nk) {
+            if (r16 < N && sjt[b] == t) av = gt[b * N + r16];
+            if (lm < H) bv = Yt[b * H + lm];
+          }
+          acc = mfma_4(av, bv, acc);
+        }
 #pragma unroll
-      for (int nn = 0; nn < kMaxN / NG; nn++) {
-        int n = nn * NG + grp;
-        if (n < a.N) c[(t * a.N + n) * a.H + lm] = acc[ch][nn];
+        for (int q = 0; q < 4; q++) {
+          int n = mfma_row<T>(lane, q);
+          if (n < N && lm < H) c[(t * N + n) * H + lm] = acc[q];
+        }
       }
     }
   }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // B2 (b2.cpp:209-231) per atom: B[k] = sum_m c[n1, lm] c[n2, lm], written at
@@ -214,32 +213,56 @@ struct SplitOut {
   __bf16 *p[2];
 };
 
-template <typename T, int NG>
+template <typename T>
 __global__ __launch_bounds__(64 * kAtomsPerBlock) void k_b2(B2Args a, T *B, double *norm,
                                                               double *nn, SplitOut Bs, int ldp,
-                                                              T *Cg) {
+                                                              T *Cg, int want_b2) {
   extern __shared__ __align__(16) unsigned char smem[];
   int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   int i = blockIdx.x * (blockDim.x >> 6) + w;
   if (i >= a.n) return;
-  int nc = a.nrad * a.H;
-  T *c = reinterpret_cast<T *>(smem) + (size_t)w * ((size_t)nc + 64 * a.H);
-  T *Yt = c + nc;
-  accumulate_c<T, NG>(a, i, lane, c, Yt);
+  int nc = a.nrad * a.H, H = a.H, nr = a.nrad;
+  T *c = reinterpret_cast<T *>(smem) + (size_t)w * b2_work(nc, H, a.N, a.nd);
+  T *tab = c + nc;
+  accumulate_c<T>(a, i, lane, c, tab);
+  for (int k = lane; k < nc; k += 64) Cg[(size_t)i * nc + k] = c[k];
+  if (!want_b2) return;
+  // B_l = C_l C_l^T on the matrix cores (C_l[n][m] = c[n, l^2 + m]); the
+  // upper triangle n2 >= n1 goes to the staged row at p(n1, n2) (L + 1) + l.
+  T *Bst = tab;
+  int L1 = a.lmax + 1;
+  int r16 = lane & 15, k4 = lane >> 4;
+  for (int l = 0; l <= a.lmax; l++) {
+    int nm = 2 * l + 1, lm0 = l * l;
+    for (int rt = 0; rt < nr; rt += 16) {
+      for (int cc = rt; cc < nr; cc += 16) {
+        typename MfmaAcc<T>::type acc = {};
+        int n1a = rt + r16, n2b = cc + r16;
+        for (int k0 = 0; k0 < nm; k0 += 4) {
+          int m = k0 + k4;
+          T av = (n1a < nr && m < nm) ? c[n1a * H + lm0 + m] : T(0);
+          T bv = (n2b < nr && m < nm) ? c[n2b * H + lm0 + m] : T(0);
+          acc = mfma_4(av, bv, acc);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          int n1 = rt + mfma_row<T>(lane, q), n2 = n2b;
+          if (n1 < nr && n2 < nr && n2 >= n1) {
+            int p = n1 * nr - (n1 * (n1 - 1)) / 2 + (n2 - n1);
+            Bst[p * L1 + l] = acc[q];
+          }
+        }
+      }
+    }
+  }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  for (int k = lane; k < nc; k += 64) Cg[(size_t)i * nc + k] = c[k];
   int row = a.row_of_atom[i];
   T *Brow = B + (size_t)row * a.nd;
-  int L1 = a.lmax + 1;
   T s2 = 0;
   for (int k = lane; k < a.nd; k += 64) {
-    int p = k / L1, l = k - p * L1;
-    int2 pr = a.pairs[p];
-    const T *c1 = c + pr.x * a.H + l * l, *c2 = c + pr.y * a.H + l * l;
-    T acc = 0;
-    for (int m = 0; m < 2 * l + 1; m++) acc += c1[m] * c2[m];
+    T acc = Bst[k];
     Brow[k] = acc;
     if constexpr (sizeof(T) == 4) {
       float r = acc;
@@ -268,9 +291,9 @@ __global__ __launch_bounds__(64 * kAtomsPerBlock) void k_b2(B2Args a, T *B, doub
 // sparse GP, G = sum_j sigma^2 alpha_j p S_j^(p-1) x_hat_j, the closed form of
 // normalized_dot_product.cpp:191-195); without it w = G (unnormalised
 // kernels). Then u = dE/dc (lammps_descriptor.cpp:286-314):
-// u[n1, lm] = sum_n2 (1 + [n1 = n2]) w[(min, max)(n1, n2), l] c[n2, lm], lanes
-// over lm, four n1 per pass. An atom with |B| < empty_norm or
-// B.B < empty_nn contributes nothing (its u is not read).
+// u[n1, lm] = sum_n2 (1 + [n1 = n2]) w[(min, max)(n1, n2), l] c[n2, lm]. An
+// atom with |B| < empty_norm or B.B < empty_nn contributes nothing (its u is
+// not read).
 struct AdjArgs {
   int project;
   double empty_norm, empty_nn;
@@ -311,31 +334,37 @@ __global__ __launch_bounds__(64 * kAtomsPerBlock) void k_dEdc(B2Args a, const T 
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  // u_l = W_l C_l per l on the matrix cores: W_l[n1][n2] = (1 + [n1 = n2])
+  // w[p(min, max), l] (nr x nr, symmetric), C_l[n2][m] = c[n2, l^2 + m]
+  // (nr x (2l + 1)); 16 x 16 output tiles, 4 n2 per MFMA. Lane L supplies
+  // A[L % 16][L / 16] and B[L / 16][L % 16]; it holds output rows
+  // tile_row(L, i), column L % 16.
   int L1 = a.lmax + 1, nr = a.nrad;
   T *urow = Ug + (size_t)i * nc;
-  // Lane groups: with H <= 32 the wave splits into ngrp = 64 / hp groups of
-  // hp lanes (hp = 16 or 32), group grp taking rows n1 = grp, grp + ngrp, ...
-  int hp = a.H <= 16 ? 16 : (a.H <= 32 ? 32 : 64);
-  int ngrp = 64 / hp, grp = lane / hp;
+  int r16 = lane & 15, k4 = lane >> 4;
+  for (int l = 0; l <= a.lmax; l++) {
+    int nm = 2 * l + 1, lm0 = l * l;
+    for (int rt = 0; rt < nr; rt += 16) {
+      for (int ct = 0; ct < nm; ct += 16) {
+        typename MfmaAcc<T>::type acc = {};
+        int n1 = rt + r16, m = ct + r16;
+        for (int k0 = 0; k0 < nr; k0 += 4) {
+          int n2 = k0 + k4;
+          T av = 0, bv = 0;
+          if (n1 < nr && n2 < nr) {
+            int lo = n1 < n2 ? n1 : n2, hi = n1 < n2 ? n2 : n1;
+            int p = lo * nr - (lo * (lo - 1)) / 2 + (hi - lo);
+            av = (n1 == n2 ? T(2) : T(1)) * wv[p * L1 + l];
+          }
+          if (n2 < nr && m < nm) bv = c[n2 * a.H + lm0 + m];
+          acc = mfma_4(av, bv, acc);
+        }
 #pragma unroll
-  for (int ch = 0; ch < kMaxHC; ch++) {
-    int lm = (lane % hp) + 64 * ch;
-    if (lm >= a.H) continue;
-    int l, m;
-    lane_lm<T>(lm, l, m);
-    const T *wl = wv + l;   // w[p, l] = wl[p L1]
-    const T *cl = c + lm;   // c[n, lm] = cl[n H]
-    // Pair p(lo, hi) = base(lo) + hi - lo, base(lo) = lo nr - lo (lo - 1) / 2.
-    for (int n1 = grp; n1 < nr; n1 += ngrp) {
-      T acc = 0;
-      int base = 0;
-      for (int n2 = 0; n2 < n1; n2++) {
-        acc += wl[(base + n1 - n2) * L1] * cl[n2 * a.H];
-        base += nr - n2;
+        for (int q = 0; q < 4; q++) {
+          int row = rt + mfma_row<T>(lane, q);
+          if (row < nr && m < nm) urow[row * a.H + lm0 + m] = acc[q];
+        }
       }
-      acc += T(2) * wl[base * L1] * cl[n1 * a.H];
-      for (int n2 = n1 + 1; n2 < nr; n2++) acc += wl[(base + n2 - n1) * L1] * cl[n2 * a.H];
-      urow[n1 * a.H + lm] = acc;
     }
   }
 }
@@ -500,7 +529,7 @@ static void upload_rcp(hipStream_t st) {
                                 st));
 }
 
-void launch_b2(fm_structure s) {
+void launch_b2(fm_structure s, bool want_b2) {
   fm_context ctx = s->ctx;
   hipStream_t st = ctx->stream;
   FM_CHECK(s->N <= kMaxN, FM_ERR_INVALID, "n_max above 16 is not supported");
@@ -509,39 +538,22 @@ void launch_b2(fm_structure s) {
   upload_rcp(st);
   B2Args a = make_args(s);
   size_t esz = s->form == FM_FORM_FP64 ? sizeof(double) : sizeof(float);
-  size_t per = ((size_t)a.nrad * a.H + 64 * (size_t)a.H) * esz;  // c and the Y table
+  size_t per = b2_work(a.nrad * a.H, a.H, a.N, want_b2 ? a.nd : 0) * esz;
   int apb = atoms_per_block(per);
   size_t lds = apb * per;
   int nblk = (s->n + apb - 1) / apb;
   s->C.alloc(&ctx->pool, std::max<size_t>(1, (size_t)s->n * a.nrad * a.H * esz));
   if (nblk == 0) return;
-  int ng = a.H <= 16 ? 4 : (a.H <= 32 ? 2 : 1);
   SplitOut none{{nullptr, nullptr}};
   SplitOut pieces{{s->Bs[0].as<__bf16>(), s->Bs[1].as<__bf16>()}};
-  int ldp = (int)split_ld(s->nd);
-  if (s->form == FM_FORM_FP64) {
-    double *Bp = s->B.as<double>(), *Cp = s->C.as<double>();
-    if (ng == 4)
-      hipLaunchKernelGGL((k_b2<double, 4>), dim3(nblk), dim3(64 * apb), lds, st, a, Bp,
-                         s->norm.as<double>(), s->nn.as<double>(), none, 0, Cp);
-    else if (ng == 2)
-      hipLaunchKernelGGL((k_b2<double, 2>), dim3(nblk), dim3(64 * apb), lds, st, a, Bp,
-                         s->norm.as<double>(), s->nn.as<double>(), none, 0, Cp);
-    else
-      hipLaunchKernelGGL((k_b2<double, 1>), dim3(nblk), dim3(64 * apb), lds, st, a, Bp,
-                         s->norm.as<double>(), s->nn.as<double>(), none, 0, Cp);
-  } else {
-    float *Bp = s->B.as<float>(), *Cp = s->C.as<float>();
-    if (ng == 4)
-      hipLaunchKernelGGL((k_b2<float, 4>), dim3(nblk), dim3(64 * apb), lds, st, a, Bp,
-                         s->norm.as<double>(), s->nn.as<double>(), pieces, ldp, Cp);
-    else if (ng == 2)
-      hipLaunchKernelGGL((k_b2<float, 2>), dim3(nblk), dim3(64 * apb), lds, st, a, Bp,
-                         s->norm.as<double>(), s->nn.as<double>(), pieces, ldp, Cp);
-    else
-      hipLaunchKernelGGL((k_b2<float, 1>), dim3(nblk), dim3(64 * apb), lds, st, a, Bp,
-                         s->norm.as<double>(), s->nn.as<double>(), pieces, ldp, Cp);
-  }
+  if (s->form == FM_FORM_FP64)
+    hipLaunchKernelGGL(k_b2<double>, dim3(nblk), dim3(64 * apb), lds, st, a, s->B.as<double>(),
+                       s->norm.as<double>(), s->nn.as<double>(), none, 0, s->C.as<double>(),
+                       (int)want_b2);
+  else
+    hipLaunchKernelGGL(k_b2<float>, dim3(nblk), dim3(64 * apb), lds, st, a, s->B.as<float>(),
+                       s->norm.as<double>(), s->nn.as<double>(), pieces,
+                       (int)split_ld(s->nd), s->C.as<float>(), (int)want_b2);
   FM_HIP(hipGetLastError());
 }
 
@@ -558,8 +570,8 @@ void launch_adjoint(fm_structure s, const void *G, int64_t g_ld, void *fbond,
   size_t nc = (size_t)a.nrad * a.H;
   DevBuf U(&ctx->pool, std::max<size_t>(1, (size_t)s->n * nc * esz));
   AdjArgs aa{project, empty_norm, empty_nn};
-  // dE/dc: c and w in LDS per atom.
-  size_t per1 = (nc + a.nd) * esz;
+  // dE/dc (B2): c and w in LDS per atom.
+  size_t per1 = (nc + (s->desc_kind == 2 ? a.nd : 0)) * esz;
   int apb1 = atoms_per_block(per1);
   int nblk1 = (s->n + apb1 - 1) / apb1;
   // Bond forces: u in LDS per atom.
@@ -567,18 +579,23 @@ void launch_adjoint(fm_structure s, const void *G, int64_t g_ld, void *fbond,
   int apb2 = atoms_per_block(per2);
   int nblk2 = (s->n + apb2 - 1) / apb2;
   if (s->n == 0) return;
+  if (s->desc_kind == 3)
+    launch_dEdc_b3(s, G, g_ld, project, empty_norm, empty_nn, U.as<void>());
   if (f64) {
-    hipLaunchKernelGGL(k_dEdc<double>, dim3(nblk1), dim3(64 * apb1), apb1 * per1, st, a,
-                       s->B.as<double>(), s->norm.as<double>(), s->nn.as<double>(),
-                       (const double *)G, (long long)g_ld, aa, s->C.as<double>(),
-                       U.as<double>());
+    if (s->desc_kind == 2)
+      hipLaunchKernelGGL(k_dEdc<double>, dim3(nblk1), dim3(64 * apb1), apb1 * per1, st, a,
+                         s->B.as<double>(), s->norm.as<double>(), s->nn.as<double>(),
+                         (const double *)G, (long long)g_ld, aa, s->C.as<double>(),
+                         U.as<double>());
     hipLaunchKernelGGL(k_bond_forces<double>, dim3(nblk2), dim3(64 * apb2), apb2 * per2, st, a,
                        s->norm.as<double>(), s->nn.as<double>(), aa, U.as<double>(),
                        (double *)fbond, virial_atom);
   } else {
-    hipLaunchKernelGGL(k_dEdc<float>, dim3(nblk1), dim3(64 * apb1), apb1 * per1, st, a,
-                       s->B.as<float>(), s->norm.as<double>(), s->nn.as<double>(),
-                       (const float *)G, (long long)g_ld, aa, s->C.as<float>(), U.as<float>());
+    if (s->desc_kind == 2)
+      hipLaunchKernelGGL(k_dEdc<float>, dim3(nblk1), dim3(64 * apb1), apb1 * per1, st, a,
+                         s->B.as<float>(), s->norm.as<double>(), s->nn.as<double>(),
+                         (const float *)G, (long long)g_ld, aa, s->C.as<float>(),
+                         U.as<float>());
     hipLaunchKernelGGL(k_bond_forces<float>, dim3(nblk2), dim3(64 * apb2), apb2 * per2, st, a,
                        s->norm.as<double>(), s->nn.as<double>(), aa, U.as<float>(),
                        (float *)fbond, virial_atom);

@@ -389,6 +389,8 @@ void compute_dervs(fm_structure s) {
   if (s->has_dervs) return;
   FM_CHECK(s->has_b2 && s->form == FM_FORM_FP64, FM_ERR_STATE,
            "dense descriptor derivatives need FP64 descriptors");
+  FM_CHECK(s->desc_kind == 2, FM_ERR_INVALID,
+           "dense descriptor derivatives are built for the B2 descriptor");
   fm_context ctx = s->ctx;
   hipStream_t st = ctx->stream;
   Pool *pool = &ctx->pool;

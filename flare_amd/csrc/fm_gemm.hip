@@ -43,6 +43,22 @@ struct TileOut {
   Acc v[TM][TN][4];
 };
 
+// Output tile of this workgroup. Workgroups are dealt to the 8 XCDs round
+// robin (id % 8); the dispatch id is remapped so each XCD gets a contiguous
+// run of logical tiles, column index fastest: the workgroups sharing a row
+// tile of A then share that XCD's L2.
+__device__ __forceinline__ void xcd_tile(int &tx, int &ty) {
+  constexpr int kXcd = 8;
+  int ntx = gridDim.x, nblk = gridDim.x * gridDim.y;
+  int pid = blockIdx.y * ntx + blockIdx.x;
+  int per = (nblk + kXcd - 1) / kXcd;
+  int tall = nblk % kXcd == 0 ? kXcd : nblk % kXcd;  // XCDs that get `per` ids
+  int xcd = pid % kXcd, local = pid / kXcd;
+  int id = xcd < tall ? xcd * per + local : tall * per + (xcd - tall) * (per - 1) + local;
+  tx = id % ntx;
+  ty = id / ntx;
+}
+
 template <typename Acc>
 __device__ __forceinline__ int tile_row(int lane, int i) {
   return sizeof(Acc) == 8 ? (lane >> 4) + 4 * i : 4 * (lane >> 4) + i;
@@ -140,7 +156,7 @@ __device__ __forceinline__ void ep_kernel(const EpArgs &e, const TileOut<Acc, TM
   __syncthreads();
   for (int q = threadIdx.x; q < BMc; q += blockDim.x) {
     int r = row0 + q;
-    if (r < M) e.part[(size_t)blockIdx.x * e.part_ld + r] = red[q] + red[BMc + q];
+    if (r < M) e.part[(size_t)(col0 / (32 * TN)) * e.part_ld + r] = red[q] + red[BMc + q];
   }
 }
 
@@ -193,7 +209,7 @@ __device__ __forceinline__ void ep_sumsq(const EpArgs &e, const TileOut<Acc, TM,
   __syncthreads();
   for (int q = threadIdx.x; q < BNc; q += blockDim.x) {
     int c = col0 + q;
-    if (c < N) e.part[(size_t)blockIdx.y * e.part_ld + c] = red[q] + red[BNc + q];
+    if (c < N) e.part[(size_t)(row0 / (32 * TM)) * e.part_ld + c] = red[q] + red[BNc + q];
   }
 }
 
@@ -224,7 +240,9 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K, const dou
   __shared__ double red[2 * (F64_BM > F64_BN ? F64_BM : F64_BN)];
   int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   int wr = wid >> 1, wc = wid & 1;
-  int row0 = blockIdx.y * F64_BM, col0 = blockIdx.x * F64_BN;
+  int tx, ty;
+  xcd_tile(tx, ty);
+  int row0 = ty * F64_BM, col0 = tx * F64_BN;
   int kend = K;
   if (lower_tri) kend = min(K, row0 + F64_BM);
   v4d acc[F64_TM][F64_TN];
@@ -306,7 +324,9 @@ __global__ __launch_bounds__(256) void k_gemm_split(int M, int N, int Kpad, Spli
   __shared__ double red[2 * (B16_BM > B16_BN ? B16_BM : B16_BN)];
   int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   int wr = wid >> 1, wc = wid & 1;
-  int row0 = blockIdx.y * B16_BM, col0 = blockIdx.x * B16_BN;
+  int tx, ty;
+  xcd_tile(tx, ty);
+  int row0 = ty * B16_BM, col0 = tx * B16_BN;
   v4f acc[B16_TM][B16_TN];
 #pragma unroll
   for (int a = 0; a < B16_TM; a++)

@@ -14,6 +14,8 @@
 
 namespace fm {
 
+struct B3State;
+
 struct Error : std::runtime_error {
   int code;
   Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
@@ -142,8 +144,10 @@ struct fm_structure_s {
   int sweep = 0;
   int64_t n_nb = 0;
   fm::DevBuf species, positions, wrapped, count, cumulative, sidx, nspecies, rel;
-  // B2 state
+  // Descriptor state (B2, or B3 built from the same bond coefficients)
   bool has_b2 = false;
+  int desc_kind = 2;           // 2: B2, 3: B3
+  fm::B3State *b3 = nullptr;   // B3 index tables (fm_b3.hip)
   int form = FM_FORM_FP64;
   int nos = 0, N = 0, lmax = 0, cutkind = 0, nd = 0;
   double r1 = 0;
@@ -209,8 +213,14 @@ void mapped_parse(const std::string &path, fm_mapped mp, std::vector<double> &be
 // Launchers (fm_neighbors.hip)
 void build_neighbors(fm_structure s);
 void build_bonds(fm_structure s);
-// fm_descriptors.hip
-void launch_b2(fm_structure s);
+// fm_descriptors.hip (want_b2 = false computes and keeps only c)
+void launch_b2(fm_structure s, bool want_b2 = true);
+// fm_b3.hip
+int b3_size(int nos, int N, int lmax);
+void launch_b3(fm_structure s);
+void launch_dEdc_b3(fm_structure s, const void *G, int64_t g_ld, int project,
+                    double empty_norm, double empty_nn, void *U);
+void free_b3_state(fm_structure s);
 void launch_adjoint(fm_structure s, const void *G, int64_t g_ld, void *fbond,
                     double *virial_atom, int project, double empty_norm, double empty_nn);
 // fm_predict.cpp

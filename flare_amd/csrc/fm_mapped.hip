@@ -167,6 +167,7 @@ void mapped_compute(fm_mapped mp, fm_structure s, double *atom_energy, double *f
   Pool *pool = &ctx->pool;
   FM_CHECK(s->has_b2 && s->form == FM_FORM_FP64, FM_ERR_STATE,
            "the structure needs FP64 B2 descriptors for the mapped potential");
+  FM_CHECK(s->desc_kind == 2, FM_ERR_INVALID, "pair_style flare evaluates the B2 descriptor");
   FM_CHECK(s->nos == mp->nos && s->N == mp->N && s->lmax == mp->lmax &&
                s->cutkind == mp->cutkind && s->r1 == 0.0,
            FM_ERR_INVALID, "structure descriptors do not match the potential's settings");

@@ -116,6 +116,11 @@ typedef struct {
  * descriptors stay resident. form is FM_FORM_FP64 or FM_FORM_CHOSEN (FP32). */
 int fm_structure_compute_b2(fm_structure s, const fm_b2_settings *settings,
                             int form);
+/* B3::compute_struc (descriptors/b3.cpp:80-443) with the same settings
+ * (l_max <= 4, the reference's Wigner 3j table); the descriptors stay resident
+ * and the prediction entry points accept them as they accept B2. */
+int fm_structure_compute_b3(fm_structure s, const fm_b2_settings *settings,
+                            int form);
 /* DescriptorValues sizes (descriptor.h:31-57): n_descriptors, and per species
  * n_clusters_by_type and n_neighbors_by_type (arrays of n_species). */
 int fm_structure_b2_sizes(fm_structure s, int *n_descriptors,

@@ -121,7 +121,20 @@ void compute_b2_struc(const Geometry &g, const NeighborList &nl,
     norms[i] = std::sqrt(nn);
   }
 
-  // b2.cpp:104-176: species-sorted repack.
+  pack_descriptor_values(g, nl, s, nd, bonds, B, dB, fdots, norms, with_dervs, out);
+}
+
+// b2.cpp:104-176 (b3.cpp:106-180): the species-sorted DescriptorValues
+// repack of per-atom rows, derivative blocks and bond data (atom order kept
+// within a species).
+void pack_descriptor_values(const Geometry &g, const NeighborList &nl, const B2Settings &s,
+                            int nd, const std::vector<std::vector<int>> &bonds,
+                            const std::vector<std::vector<double>> &B,
+                            const std::vector<std::vector<double>> &dB,
+                            const std::vector<std::vector<double>> &fdots,
+                            const std::vector<double> &norms, bool with_dervs,
+                            DescriptorValues &out) {
+  int noa = g.noa, nos = s.nos;
   out = DescriptorValues();
   out.n_descriptors = nd;
   out.n_types = nos;

@@ -1,4 +1,5 @@
 // Oracle: plain C surface of ref_cpu for ctypes (test infrastructure only).
+#include <complex>
 #include <cstring>
 #include <string>
 
@@ -72,6 +73,15 @@ void *ref_desc_new(void *geom, int nos, int N, int lmax, int cutoff_kind,
   auto *h = new DescHandle();
   B2Settings s = make_settings(nos, N, lmax, cutoff_kind, r1, cutoffs);
   compute_b2_struc(gh->g, gh->nl, s, with_dervs != 0, h->d);
+  return h;
+}
+// The same DescriptorValues for the B3 descriptor (b3.cpp:80-289).
+void *ref_desc_new_b3(void *geom, int nos, int N, int lmax, int cutoff_kind, double r1,
+                      const double *cutoffs, int with_dervs) {
+  auto *gh = (GeomHandle *)geom;
+  auto *h = new DescHandle();
+  B2Settings s = make_settings(nos, N, lmax, cutoff_kind, r1, cutoffs);
+  compute_b3_struc(gh->g, gh->nl, s, with_dervs != 0, h->d);
   return h;
 }
 void ref_desc_free(void *p) { delete (DescHandle *)p; }
@@ -261,6 +271,24 @@ int ref_pair_flare(const char *text, void *geom, double *atom_energy,
   std::memcpy(atom_energy, e.data(), e.size() * 8);
   std::memcpy(forces, f.data(), f.size() * 8);
   return 0;
+}
+
+double ref_wigner3j(int j1, int j2, int j3, int m1, int m2, int m3) {
+  return wigner3j(j1, j2, j3, m1, m2, m3);
+}
+int ref_b3_n_ls(int lmax) { return b3_n_ls(lmax); }
+
+// Complex Y_l^m and gradients, interleaved (re, im): out[8 H].
+void ref_complex_sph_harm(double x, double y, double z, int lmax, double *out) {
+  int H = (lmax + 1) * (lmax + 1);
+  std::vector<std::complex<double>> Y(H), Yx(H), Yy(H), Yz(H);
+  complex_sph_harm(Y.data(), Yx.data(), Yy.data(), Yz.data(), x, y, z, lmax);
+  const std::vector<std::complex<double>> *parts[4] = {&Y, &Yx, &Yy, &Yz};
+  for (int p = 0; p < 4; p++)
+    for (int k = 0; k < H; k++) {
+      out[2 * (p * H + k)] = (*parts[p])[k].real();
+      out[2 * (p * H + k) + 1] = (*parts[p])[k].imag();
+    }
 }
 
 void ref_sph_harm(double x, double y, double z, int lmax, double *out) {

@@ -9,6 +9,7 @@
 // (paths relative to the mir-group/flare source tree). The checks that pin
 // each part live in tests/test_oracle_*.py.
 #pragma once
+#include <complex>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -98,6 +99,28 @@ void compute_b2_struc(const Geometry &g, const NeighborList &nl,
 void single_bond_atom(const Geometry &g, const NeighborList &nl,
                       const B2Settings &s, int i, double *c);
 void b2_from_c(const B2Settings &s, const double *c, double *B);
+// Species-sorted repack shared by B2 and B3 (b2.cpp:104-176).
+void pack_descriptor_values(const Geometry &g, const NeighborList &nl, const B2Settings &s,
+                            int nd, const std::vector<std::vector<int>> &bonds,
+                            const std::vector<std::vector<double>> &B,
+                            const std::vector<std::vector<double>> &dB,
+                            const std::vector<std::vector<double>> &fdots,
+                            const std::vector<double> &norms, bool with_dervs,
+                            DescriptorValues &out);
+
+// ---------------------------------------------------------------------------
+// B3 descriptor (descriptors/b3.cpp); lmax <= 4 as the reference's Wigner
+// table (wigner3j.cpp:1983-1985). Settings as B2.
+// ---------------------------------------------------------------------------
+double wigner3j(int j1, int j2, int j3, int m1, int m2, int m3);
+// Complex Y_l^m with the Condon-Shortley phase, index l*l + (m + l)
+// (y_grad.cpp:24-665), and gradients.
+void complex_sph_harm(std::complex<double> *Y, std::complex<double> *Yx,
+                      std::complex<double> *Yy, std::complex<double> *Yz, double x, double y,
+                      double z, int lmax);
+int b3_n_ls(int lmax);
+void compute_b3_struc(const Geometry &g, const NeighborList &nl, const B2Settings &s,
+                      bool with_dervs, DescriptorValues &out);
 
 // ---------------------------------------------------------------------------
 // Sparse set and model state (descriptor.h:61-90, sparse_gp.h)

@@ -35,6 +35,14 @@ def lib(fast=False):
     L.ref_geom_get.argtypes = [vp, C.c_int, C.c_void_p]
     L.ref_desc_new.restype = vp
     L.ref_desc_new.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, _dp, C.c_int]
+    L.ref_desc_new_b3.restype = vp
+    L.ref_desc_new_b3.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, _dp,
+                                  C.c_int]
+    L.ref_wigner3j.restype = C.c_double
+    L.ref_wigner3j.argtypes = [C.c_int] * 6
+    L.ref_b3_n_ls.restype = C.c_int
+    L.ref_b3_n_ls.argtypes = [C.c_int]
+    L.ref_complex_sph_harm.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int, _dp]
     L.ref_desc_free.argtypes = [vp]
     L.ref_desc_sizes.argtypes = [vp, C.c_int, _ip]
     L.ref_desc_get.argtypes = [vp, C.c_int, C.c_int, C.c_void_p]
@@ -133,6 +141,8 @@ class B2Values:
               "neighbor_counts": 6, "cumulative_neighbor_counts": 7, "atom_indices": 8,
               "neighbor_indices": 9}
 
+    _new = "ref_desc_new"
+
     def __init__(self, nbrs, nos, N, lmax, cutoffs, cutoff_function="quadratic", r1=0.0,
                  with_dervs=True):
         self._L = nbrs._L
@@ -141,8 +151,8 @@ class B2Values:
         self.cutoffs = np.ascontiguousarray(cutoffs, dtype=np.float64).reshape(nos * nos)
         self.cutoff_kind = CUTOFF_KIND[cutoff_function]
         self.r1 = r1
-        self.h = self._L.ref_desc_new(nbrs.h, nos, N, lmax, self.cutoff_kind, r1, self.cutoffs,
-                                      1 if with_dervs else 0)
+        self.h = getattr(self._L, self._new)(nbrs.h, nos, N, lmax, self.cutoff_kind, r1,
+                                             self.cutoffs, 1 if with_dervs else 0)
         sz = np.zeros(8, np.int32)
         self._L.ref_desc_sizes(self.h, 0, sz)
         self.n_descriptors, self.n_types, self.n_atoms, self.n_clusters = map(int, sz[:4])
@@ -185,6 +195,31 @@ class B2Values:
             self._L.ref_desc_free(self.h)
         except Exception:
             pass
+
+
+class B3Values(B2Values):
+    """DescriptorValues of the B3 descriptor (b3.cpp:80-289), per species;
+    lmax <= 4."""
+
+    _new = "ref_desc_new_b3"
+
+
+def wigner3j(j1, j2, j3, m1, m2, m3):
+    return lib().ref_wigner3j(j1, j2, j3, m1, m2, m3)
+
+
+def b3_n_ls(lmax):
+    return lib().ref_b3_n_ls(lmax)
+
+
+def complex_sph_harm(x, y, z, lmax):
+    """Complex Y_l^m (Condon-Shortley phase, index l^2 + m + l) and its
+    Cartesian gradient: arrays Y, Yx, Yy, Yz of (lmax + 1)^2 complex."""
+    H = (lmax + 1) ** 2
+    out = np.zeros(8 * H)
+    lib().ref_complex_sph_harm(x, y, z, lmax, out)
+    z_ = out[0::2] + 1j * out[1::2]
+    return z_[:H], z_[H:2 * H], z_[2 * H:3 * H], z_[3 * H:]
 
 
 class Model:

@@ -1,0 +1,106 @@
+"""B3 on the GPU against the oracle (b3.cpp:80-443): descriptors in both
+forms, and E/F/stress/local variances of a B3 model (FP64 drop-in bar), a
+batch equal to its structures alone, and the chosen form's force error."""
+import numpy as np
+import pytest
+
+from helpers import model_state, random_cell, rel_err, var_err
+
+pytestmark = pytest.mark.gpu
+
+NOS, NMAX, LMAX = 2, 3, 3
+CUT = np.array([[4.6, 4.2], [4.2, 4.4]])
+TOL = 1e-10
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def C():
+    from flare_amd import _C_flare
+    return _C_flare
+
+
+@pytest.fixture(scope="module")
+def system(C, oracle):
+    cell, species, pos = random_cell(20, 41, nos=NOS, scale=7.0, skew=0.4)
+    calc = C.B3("chebyshev", "quadratic", [0.0, 4.6], [], [NOS, NMAX, LMAX], CUT)
+    ref = oracle.B3Values(oracle.Neighbors(cell, species, pos, 4.6, "literal"), NOS, NMAX, LMAX,
+                          CUT)
+    return dict(cell=cell, species=species, pos=pos, calc=calc, ref=ref)
+
+
+@pytest.mark.parametrize("form", [0, 1])
+def test_b3_descriptors(C, system, form):
+    say("B3 descriptors, form", form)
+    s = C.Structure(system["cell"], system["species"], system["pos"], 4.6, [system["calc"]],
+                    form=form)
+    d, ref = s.descriptors[0], system["ref"]
+    assert d.n_descriptors == ref.n_descriptors == system["calc"].n_descriptors
+    tol = TOL if form == 0 else 2e-5
+    for t in range(NOS):
+        assert rel_err(d.descriptors[t], ref.descriptors[t]) < tol
+        assert rel_err(d.descriptor_norms[t], ref.descriptor_norms[t]) < tol
+        assert list(d.atom_indices[t]) == list(ref.atom_indices[t])
+
+
+def model(C, oracle, system, power):
+    ref = system["ref"]
+    picks = [list(range(min(4, ref.n_clusters_by_type[t]))) for t in range(NOS)]
+    X, N, alpha, Linv = model_state(ref, picks, 1.3, power, seed=11)
+    m = oracle.Model(1.3, power, X, N, alpha, L_inv=Linv)
+    s = C.Structure(system["cell"], system["species"], system["pos"], 4.6, [system["calc"]])
+    sgp = C.SparseGP([C.NormalizedDotProduct(1.3, power)], 0.01, 0.05, 0.005)
+    sgp.add_specific_environments(s, [int(ref.atom_indices[t][k]) for t in range(NOS)
+                                      for k in picks[t]])
+    sgp.set_state(alpha, L_inv=Linv)
+    return m, sgp
+
+
+@pytest.mark.parametrize("power", [1.0, 2.0])
+def test_b3_predict_fp64(C, oracle, system, power):
+    say("B3 predict, power", power)
+    m, sgp = model(C, oracle, system, power)
+    s = C.Structure(system["cell"], system["species"], system["pos"], 4.6, [system["calc"]])
+    sgp.predict_local_uncertainties(s)
+    mean, var = m.predict(system["ref"], "local")
+    n = s.noa
+    assert abs(s.mean_efs[0] - mean[0]) / abs(mean[0]) < TOL
+    assert rel_err(s.mean_efs[1:1 + 3 * n], mean[1:1 + 3 * n]) < TOL
+    assert rel_err(s.mean_efs[-6:], mean[-6:]) < TOL
+    assert var_err(s.local_uncertainties[0], var, 1.3 ** 2) < TOL
+
+
+def test_b3_batch_and_chosen_form(C, oracle, system):
+    say("B3 batch and chosen form")
+    from flare_amd.batch import Batch
+    m, sgp = model(C, oracle, system, 2.0)
+    structs = [(system["cell"], system["species"], system["pos"])]
+    cell2, sp2, pos2 = random_cell(14, 43, nos=NOS, scale=6.5, skew=0.3)
+    structs.append((cell2, sp2, pos2))
+    for form in (0, 1):
+        means, vars_ = Batch(structs, 4.6, system["calc"], form=form).predict(sgp)
+        for k, st in enumerate(structs):
+            s = C.Structure(*st, 4.6, [system["calc"]], form=form)
+            sgp.predict_local_uncertainties(s)
+            assert np.array_equal(means[k], s.mean_efs), (form, k)
+            assert np.array_equal(vars_[k], s.local_uncertainties[0]), (form, k)
+    s64 = C.Structure(*structs[0], 4.6, [system["calc"]])
+    s32 = C.Structure(*structs[0], 4.6, [system["calc"]], form=1)
+    sgp.predict_mean(s64)
+    sgp.predict_mean(s32)
+    n = s64.noa
+    dF = np.abs(s32.mean_efs[1:1 + 3 * n] - s64.mean_efs[1:1 + 3 * n]).max()
+    say(f"B3 chosen form: dF {dF * 1e3:.3e} meV/A")
+    assert dF < 1e-3
+
+
+def test_b3_limits(C, system):
+    with pytest.raises(ValueError, match="l_max"):
+        C.Structure(system["cell"], system["species"], system["pos"], 4.6,
+                    [C.B3("chebyshev", "quadratic", [0.0, 4.6], [], [NOS, 2, 5], CUT)])
+    s = C.Structure(system["cell"], system["species"], system["pos"], 4.6, [system["calc"]])
+    with pytest.raises(RuntimeError, match="B2"):
+        s.descriptors[0].descriptor_force_dervs
