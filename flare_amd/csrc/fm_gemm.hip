@@ -22,6 +22,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
 
 #include "fm_device.hpp"
 #include "fm_gemm.hpp"
@@ -223,42 +226,53 @@ __device__ __forceinline__ void run_epilogue(const EpArgs &e, const TileOut<Acc,
 }
 
 // ---------------------------------------------------------------------------
-// FP64 MFMA GEMM, 64 x 64 tiles. Grid: x = column tiles (rows of B),
-// y = row tiles (rows of A). LDS holds k-major tiles so a fragment read
-// (16 rows at one k) is contiguous.
+// FP64 MFMA GEMM, (32 TM) x (32 TN) tiles; 4 waves (2 x 2), each TM x TN
+// MFMA tiles of 16 x 16. LDS holds k-major tiles so a fragment read (16 rows
+// at one k) is contiguous. Each thread loads 4 consecutive k of rows
+// tid / 4 + 64 j.
 // ---------------------------------------------------------------------------
+// Base shape: the partial arrays (gemm_col_tiles / gemm_row_tiles) follow it.
 constexpr int F64_TM = 2, F64_TN = 2, F64_BM = 32 * F64_TM, F64_BN = 32 * F64_TN;
 constexpr int BK64 = 16;
 constexpr int PAD64 = 4;
 
-template <int EP>
+template <int EP, int TM, int TN>
 __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K, const double *A, long long lda,
                                                   const double *B, long long ldb, int lower_tri,
                                                   EpArgs e) {
-  __shared__ double As[BK64][F64_BM + PAD64];
-  __shared__ double Bs[BK64][F64_BN + PAD64];
-  __shared__ double red[2 * (F64_BM > F64_BN ? F64_BM : F64_BN)];
+  constexpr int BM = 32 * TM, BN = 32 * TN, RA = BM / 64, RB = BN / 64;
+  static_assert(BM % 64 == 0 && BN % 64 == 0, "tiles are multiples of 64");
+  __shared__ double As[BK64][BM + PAD64];
+  __shared__ double Bs[BK64][BN + PAD64];
+  __shared__ double red[2 * (BM > BN ? BM : BN)];
   int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   int wr = wid >> 1, wc = wid & 1;
   int tx, ty;
   xcd_tile(tx, ty);
-  int row0 = ty * F64_BM, col0 = tx * F64_BN;
+  int row0 = ty * BM, col0 = tx * BN;
   int kend = K;
-  if (lower_tri) kend = min(K, row0 + F64_BM);
-  v4d acc[F64_TM][F64_TN];
+  if (lower_tri) kend = min(K, row0 + BM);
+  v4d acc[TM][TN];
 #pragma unroll
-  for (int a = 0; a < F64_TM; a++)
+  for (int a = 0; a < TM; a++)
 #pragma unroll
-    for (int b = 0; b < F64_TN; b++) acc[a][b] = v4d{0, 0, 0, 0};
+    for (int b = 0; b < TN; b++) acc[a][b] = v4d{0, 0, 0, 0};
   int lr = tid >> 2, lk = (tid & 3) * 4;
-  double ra[4], rb[4];
+  double ra[RA][4], rb[RB][4];
   auto gload = [&](int k0) {
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       int k = k0 + lk + q;
-      int r = row0 + lr, c = col0 + lr;
-      ra[q] = (r < M && k < kend) ? A[(size_t)r * lda + k] : 0.0;
-      rb[q] = (c < N && k < kend) ? B[(size_t)c * ldb + k] : 0.0;
+#pragma unroll
+      for (int j = 0; j < RA; j++) {
+        int r = row0 + lr + 64 * j;
+        ra[j][q] = (r < M && k < kend) ? A[(size_t)r * lda + k] : 0.0;
+      }
+#pragma unroll
+      for (int j = 0; j < RB; j++) {
+        int c = col0 + lr + 64 * j;
+        rb[j][q] = (c < N && k < kend) ? B[(size_t)c * ldb + k] : 0.0;
+      }
     }
   };
   gload(0);
@@ -266,45 +280,47 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K, const dou
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-      As[lk + q][lr] = ra[q];
-      Bs[lk + q][lr] = rb[q];
+#pragma unroll
+      for (int j = 0; j < RA; j++) As[lk + q][lr + 64 * j] = ra[j][q];
+#pragma unroll
+      for (int j = 0; j < RB; j++) Bs[lk + q][lr + 64 * j] = rb[j][q];
     }
     __syncthreads();
     if (k0 + BK64 < kend) gload(k0 + BK64);
 #pragma unroll
     for (int kk = 0; kk < BK64; kk += 4) {
-      double af[F64_TM], bf[F64_TN];
+      double af[TM], bf[TN];
 #pragma unroll
-      for (int t = 0; t < F64_TM; t++) af[t] = As[kk + (lane >> 4)][wr * 16 * F64_TM + t * 16 + (lane & 15)];
+      for (int t = 0; t < TM; t++) af[t] = As[kk + (lane >> 4)][wr * 16 * TM + t * 16 + (lane & 15)];
 #pragma unroll
-      for (int t = 0; t < F64_TN; t++) bf[t] = Bs[kk + (lane >> 4)][wc * 16 * F64_TN + t * 16 + (lane & 15)];
+      for (int t = 0; t < TN; t++) bf[t] = Bs[kk + (lane >> 4)][wc * 16 * TN + t * 16 + (lane & 15)];
 #pragma unroll
-      for (int a = 0; a < F64_TM; a++)
+      for (int a = 0; a < TM; a++)
 #pragma unroll
-        for (int b = 0; b < F64_TN; b++)
+        for (int b = 0; b < TN; b++)
           acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[a], bf[b], acc[a][b], 0, 0, 0);
     }
   }
-  TileOut<double, F64_TM, F64_TN> t;
+  TileOut<double, TM, TN> t;
 #pragma unroll
-  for (int a = 0; a < F64_TM; a++)
+  for (int a = 0; a < TM; a++)
 #pragma unroll
-    for (int b = 0; b < F64_TN; b++)
+    for (int b = 0; b < TN; b++)
 #pragma unroll
       for (int i = 0; i < 4; i++) t.v[a][b][i] = acc[a][b][i];
-  run_epilogue<EP, double, double, F64_TM, F64_TN>(e, t, row0, col0, M, N, lane, wr, wc, red);
+  run_epilogue<EP, double, double, TM, TN>(e, t, row0, col0, M, N, lane, wr, wc, red);
 }
 
 // ---------------------------------------------------------------------------
-// Split-BF16 x3 GEMM (chosen form), 128 x 128 tiles, on operands stored as
+// Split-BF16 x3 GEMM (chosen form), (32 TM) x (32 TN) tiles, on operands stored as
 // two BF16 pieces of FP32 values (x = x_0 + x_1, split once where the operand
 // is produced: the sparse set at model upload, descriptors in the descriptor
 // kernel, w in the kernel-contraction epilogue). The products x_1 y_0,
-// x_0 y_1 and x_0 y_0 are summed in that order (smallest first). Each thread copies
-// 16 consecutive k of one row of A and of B per piece and k-step as 16-byte
-// vectors into k-contiguous LDS rows; lane l then reads 8 consecutive k at
-// row l % 16, k offset 8 (l / 16).
+// x_0 y_1 and x_0 y_0 are summed in that order (smallest first). Threads copy
+// 16-byte vectors of consecutive k into k-contiguous LDS rows (SplitLoader);
+// lane l then reads 8 consecutive k at row l % 16, k offset 8 (l / 16).
 // ---------------------------------------------------------------------------
+// Base shape of the split contraction (partials follow it).
 constexpr int B16_TM = 4, B16_TN = 4, B16_BM = 32 * B16_TM, B16_BN = 32 * B16_TN;
 constexpr int BK16 = 32;
 constexpr int PADB = 8;
@@ -315,76 +331,94 @@ struct SplitPtrs {
   const __bf16 *p[3];
 };
 
-template <int EP, int NP>
+// Loader of one operand's rows [row0, row0 + BR) x 32 k, NP pieces: with
+// TPR = 256 / BR threads per row, thread tid takes row tid / TPR and
+// 32 / TPR consecutive k (one or two 16-byte vectors). Rows past `rows` read
+// row 0 (in bounds) and load zeros.
+template <int NP, int BR>
+struct SplitLoader {
+  static constexpr int TPR = 256 / BR, VPT = 4 / TPR;
+  int lr, lk;
+  bool valid;
+  size_t off;
+  v8bf r[NP][VPT];
+  __device__ __forceinline__ SplitLoader(int tid, int row0, int rows, long long ld) {
+    lr = tid / TPR;
+    lk = (tid % TPR) * 8 * VPT;
+    valid = row0 + lr < rows;
+    off = (size_t)(valid ? row0 + lr : 0) * ld + lk;
+  }
+  __device__ __forceinline__ void load(const SplitPtrs &g, int k0) {
+    const v8bf zero = {};
+    int q = k0 / 8;
+#pragma unroll
+    for (int pc = 0; pc < NP; pc++) {
+      const v8bf *p = reinterpret_cast<const v8bf *>(g.p[pc] + off);
+#pragma unroll
+      for (int v = 0; v < VPT; v++) r[pc][v] = valid ? p[q + v] : zero;
+    }
+  }
+  __device__ __forceinline__ void store(__bf16 (*lds)[BR * LDK]) const {
+#pragma unroll
+    for (int pc = 0; pc < NP; pc++)
+#pragma unroll
+      for (int v = 0; v < VPT; v++)
+        *reinterpret_cast<v8bf *>(&lds[pc][lr * LDK + lk + 8 * v]) = r[pc][v];
+  }
+};
+
+template <int EP, int NP, int TM, int TN>
 __global__ __launch_bounds__(256) void k_gemm_split(int M, int N, int Kpad, SplitPtrs Ag,
                                                     long long lda, SplitPtrs Bg, long long ldb,
                                                     EpArgs e) {
-  __shared__ __align__(16) __bf16 As[NP][B16_BM * LDK];
-  __shared__ __align__(16) __bf16 Bs[NP][B16_BN * LDK];
-  __shared__ double red[2 * (B16_BM > B16_BN ? B16_BM : B16_BN)];
+  constexpr int BM = 32 * TM, BN = 32 * TN;
+  __shared__ __align__(16) __bf16 As[NP][BM * LDK];
+  __shared__ __align__(16) __bf16 Bs[NP][BN * LDK];
+  __shared__ double red[2 * (BM > BN ? BM : BN)];
   int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   int wr = wid >> 1, wc = wid & 1;
   int tx, ty;
   xcd_tile(tx, ty);
-  int row0 = ty * B16_BM, col0 = tx * B16_BN;
-  v4f acc[B16_TM][B16_TN];
+  int row0 = ty * BM, col0 = tx * BN;
+  v4f acc[TM][TN];
 #pragma unroll
-  for (int a = 0; a < B16_TM; a++)
+  for (int a = 0; a < TM; a++)
 #pragma unroll
-    for (int b = 0; b < B16_TN; b++) acc[a][b] = v4f{0, 0, 0, 0};
-  // 128 rows x 32 k per operand and piece: thread -> row tid / 2, k half
-  // (tid & 1) * 16. Rows past M or N read row 0 (in bounds) and load zeros.
-  int lr = tid >> 1, lk = (tid & 1) * 16;
-  bool va = row0 + lr < M, vb = col0 + lr < N;
-  size_t oa = (size_t)(va ? row0 + lr : 0) * lda + lk;
-  size_t ob = (size_t)(vb ? col0 + lr : 0) * ldb + lk;
-  const v8bf zero = {};
-  v8bf ra[NP][2], rb[NP][2];
-  auto gload = [&](int k0) {
-    int q = k0 / 8;
-#pragma unroll
-    for (int pc = 0; pc < NP; pc++) {
-      const v8bf *pa = reinterpret_cast<const v8bf *>(Ag.p[pc] + oa);
-      const v8bf *pb = reinterpret_cast<const v8bf *>(Bg.p[pc] + ob);
-      ra[pc][0] = va ? pa[q] : zero;
-      ra[pc][1] = va ? pa[q + 1] : zero;
-      rb[pc][0] = vb ? pb[q] : zero;
-      rb[pc][1] = vb ? pb[q + 1] : zero;
-    }
-  };
-  gload(0);
+    for (int b = 0; b < TN; b++) acc[a][b] = v4f{0, 0, 0, 0};
+  SplitLoader<NP, BM> la(tid, row0, M, lda);
+  SplitLoader<NP, BN> lb(tid, col0, N, ldb);
+  la.load(Ag, 0);
+  lb.load(Bg, 0);
   for (int k0 = 0; k0 < Kpad; k0 += BK16) {
     __syncthreads();
-#pragma unroll
-    for (int pc = 0; pc < NP; pc++) {
-      *reinterpret_cast<v8bf *>(&As[pc][lr * LDK + lk]) = ra[pc][0];
-      *reinterpret_cast<v8bf *>(&As[pc][lr * LDK + lk + 8]) = ra[pc][1];
-      *reinterpret_cast<v8bf *>(&Bs[pc][lr * LDK + lk]) = rb[pc][0];
-      *reinterpret_cast<v8bf *>(&Bs[pc][lr * LDK + lk + 8]) = rb[pc][1];
-    }
+    la.store(As);
+    lb.store(Bs);
     __syncthreads();
-    if (k0 + BK16 < Kpad) gload(k0 + BK16);
-    v8bf af[NP][B16_TM], bf[NP][B16_TN];
+    if (k0 + BK16 < Kpad) {
+      la.load(Ag, k0 + BK16);
+      lb.load(Bg, k0 + BK16);
+    }
+    v8bf af[NP][TM], bf[NP][TN];
     int kk = 8 * (lane >> 4);
 #pragma unroll
     for (int pc = 0; pc < NP; pc++) {
 #pragma unroll
-      for (int t = 0; t < B16_TM; t++) {
-        int ar = wr * 16 * B16_TM + t * 16 + (lane & 15);
+      for (int t = 0; t < TM; t++) {
+        int ar = wr * 16 * TM + t * 16 + (lane & 15);
         af[pc][t] = *reinterpret_cast<const v8bf *>(&As[pc][ar * LDK + kk]);
       }
 #pragma unroll
-      for (int t = 0; t < B16_TN; t++) {
-        int br = wc * 16 * B16_TN + t * 16 + (lane & 15);
+      for (int t = 0; t < TN; t++) {
+        int br = wc * 16 * TN + t * 16 + (lane & 15);
         bf[pc][t] = *reinterpret_cast<const v8bf *>(&Bs[pc][br * LDK + kk]);
       }
     }
     // Products of piece orders i + j <= NP - 1, smallest terms first.
-    auto prod = [&](const v8bf (&x)[B16_TM], const v8bf (&y)[B16_TN]) {
+    auto prod = [&](const v8bf (&x)[TM], const v8bf (&y)[TN]) {
 #pragma unroll
-      for (int a = 0; a < B16_TM; a++)
+      for (int a = 0; a < TM; a++)
 #pragma unroll
-        for (int b = 0; b < B16_TN; b++)
+        for (int b = 0; b < TN; b++)
           acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(x[a], y[b], acc[a][b], 0, 0, 0);
     };
     static_assert(NP == 2, "two pieces: products lo.hi, hi.lo, hi.hi");
@@ -392,14 +426,14 @@ __global__ __launch_bounds__(256) void k_gemm_split(int M, int N, int Kpad, Spli
     prod(af[0], bf[1]);
     prod(af[0], bf[0]);
   }
-  TileOut<float, B16_TM, B16_TN> t;
+  TileOut<float, TM, TN> t;
 #pragma unroll
-  for (int a = 0; a < B16_TM; a++)
+  for (int a = 0; a < TM; a++)
 #pragma unroll
-    for (int b = 0; b < B16_TN; b++)
+    for (int b = 0; b < TN; b++)
 #pragma unroll
       for (int i = 0; i < 4; i++) t.v[a][b][i] = acc[a][b][i];
-  run_epilogue<EP, float, float, B16_TM, B16_TN>(e, t, row0, col0, M, N, lane, wr, wc, red);
+  run_epilogue<EP, float, float, TM, TN>(e, t, row0, col0, M, N, lane, wr, wc, red);
 }
 
 int gemm_col_tiles(int form, int N) {
@@ -412,35 +446,126 @@ int gemm_row_tiles(int form, int M) {
   return (M + bm - 1) / bm;
 }
 
+// ---------------------------------------------------------------------------
+// Tile autotuner. The first contraction of a given shape (epilogue, M rounded
+// up to a power of two, N, K, device) times every admissible tile shape once
+// on the caller's stream (after a warm-up launch) and keeps the fastest for
+// later calls. Admissible shapes never change results: the energy partials
+// of EP_KERNEL are per column tile and the |Q|^2 partials of EP_SUMSQ per
+// row tile, so EP_KERNEL only varies TM and EP_SUMSQ only TN; every output
+// element sums its k terms in the same order whatever the tile.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct TileShape {
+  int tm, tn;
+};
+constexpr TileShape kF64Shapes[] = {{2, 2}, {4, 2}, {2, 4}, {4, 4}};
+constexpr TileShape kSplitShapes[] = {{4, 4}, {2, 4}, {4, 2}};
+
+bool admissible(int ep, TileShape t, TileShape base) {
+  if (ep == EP_KERNEL) return t.tn == base.tn;
+  if (ep == EP_SUMSQ) return t.tm == base.tm;
+  return true;
+}
+
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, int, int, int, int, int>, int> g_tune;
+
+int next_pow2(int x) {
+  int p = 64;
+  while (p < x) p <<= 1;
+  return p;
+}
+
+// Returns the tuned shape index for `key`, timing `launch(i)` for each
+// admissible index when the key is new. `launch` enqueues one contraction.
+template <typename Launch>
+int tuned(int family, int ep, int M, int N, int K, hipStream_t st, const TileShape *shapes,
+          int n_shapes, Launch &&launch) {
+  int dev = 0;
+  FM_HIP(hipGetDevice(&dev));
+  auto key = std::make_tuple(family, ep, next_pow2(M), N, K, dev, 0);
+  {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    auto it = g_tune.find(key);
+    if (it != g_tune.end()) return it->second;
+  }
+  hipEvent_t e0, e1;
+  FM_HIP(hipEventCreate(&e0));
+  FM_HIP(hipEventCreate(&e1));
+  int best = -1;
+  float best_ms = 0;
+  for (int i = 0; i < n_shapes; i++) {
+    if (!admissible(ep, shapes[i], shapes[0])) continue;
+    launch(i);  // warm-up (code object load, caches)
+    FM_HIP(hipEventRecord(e0, st));
+    launch(i);
+    FM_HIP(hipEventRecord(e1, st));
+    FM_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    FM_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (best < 0 || ms < best_ms) {
+      best = i;
+      best_ms = ms;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  g_tune[key] = best;
+  return best;
+}
+
+template <int EP, int TM, int TN>
+void f64_launch(hipStream_t st, int M, int N, int K, const double *A, long long lda,
+                const double *B, long long ldb, int lower_tri, const EpArgs &e) {
+  dim3 grid((N + 32 * TN - 1) / (32 * TN), (M + 32 * TM - 1) / (32 * TM));
+  hipLaunchKernelGGL((k_gemm_f64<EP, TM, TN>), grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb,
+                     lower_tri, e);
+}
+
+template <int EP>
+void f64_shape(int i, hipStream_t st, int M, int N, int K, const double *A, long long lda,
+               const double *B, long long ldb, int lower_tri, const EpArgs &e) {
+  switch (i) {
+    case 0: f64_launch<EP, 2, 2>(st, M, N, K, A, lda, B, ldb, lower_tri, e); break;
+    case 1: f64_launch<EP, 4, 2>(st, M, N, K, A, lda, B, ldb, lower_tri, e); break;
+    case 2: f64_launch<EP, 2, 4>(st, M, N, K, A, lda, B, ldb, lower_tri, e); break;
+    default: f64_launch<EP, 4, 4>(st, M, N, K, A, lda, B, ldb, lower_tri, e); break;
+  }
+}
+
+template <int EP, int TM, int TN>
+void split_launch(hipStream_t st, int M, int N, int Kpad, const SplitPtrs &a, long long lda,
+                  const SplitPtrs &b, long long ldb, const EpArgs &e) {
+  dim3 grid((N + 32 * TN - 1) / (32 * TN), (M + 32 * TM - 1) / (32 * TM));
+  hipLaunchKernelGGL((k_gemm_split<EP, 2, TM, TN>), grid, dim3(256), 0, st, M, N, Kpad, a, lda,
+                     b, ldb, e);
+}
+
+template <int EP>
+void split_shape(int i, hipStream_t st, int M, int N, int Kpad, const SplitPtrs &a,
+                 long long lda, const SplitPtrs &b, long long ldb, const EpArgs &e) {
+  switch (i) {
+    case 0: split_launch<EP, 4, 4>(st, M, N, Kpad, a, lda, b, ldb, e); break;
+    case 1: split_launch<EP, 2, 4>(st, M, N, Kpad, a, lda, b, ldb, e); break;
+    default: split_launch<EP, 4, 2>(st, M, N, Kpad, a, lda, b, ldb, e); break;
+  }
+}
+
+}  // namespace
+
 void gemm_f64(hipStream_t st, int ep, int M, int N, int K, const double *A, long long lda,
               const double *B, long long ldb, int lower_tri, const EpArgs &e) {
   if (M <= 0 || N <= 0) return;
-  dim3 grid((N + F64_BN - 1) / F64_BN, (M + F64_BM - 1) / F64_BM);
-  if (ep == EP_KERNEL)
-    hipLaunchKernelGGL(k_gemm_f64<EP_KERNEL>, grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb,
-                       lower_tri, e);
-  else if (ep == EP_STORE)
-    hipLaunchKernelGGL(k_gemm_f64<EP_STORE>, grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb,
-                       lower_tri, e);
-  else
-    hipLaunchKernelGGL(k_gemm_f64<EP_SUMSQ>, grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb,
-                       lower_tri, e);
-  FM_HIP(hipGetLastError());
-}
-
-template <int NP>
-static void launch_split(hipStream_t st, int ep, dim3 grid, int M, int N, int Kpad,
-                         const SplitPtrs &a, long long lda, const SplitPtrs &b, long long ldb,
-                         const EpArgs &e) {
-  if (ep == EP_KERNEL)
-    hipLaunchKernelGGL((k_gemm_split<EP_KERNEL, NP>), grid, dim3(256), 0, st, M, N, Kpad, a, lda,
-                       b, ldb, e);
-  else if (ep == EP_STORE)
-    hipLaunchKernelGGL((k_gemm_split<EP_STORE, NP>), grid, dim3(256), 0, st, M, N, Kpad, a, lda,
-                       b, ldb, e);
-  else
-    hipLaunchKernelGGL((k_gemm_split<EP_SUMSQ, NP>), grid, dim3(256), 0, st, M, N, Kpad, a, lda,
-                       b, ldb, e);
+  auto launch = [&](int i) {
+    if (ep == EP_KERNEL) f64_shape<EP_KERNEL>(i, st, M, N, K, A, lda, B, ldb, lower_tri, e);
+    else if (ep == EP_STORE) f64_shape<EP_STORE>(i, st, M, N, K, A, lda, B, ldb, lower_tri, e);
+    else f64_shape<EP_SUMSQ>(i, st, M, N, K, A, lda, B, ldb, lower_tri, e);
+    FM_HIP(hipGetLastError());
+  };
+  launch(tuned(0, ep, M, N, K, st, kF64Shapes, 4, launch));
 }
 
 void gemm_split(hipStream_t st, int ep, int np, int M, int N, int Kpad, const SplitOperand &A,
@@ -458,9 +583,13 @@ void gemm_split(hipStream_t st, int ep, int np, int M, int N, int Kpad, const Sp
     a.p[pc] = reinterpret_cast<const __bf16 *>(A.piece[pc]);
     b.p[pc] = reinterpret_cast<const __bf16 *>(B.piece[pc]);
   }
-  dim3 grid((N + B16_BN - 1) / B16_BN, (M + B16_BM - 1) / B16_BM);
-  launch_split<2>(st, ep, grid, M, N, Kpad, a, A.ld, b, B.ld, e);
-  FM_HIP(hipGetLastError());
+  auto launch = [&](int i) {
+    if (ep == EP_KERNEL) split_shape<EP_KERNEL>(i, st, M, N, Kpad, a, A.ld, b, B.ld, e);
+    else if (ep == EP_STORE) split_shape<EP_STORE>(i, st, M, N, Kpad, a, A.ld, b, B.ld, e);
+    else split_shape<EP_SUMSQ>(i, st, M, N, Kpad, a, A.ld, b, B.ld, e);
+    FM_HIP(hipGetLastError());
+  };
+  launch(tuned(1, ep, M, N, Kpad, st, kSplitShapes, 3, launch));
 }
 
 static uint16_t bf16_rne(float x) {
