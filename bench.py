@@ -198,6 +198,10 @@ def serving_leg(sgp, calc, batched, n_requests=2000, n_fp64=200):
         srv.close()
     st = latency_stats(reqs)
     atoms = sum(len(s[1]) for s in reqs_s)
+    # Batch wall time against its atoms: the intercept is the fixed cost of a
+    # batch (host work, launches and synchronisations), the slope its rate.
+    bt = np.array([(b[1], b[2]) for b in srv.batches], np.float64)
+    slope, icpt = np.polyfit(bt[:, 0], bt[:, 1], 1) if len(bt) > 2 else (float("nan"),) * 2
     # FP64 drop-in, request by request, on the first n_fp64 requests.
     lat64 = []
     t0 = time.perf_counter()
@@ -212,6 +216,8 @@ def serving_leg(sgp, calc, batched, n_requests=2000, n_fp64=200):
                         f"rate {rate:.1f}/s (0.7x batched chosen throughput)",
             "atom_evals_per_s": atoms / wall, "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
             "batches": len(srv.batches),
+            "mean_batch_atoms": float(bt[:, 0].mean()),
+            "batch_time_fit": {"fixed_ms": float(icpt * 1e3), "ns_per_atom": float(slope * 1e9)},
             "fp64_request_by_request": {"atom_evals_per_s": atoms64 / dt64,
                                         "p50_ms": float(np.percentile(lat64, 50) * 1e3),
                                         "p99_ms": float(np.percentile(lat64, 99) * 1e3),
@@ -235,6 +241,22 @@ def cpu_baseline(sgp, threads):
     return {"value": len(pos) / dt, "unit": "atom-evals/s", "cores": threads, "kind": "port",
             "sample": f"{len(pos)}-atom 2-species FCC alloy (10^3 cells, a = 4.0 A), same model "
                       f"(2000 sparse, d = 2100), one FP64 evaluation in {dt:.2f} s"}
+
+
+def measured_traffic():
+    """HBM bytes per launch of the kernel contraction, per form, as measured
+    with rocprofv3 FETCH_SIZE + WRITE_SIZE (calibrated against a plain copy)
+    and recorded in profiles/roofline_traffic.md; None when absent."""
+    import re
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "roofline_traffic.md")
+    out = {}
+    if os.path.exists(path):
+        for line in open(path):
+            m = re.match(r"^traffic_bytes_per_launch (\w+) ([0-9.eE+-]+)\s*$", line.strip())
+            if m:
+                out[m.group(1)] = float(m.group(2))
+    return out
 
 
 def sigma_errors(v64, vch, sig_f):
@@ -265,11 +287,24 @@ def main():
                     help="single: the 32k-atom structure only; all: also the batched and "
                          "served workloads; batched: the batched workload alone (its own "
                          "JSON line, for profiling)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy a 512 MiB buffer three times (the known-size kernel "
+                         "that FETCH_SIZE / WRITE_SIZE counters are scaled by)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
-              "(the multi-GPU path is a design, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported: "
+              "this build evaluates one GPU's share; sharding is a design only",
+              file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        import torch
+        a = torch.ones(128 << 20, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": a.numel() * 4, "copies": 3}), flush=True)
+        return
 
     from flare_amd import _lib
     from flare_amd.workloads import fcc_alloy_32k
@@ -307,26 +342,28 @@ def main():
     n_by = [int((species == t).sum()) for t in range(2)]
     kflops = sum(2.0 * a * b * d for a, b in zip(n_by, M_by))  # S contraction, all species
 
-    def roof(r, peak):
+    traffic = measured_traffic()
+
+    def roof(r, peak, form_name):
         ms, nl = r["stage"]["kernel_contraction"]
         per_launch_ms = ms / max(nl, 1)
         per_launch_flops = kflops / len(M_by)
         achieved = per_launch_flops / (per_launch_ms * 1e-3) / 1e12
         return {"bound": "mfma", "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s",
-                "frac": round(achieved / peak, 4), "traffic": None,
+                "frac": round(achieved / peak, 4), "traffic": traffic.get(form_name),
                 "kernel": "kernel contraction S = B X_hat^T + epilogue",
                 "avg_launch_ms": round(per_launch_ms, 4), "flops_per_launch": per_launch_flops}
 
-    def summary(r, peak):
+    def summary(r, peak, form_name):
         return {"atom_evals_per_s": n * args.steps / r["total"],
                 "ms_per_step": r["total"] / args.steps * 1e3,
                 "p50_ms": float(np.percentile(r["lat"], 50) * 1e3),
                 "p99_ms": float(np.percentile(r["lat"], 99) * 1e3),
                 "stage_ms": {k: round(v[0] / args.steps, 4) for k, v in r["stage"].items()},
-                "roofline": roof(r, peak)}
+                "roofline": roof(r, peak, form_name)}
 
-    chosen = summary(ch, PEAK["bf16"])
-    fp64 = summary(f64, PEAK["fp64"])
+    chosen = summary(ch, PEAK["bf16"], "chosen")
+    fp64 = summary(f64, PEAK["fp64"], "fp64")
     out = {
         "metric": METRIC,
         "value": chosen["atom_evals_per_s"],

@@ -216,13 +216,13 @@ struct SplitOut {
 template <typename T>
 __global__ __launch_bounds__(64 * kAtomsPerBlock) void k_b2(B2Args a, T *B, double *norm,
                                                               double *nn, SplitOut Bs, int ldp,
-                                                              T *Cg, int want_b2) {
+                                                              T *Cg, int want_b2, int work) {
   extern __shared__ __align__(16) unsigned char smem[];
   int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   int i = blockIdx.x * (blockDim.x >> 6) + w;
   if (i >= a.n) return;
   int nc = a.nrad * a.H, H = a.H, nr = a.nrad;
-  T *c = reinterpret_cast<T *>(smem) + (size_t)w * b2_work(nc, H, a.N, a.nd);
+  T *c = reinterpret_cast<T *>(smem) + (size_t)w * work;  // work: elements per atom
   T *tab = c + nc;
   accumulate_c<T>(a, i, lane, c, tab);
   for (int k = lane; k < nc; k += 64) Cg[(size_t)i * nc + k] = c[k];
@@ -314,23 +314,29 @@ __global__ __launch_bounds__(64 * kAtomsPerBlock) void k_dEdc(B2Args a, const T 
   int i = blockIdx.x * (blockDim.x >> 6) + w;
   if (i >= a.n) return;
   int nc = a.nrad * a.H;
-  T *c = reinterpret_cast<T *>(smem) + (size_t)w * ((size_t)nc + a.nd);
-  T *wv = c + nc;
+  T *c = reinterpret_cast<T *>(smem) + (size_t)w * ((size_t)nc + 2 * (size_t)a.nd);
+  T *wg = c + nc, *wb = wg + a.nd;  // G and B rows
   int row = a.row_of_atom[i];
   double nrm = norm[row];
   if (nrm < aa.empty_norm || nnv[row] < aa.empty_nn) return;
+  // One pass over HBM: c, G and B rows to LDS, G . B on the way (loads
+  // unrolled so several are in flight per lane).
   for (int k = lane; k < nc; k += 64) c[k] = Cg[(size_t)i * nc + k];
   const T *Brow = B + (size_t)row * a.nd;
   const T *Grow = G + (size_t)row * g_ld;
-  if (aa.project) {
-    T ninv = T(1) / (T)nrm;
-    T gb = 0;
-    for (int k = lane; k < a.nd; k += 64) gb += Grow[k] * Brow[k];
-    gb = wave_sum(gb) * ninv;
-    for (int k = lane; k < a.nd; k += 64) wv[k] = (Grow[k] - gb * Brow[k] * ninv) * ninv;
-  } else {
-    for (int k = lane; k < a.nd; k += 64) wv[k] = Grow[k];
+  T gb = 0;
+#pragma unroll 4
+  for (int k = lane; k < a.nd; k += 64) {
+    T gv = Grow[k], bv = Brow[k];
+    wg[k] = gv;
+    wb[k] = bv;
+    gb += gv * bv;
   }
+  // w = dE/dB = (G - (G . B_hat) B_hat) / |B| = sg G - sb B, formed when the
+  // MFMA operand is read.
+  T ninv = T(1) / (T)nrm;
+  T sg = aa.project ? ninv : T(1);
+  T sb = aa.project ? wave_sum(gb) * ninv * ninv * ninv : T(0);
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -353,8 +359,8 @@ __global__ __launch_bounds__(64 * kAtomsPerBlock) void k_dEdc(B2Args a, const T 
           T av = 0, bv = 0;
           if (n1 < nr && n2 < nr) {
             int lo = n1 < n2 ? n1 : n2, hi = n1 < n2 ? n2 : n1;
-            int p = lo * nr - (lo * (lo - 1)) / 2 + (hi - lo);
-            av = (n1 == n2 ? T(2) : T(1)) * wv[p * L1 + l];
+            int p = (lo * nr - (lo * (lo - 1)) / 2 + (hi - lo)) * L1 + l;
+            av = (n1 == n2 ? T(2) : T(1)) * (sg * wg[p] - sb * wb[p]);
           }
           if (n2 < nr && m < nm) bv = c[n2 * a.H + lm0 + m];
           acc = mfma_4(av, bv, acc);
@@ -538,7 +544,8 @@ void launch_b2(fm_structure s, bool want_b2) {
   upload_rcp(st);
   B2Args a = make_args(s);
   size_t esz = s->form == FM_FORM_FP64 ? sizeof(double) : sizeof(float);
-  size_t per = b2_work(a.nrad * a.H, a.H, a.N, want_b2 ? a.nd : 0) * esz;
+  size_t work = b2_work(a.nrad * a.H, a.H, a.N, want_b2 ? a.nd : 0);
+  size_t per = work * esz;
   int apb = atoms_per_block(per);
   size_t lds = apb * per;
   int nblk = (s->n + apb - 1) / apb;
@@ -549,11 +556,11 @@ void launch_b2(fm_structure s, bool want_b2) {
   if (s->form == FM_FORM_FP64)
     hipLaunchKernelGGL(k_b2<double>, dim3(nblk), dim3(64 * apb), lds, st, a, s->B.as<double>(),
                        s->norm.as<double>(), s->nn.as<double>(), none, 0, s->C.as<double>(),
-                       (int)want_b2);
+                       (int)want_b2, (int)work);
   else
     hipLaunchKernelGGL(k_b2<float>, dim3(nblk), dim3(64 * apb), lds, st, a, s->B.as<float>(),
                        s->norm.as<double>(), s->nn.as<double>(), pieces,
-                       (int)split_ld(s->nd), s->C.as<float>(), (int)want_b2);
+                       (int)split_ld(s->nd), s->C.as<float>(), (int)want_b2, (int)work);
   FM_HIP(hipGetLastError());
 }
 
@@ -570,8 +577,8 @@ void launch_adjoint(fm_structure s, const void *G, int64_t g_ld, void *fbond,
   size_t nc = (size_t)a.nrad * a.H;
   DevBuf U(&ctx->pool, std::max<size_t>(1, (size_t)s->n * nc * esz));
   AdjArgs aa{project, empty_norm, empty_nn};
-  // dE/dc (B2): c and w in LDS per atom.
-  size_t per1 = (nc + (s->desc_kind == 2 ? a.nd : 0)) * esz;
+  // dE/dc (B2): c, G and B rows in LDS per atom.
+  size_t per1 = (nc + (s->desc_kind == 2 ? 2 * (size_t)a.nd : 0)) * esz;
   int apb1 = atoms_per_block(per1);
   int nblk1 = (s->n + apb1 - 1) / apb1;
   // Bond forces: u in LDS per atom.

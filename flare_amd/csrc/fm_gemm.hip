@@ -48,18 +48,24 @@ struct TileOut {
 
 // Output tile of this workgroup. Workgroups are dealt to the 8 XCDs round
 // robin (id % 8); the dispatch id is remapped so each XCD gets a contiguous
-// run of logical tiles, column index fastest: the workgroups sharing a row
-// tile of A then share that XCD's L2.
-__device__ __forceinline__ void xcd_tile(int &tx, int &ty) {
+// run of logical tiles. With column index fastest (the default) the
+// workgroups sharing a row tile of A share that XCD's L2; with row index
+// fastest those sharing a column tile of B do.
+__device__ __forceinline__ void xcd_tile(int &tx, int &ty, bool rows_fastest = false) {
   constexpr int kXcd = 8;
-  int ntx = gridDim.x, nblk = gridDim.x * gridDim.y;
+  int ntx = gridDim.x, nty = gridDim.y, nblk = ntx * nty;
   int pid = blockIdx.y * ntx + blockIdx.x;
   int per = (nblk + kXcd - 1) / kXcd;
   int tall = nblk % kXcd == 0 ? kXcd : nblk % kXcd;  // XCDs that get `per` ids
   int xcd = pid % kXcd, local = pid / kXcd;
   int id = xcd < tall ? xcd * per + local : tall * per + (xcd - tall) * (per - 1) + local;
-  tx = id % ntx;
-  ty = id / ntx;
+  if (rows_fastest) {
+    ty = id % nty;
+    tx = id / nty;
+  } else {
+    tx = id % ntx;
+    ty = id / ntx;
+  }
 }
 
 template <typename Acc>
@@ -72,10 +78,13 @@ __device__ __forceinline__ double fpow(double x, double p, int pi) {
   if (pi == 1) return x;
   return pow(x, p);
 }
+// Kept out of line: inlined into every element of the unrolled FP32
+// epilogue, powf makes the loop too large to unroll.
+__device__ __noinline__ float powf_general(float x, float p) { return powf(x, p); }
 __device__ __forceinline__ float fpow(float x, float p, int pi) {
   if (pi == 2) return x * x;
   if (pi == 1) return x;
-  return powf(x, p);
+  return powf_general(x, p);
 }
 
 template <typename Acc, typename TW, int TM, int TN>
@@ -115,7 +124,7 @@ __device__ __forceinline__ void ep_kernel(const EpArgs &e, const TileOut<Acc, TM
         } else {
           e.W_as<TW>()[(size_t)r * e.ldw + c] = (TW)wv;
         }
-        if (e.K) e.K[(size_t)r * e.ldk + c] = (double)kv;
+        if (e.K) e.K_as<Acc>()[(size_t)r * e.ldk + c] = kv;
         rs[ti][i] += (double)(kv * a);
       }
     }
@@ -236,10 +245,32 @@ constexpr int F64_TM = 2, F64_TN = 2, F64_BM = 32 * F64_TM, F64_BN = 32 * F64_TN
 constexpr int BK64 = 16;
 constexpr int PAD64 = 4;
 
-template <int EP, int TM, int TN>
+// Four consecutive k of one row, widened to FP64: one or two 16-byte loads
+// when `vec` (the row start is 16-byte aligned and k + 4 <= kend), else
+// element by element with zeros past the row count or kend.
+template <typename T>
+__device__ __forceinline__ void load_row4(const T *P, long long ld, int r, int rows, int k,
+                                          int kend, bool vec, double (&o)[4]) {
+  const T *p = P + (size_t)(r < rows ? r : 0) * ld + k;
+  if (r < rows && vec && k + 4 <= kend) {
+    if constexpr (sizeof(T) == 4) {
+      float4 v = *reinterpret_cast<const float4 *>(p);
+      o[0] = v.x, o[1] = v.y, o[2] = v.z, o[3] = v.w;
+    } else {
+      double2 a = *reinterpret_cast<const double2 *>(p);
+      double2 b = *reinterpret_cast<const double2 *>(p + 2);
+      o[0] = a.x, o[1] = a.y, o[2] = b.x, o[3] = b.y;
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < 4; q++) o[q] = (r < rows && k + q < kend) ? (double)p[q] : 0.0;
+  }
+}
+
+template <int EP, int TM, int TN, typename TB = double>
 __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K, const double *A, long long lda,
-                                                  const double *B, long long ldb, int lower_tri,
-                                                  EpArgs e) {
+                                                  const TB *B, long long ldb, int lower_tri,
+                                                  int vec, EpArgs e) {
   constexpr int BM = 32 * TM, BN = 32 * TN, RA = BM / 64, RB = BN / 64;
   static_assert(BM % 64 == 0 && BN % 64 == 0, "tiles are multiples of 64");
   __shared__ double As[BK64][BM + PAD64];
@@ -247,8 +278,15 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K, const dou
   __shared__ double red[2 * (BM > BN ? BM : BN)];
   int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   int wr = wid >> 1, wc = wid & 1;
+  // A lower-triangular A makes row tiles uneven (row tile r reads r + 1
+  // k-tiles): keep the hardware's round-robin spread of tiles there.
   int tx, ty;
-  xcd_tile(tx, ty);
+  if (lower_tri) {
+    tx = blockIdx.x;
+    ty = blockIdx.y;
+  } else {
+    xcd_tile(tx, ty);
+  }
   int row0 = ty * BM, col0 = tx * BN;
   int kend = K;
   if (lower_tri) kend = min(K, row0 + BM);
@@ -261,19 +299,11 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K, const dou
   double ra[RA][4], rb[RB][4];
   auto gload = [&](int k0) {
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      int k = k0 + lk + q;
+    for (int j = 0; j < RA; j++)
+      load_row4(A, lda, row0 + lr + 64 * j, M, k0 + lk, kend, vec & 1, ra[j]);
 #pragma unroll
-      for (int j = 0; j < RA; j++) {
-        int r = row0 + lr + 64 * j;
-        ra[j][q] = (r < M && k < kend) ? A[(size_t)r * lda + k] : 0.0;
-      }
-#pragma unroll
-      for (int j = 0; j < RB; j++) {
-        int c = col0 + lr + 64 * j;
-        rb[j][q] = (c < N && k < kend) ? B[(size_t)c * ldb + k] : 0.0;
-      }
-    }
+    for (int j = 0; j < RB; j++)
+      load_row4(B, ldb, col0 + lr + 64 * j, N, k0 + lk, kend, vec & 2, rb[j]);
   };
   gload(0);
   for (int k0 = 0; k0 < kend; k0 += BK64) {
@@ -517,17 +547,22 @@ int tuned(int family, int ep, int M, int N, int K, hipStream_t st, const TileSha
   return best;
 }
 
-template <int EP, int TM, int TN>
+template <int EP, int TM, int TN, typename TB>
 void f64_launch(hipStream_t st, int M, int N, int K, const double *A, long long lda,
-                const double *B, long long ldb, int lower_tri, const EpArgs &e) {
+                const TB *B, long long ldb, int lower_tri, const EpArgs &e) {
   dim3 grid((N + 32 * TN - 1) / (32 * TN), (M + 32 * TM - 1) / (32 * TM));
-  hipLaunchKernelGGL((k_gemm_f64<EP, TM, TN>), grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb,
-                     lower_tri, e);
+  // 16-byte loads where every row start is 16-byte aligned.
+  auto aligned = [](const void *p, long long ld, size_t esz) {
+    return (uintptr_t)p % 16 == 0 && (ld * (long long)esz) % 16 == 0;
+  };
+  int vec = (aligned(A, lda, sizeof(double)) ? 1 : 0) | (aligned(B, ldb, sizeof(TB)) ? 2 : 0);
+  hipLaunchKernelGGL((k_gemm_f64<EP, TM, TN, TB>), grid, dim3(256), 0, st, M, N, K, A, lda, B,
+                     ldb, lower_tri, vec, e);
 }
 
-template <int EP>
+template <int EP, typename TB>
 void f64_shape(int i, hipStream_t st, int M, int N, int K, const double *A, long long lda,
-               const double *B, long long ldb, int lower_tri, const EpArgs &e) {
+               const TB *B, long long ldb, int lower_tri, const EpArgs &e) {
   switch (i) {
     case 0: f64_launch<EP, 2, 2>(st, M, N, K, A, lda, B, ldb, lower_tri, e); break;
     case 1: f64_launch<EP, 4, 2>(st, M, N, K, A, lda, B, ldb, lower_tri, e); break;
@@ -566,6 +601,17 @@ void gemm_f64(hipStream_t st, int ep, int M, int N, int K, const double *A, long
     FM_HIP(hipGetLastError());
   };
   launch(tuned(0, ep, M, N, K, st, kF64Shapes, 4, launch));
+}
+
+void gemm_f64(hipStream_t st, int ep, int M, int N, int K, const double *A, long long lda,
+              const float *B, long long ldb, int lower_tri, const EpArgs &e) {
+  if (M <= 0 || N <= 0) return;
+  FM_CHECK(ep == EP_SUMSQ, FM_ERR_INVALID, "FP32 B operand: variance contraction only");
+  auto launch = [&](int i) {
+    f64_shape<EP_SUMSQ>(i, st, M, N, K, A, lda, B, ldb, lower_tri, e);
+    FM_HIP(hipGetLastError());
+  };
+  launch(tuned(2, ep, M, N, K, st, kF64Shapes, 4, launch));
 }
 
 void gemm_split(hipStream_t st, int ep, int np, int M, int N, int Kpad, const SplitOperand &A,

@@ -487,27 +487,44 @@ void build_neighbors(fm_structure s) {
 // its transpose by neighbour atom (for the deterministic force gather), and
 // the species-sorted row of each atom (b2.cpp:148-176).
 // ---------------------------------------------------------------------------
-__global__ void k_bond_count(int n, const int *species, const int *cum, const int *nspec,
-                             const double *rel, const double *cutoffs, int nos, int *bcount) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+// Bonds are the neighbours within the species-pair cutoff, kept in neighbour
+// order. One wave per atom walks its neighbour list 64 entries at a time; a
+// ballot gives each kept entry its place.
+__global__ __launch_bounds__(256) void k_bond_count(int n, const int *species, const int *cum,
+                                                    const int *nspec, const double *rel,
+                                                    const double *cutoffs, int nos, int *bcount) {
+  int lane = threadIdx.x & 63;
+  int i = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (i >= n) return;
   int si = species[i], c = 0;
-  for (int q = cum[i]; q < cum[i + 1]; q++)
-    if (rel[4 * (size_t)q] <= cutoffs[si * nos + nspec[q]]) c++;
-  bcount[i] = c;
+  for (int q0 = cum[i]; q0 < cum[i + 1]; q0 += 64) {
+    int q = q0 + lane;
+    bool keep = q < cum[i + 1] && rel[4 * (size_t)q] <= cutoffs[si * nos + nspec[q]];
+    c += __popcll(__ballot(keep));
+  }
+  if (lane == 0) bcount[i] = c;
 }
 
-__global__ void k_bond_fill(int n, const int *species, const int *cum, const int *nspec,
-                            const double *rel, const double *cutoffs, int nos, const int *boff,
-                            const int *sidx, int *bond_q, int *in_count) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_bond_fill(int n, const int *species, const int *cum,
+                                                   const int *nspec, const double *rel,
+                                                   const double *cutoffs, int nos,
+                                                   const int *boff, const int *sidx, int *bond_q,
+                                                   int *in_count) {
+  int lane = threadIdx.x & 63;
+  int i = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (i >= n) return;
   int si = species[i], o = boff[i];
-  for (int q = cum[i]; q < cum[i + 1]; q++)
-    if (rel[4 * (size_t)q] <= cutoffs[si * nos + nspec[q]]) {
-      bond_q[o++] = q;
+  unsigned long long below = (1ull << lane) - 1ull;
+  for (int q0 = cum[i]; q0 < cum[i + 1]; q0 += 64) {
+    int q = q0 + lane;
+    bool keep = q < cum[i + 1] && rel[4 * (size_t)q] <= cutoffs[si * nos + nspec[q]];
+    unsigned long long m = __ballot(keep);
+    if (keep) {
+      bond_q[o + __popcll(m & below)] = q;
       atomicAdd(&in_count[sidx[q]], 1);
     }
+    o += __popcll(m);
+  }
 }
 
 __global__ void k_in_fill(int nbonds, const int *bond_q, const int *sidx, const int *in_off,
@@ -519,17 +536,21 @@ __global__ void k_in_fill(int nbonds, const int *bond_q, const int *sidx, const 
   in_bond[in_off[j] + slot] = b;
 }
 
-__global__ void k_in_sort(int n, const int *in_off, int *in_bond) {
-  int j = blockIdx.x * blockDim.x + threadIdx.x;
+// Incoming bonds of atom j in ascending bond order (so the force gather sums
+// them in a fixed order): one wave per atom; each lane takes elements of the
+// unordered list and writes each to its rank, the number of smaller bond ids
+// in the list (ids are distinct). The list is read with wave-uniform loads.
+__global__ __launch_bounds__(256) void k_in_rank(int n, const int *in_off, const int *unordered,
+                                                 int *in_bond) {
+  int lane = threadIdx.x & 63;
+  int j = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (j >= n) return;
-  int a0 = in_off[j], a1 = in_off[j + 1];
-  for (int a = a0 + 1; a < a1; a++) {
-    int v = in_bond[a], c = a - 1;
-    while (c >= a0 && in_bond[c] > v) {
-      in_bond[c + 1] = in_bond[c];
-      c--;
-    }
-    in_bond[c + 1] = v;
+  int a0 = in_off[j], cnt = in_off[j + 1] - a0;
+  const int *u = unordered + a0;
+  for (int e = lane; e < cnt; e += 64) {
+    int v = u[e], r = 0;
+    for (int k = 0; k < cnt; k++) r += u[k] < v;
+    in_bond[a0 + r] = v;
   }
 }
 
@@ -558,7 +579,7 @@ void build_bonds(fm_structure s) {
                         s->cutoffs.size() * sizeof(double), hipMemcpyHostToDevice, st));
   DevBuf bcount(pool, (size_t)n * sizeof(int));
   s->bond_off.alloc(pool, ((size_t)n + 1) * sizeof(int));
-  hipLaunchKernelGGL(k_bond_count, dim3(nblk), dim3(tpb), 0, st, n, s->species.as<int>(),
+  hipLaunchKernelGGL(k_bond_count, dim3((n + 3) / 4), dim3(256), 0, st, n, s->species.as<int>(),
                      s->cumulative.as<int>(), s->nspecies.as<int>(), s->rel.as<double>(),
                      s->d_cutoffs.as<double>(), s->nos, bcount.as<int>());
   int nbonds = scan_exclusive(ctx, bcount.as<int>(), s->bond_off.as<int>(), n, true);
@@ -566,7 +587,7 @@ void build_bonds(fm_structure s) {
   s->bond_q.alloc(pool, (size_t)nbonds * sizeof(int));
   DevBuf in_count(pool, (size_t)n * sizeof(int));
   FM_HIP(hipMemsetAsync(in_count.as<int>(), 0, (size_t)n * sizeof(int), st));
-  hipLaunchKernelGGL(k_bond_fill, dim3(nblk), dim3(tpb), 0, st, n, s->species.as<int>(),
+  hipLaunchKernelGGL(k_bond_fill, dim3((n + 3) / 4), dim3(256), 0, st, n, s->species.as<int>(),
                      s->cumulative.as<int>(), s->nspecies.as<int>(), s->rel.as<double>(),
                      s->d_cutoffs.as<double>(), s->nos, s->bond_off.as<int>(), s->sidx.as<int>(),
                      s->bond_q.as<int>(), in_count.as<int>());
@@ -574,13 +595,15 @@ void build_bonds(fm_structure s) {
   scan_exclusive(ctx, in_count.as<int>(), s->in_off.as<int>(), n, false);
   s->in_bond.alloc(pool, (size_t)nbonds * sizeof(int));
   DevBuf cursor(pool, (size_t)n * sizeof(int));
+  DevBuf unordered(pool, (size_t)nbonds * sizeof(int));
   FM_HIP(hipMemsetAsync(cursor.as<int>(), 0, (size_t)n * sizeof(int), st));
-  if (nbonds > 0)
+  if (nbonds > 0) {
     hipLaunchKernelGGL(k_in_fill, dim3((nbonds + tpb - 1) / tpb), dim3(tpb), 0, st, nbonds,
                        s->bond_q.as<int>(), s->sidx.as<int>(), s->in_off.as<int>(),
-                       cursor.as<int>(), s->in_bond.as<int>());
-  hipLaunchKernelGGL(k_in_sort, dim3(nblk), dim3(tpb), 0, st, n, s->in_off.as<int>(),
-                     s->in_bond.as<int>());
+                       cursor.as<int>(), unordered.as<int>());
+    hipLaunchKernelGGL(k_in_rank, dim3((n + 3) / 4), dim3(256), 0, st, n, s->in_off.as<int>(),
+                       unordered.as<int>(), s->in_bond.as<int>());
+  }
   // Species-sorted rows: one scan of species flags per species, the
   // per-species totals fetched with a single copy.
   s->row_of_atom.alloc(pool, (size_t)n * sizeof(int));

@@ -123,10 +123,14 @@ void predict(fm_model m, fm_structure s, int flags, int form, double *mean_efs,
   // Per-species slices of the atom-major w and k matrices (n_s x M_s each)
   // and of the energy / variance partials.
   // The chosen form keeps w as BF16 pairs with padded rows (ws = split_ld).
-  std::vector<size_t> wo(nos + 1, 0), eo(nos + 1, 0), qo(nos + 1, 0), so(nos + 1, 0);
+  // k is FP32 in the chosen form, with rows padded to 4 values (16 bytes).
+  std::vector<size_t> wo(nos + 1, 0), eo(nos + 1, 0), qo(nos + 1, 0), so(nos + 1, 0),
+      ko(nos + 1, 0), kld(nos, 0);
   for (int t = 0; t < nos; t++) {
     size_t ns = s->n_by_type[t], Ms = m->M_by_type[t];
     wo[t + 1] = wo[t] + ns * Ms;
+    kld[t] = f64 ? Ms : (Ms + 3) / 4 * 4;
+    ko[t + 1] = ko[t] + ns * kld[t];
     so[t + 1] = so[t] + ns * (size_t)split_ld((long long)Ms);
     eo[t + 1] = eo[t] + ns * (size_t)std::max(1, gemm_col_tiles(form, (int)Ms));
     int Lr = want_var ? m->L_rows[t] : 0;
@@ -140,7 +144,9 @@ void predict(fm_model m, fm_structure s, int flags, int form, double *mean_efs,
     Wl.alloc(pool, std::max<size_t>(1, so[nos] * sizeof(uint16_t)));
   }
   DevBuf Kb;
-  if (want_var) Kb.alloc(pool, std::max<size_t>(1, wo[nos] * sizeof(double)));
+  // k in FP64, or in FP32 for the split form (whose k is an FP32 value).
+  size_t ksz = f64 ? sizeof(double) : sizeof(float);
+  if (want_var) Kb.alloc(pool, std::max<size_t>(1, ko[nos] * ksz));
   DevBuf G(pool, std::max<size_t>(1, (size_t)n * nd * esz));
   DevBuf Epart(pool, std::max<size_t>(1, eo[nos] * sizeof(double)));
   DevBuf Qpart(pool, std::max<size_t>(1, qo[nos] * sizeof(double)));
@@ -173,8 +179,8 @@ void predict(fm_model m, fm_structure s, int flags, int form, double *mean_efs,
       ek.ldw = split_ld(Ms);
     }
     if (want_var) {
-      ek.K = Kb.as<double>() + wo[t];
-      ek.ldk = Ms;
+      ek.K = Kb.as<char>() + ko[t] * ksz;
+      ek.ldk = (long long)kld[t];
     }
     ek.part = Epart.as<double>() + eo[t];
     ek.part_ld = ns;
@@ -235,8 +241,12 @@ void predict(fm_model m, fm_structure s, int flags, int form, double *mean_efs,
       EpArgs eq;
       eq.part = Qpart.as<double>() + qo[t];
       eq.part_ld = ns;
-      gemm_f64(st, EP_SUMSQ, m->L_rows[t], ns, Ms, m->L[t].as<double>(), Ms,
-               Kb.as<double>() + wo[t], Ms, m->L_lower[t], eq);
+      if (f64)
+        gemm_f64(st, EP_SUMSQ, m->L_rows[t], ns, Ms, m->L[t].as<double>(), Ms,
+                 Kb.as<double>() + ko[t], (long long)kld[t], m->L_lower[t], eq);
+      else
+        gemm_f64(st, EP_SUMSQ, m->L_rows[t], ns, Ms, m->L[t].as<double>(), Ms,
+                 Kb.as<float>() + ko[t], (long long)kld[t], m->L_lower[t], eq);
       ctx->launches[ST_VAR]++;
     }
   }

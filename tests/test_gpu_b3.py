@@ -102,5 +102,5 @@ def test_b3_limits(C, system):
         C.Structure(system["cell"], system["species"], system["pos"], 4.6,
                     [C.B3("chebyshev", "quadratic", [0.0, 4.6], [], [NOS, 2, 5], CUT)])
     s = C.Structure(system["cell"], system["species"], system["pos"], 4.6, [system["calc"]])
-    with pytest.raises(RuntimeError, match="B2"):
+    with pytest.raises(ValueError, match="B2"):
         s.descriptors[0].descriptor_force_dervs
