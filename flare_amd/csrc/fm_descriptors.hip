@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 #include "fm_device.hpp"
 #include "fm_gemm.hpp"
@@ -303,6 +304,53 @@ __device__ __forceinline__ int pair_index(int lo, int hi, int nr) {
   return lo * nr - (lo * (lo - 1)) / 2 + (hi - lo);
 }
 
+__host__ __device__ inline int pad4(int n) { return (n + 3) & ~3; }
+
+// n elements from global memory to LDS: 16-byte vectors when the source is
+// 16-byte aligned (the LDS destination always is), four in flight per lane.
+template <typename T>
+__device__ __forceinline__ void stage_row(const T *src, T *dst, int n, int lane) {
+  constexpr int E = 16 / sizeof(T);
+  int done = 0;
+  if ((reinterpret_cast<uintptr_t>(src) & 15) == 0) {
+    using V = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
+    const V *sv = reinterpret_cast<const V *>(src);
+    V *dv = reinterpret_cast<V *>(dst);
+    int nv = n / E;
+#pragma unroll 4
+    for (int v = lane; v < nv; v += 64) dv[v] = sv[v];
+    done = nv * E;
+  }
+  for (int k = done + lane; k < n; k += 64) dst[k] = src[k];
+}
+
+// Two rows of n elements each (the G and B rows of an atom) to LDS, their
+// loads interleaved so eight 16-byte vectors are in flight per lane.
+template <typename T>
+__device__ __forceinline__ void stage_rows2(const T *s0, T *d0, const T *s1, T *d1, int n,
+                                            int lane) {
+  constexpr int E = 16 / sizeof(T);
+  if (((reinterpret_cast<uintptr_t>(s0) | reinterpret_cast<uintptr_t>(s1)) & 15) != 0) {
+    stage_row(s0, d0, n, lane);
+    stage_row(s1, d1, n, lane);
+    return;
+  }
+  using V = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
+  const V *a = reinterpret_cast<const V *>(s0), *b = reinterpret_cast<const V *>(s1);
+  V *da = reinterpret_cast<V *>(d0), *db = reinterpret_cast<V *>(d1);
+  int nv = n / E;
+#pragma unroll 4
+  for (int v = lane; v < nv; v += 64) {
+    V x = a[v], y = b[v];
+    da[v] = x;
+    db[v] = y;
+  }
+  for (int k = nv * E + lane; k < n; k += 64) {
+    d0[k] = s0[k];
+    d1[k] = s1[k];
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(64 * kAtomsPerBlock) void k_dEdc(B2Args a, const T *B,
                                                                 const double *norm,
@@ -313,33 +361,27 @@ __global__ __launch_bounds__(64 * kAtomsPerBlock) void k_dEdc(B2Args a, const T 
   int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   int i = blockIdx.x * (blockDim.x >> 6) + w;
   if (i >= a.n) return;
-  int nc = a.nrad * a.H;
-  T *c = reinterpret_cast<T *>(smem) + (size_t)w * ((size_t)nc + 2 * (size_t)a.nd);
-  T *wg = c + nc, *wb = wg + a.nd;  // G and B rows
+  int nc = a.nrad * a.H, ncp = pad4(nc), ndp = pad4(a.nd);
+  T *c = reinterpret_cast<T *>(smem) + (size_t)w * ((size_t)ncp + 2 * (size_t)ndp);
+  T *wg = c + ncp, *wb = wg + ndp;  // G and B rows
   int row = a.row_of_atom[i];
   double nrm = norm[row];
   if (nrm < aa.empty_norm || nnv[row] < aa.empty_nn) return;
-  // One pass over HBM: c, G and B rows to LDS, G . B on the way (loads
-  // unrolled so several are in flight per lane).
-  for (int k = lane; k < nc; k += 64) c[k] = Cg[(size_t)i * nc + k];
-  const T *Brow = B + (size_t)row * a.nd;
-  const T *Grow = G + (size_t)row * g_ld;
-  T gb = 0;
-#pragma unroll 4
-  for (int k = lane; k < a.nd; k += 64) {
-    T gv = Grow[k], bv = Brow[k];
-    wg[k] = gv;
-    wb[k] = bv;
-    gb += gv * bv;
-  }
-  // w = dE/dB = (G - (G . B_hat) B_hat) / |B| = sg G - sb B, formed when the
-  // MFMA operand is read.
-  T ninv = T(1) / (T)nrm;
-  T sg = aa.project ? ninv : T(1);
-  T sb = aa.project ? wave_sum(gb) * ninv * ninv * ninv : T(0);
+  // One pass over HBM with 16-byte loads: c, G and B rows to LDS.
+  stage_row(Cg + (size_t)i * nc, c, nc, lane);
+  stage_rows2(G + (size_t)row * g_ld, wg, B + (size_t)row * a.nd, wb, a.nd, lane);
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  // w = dE/dB = (G - (G . B_hat) B_hat) / |B| = sg G - sb B, formed when the
+  // MFMA operand is read.
+  T ninv = T(1) / (T)nrm;
+  T sg = aa.project ? ninv : T(1), sb = 0;
+  if (aa.project) {
+    T gb = 0;
+    for (int k = lane; k < a.nd; k += 64) gb += wg[k] * wb[k];
+    sb = wave_sum(gb) * ninv * ninv * ninv;
+  }
   // u_l = W_l C_l per l on the matrix cores: W_l[n1][n2] = (1 + [n1 = n2])
   // w[p(min, max), l] (nr x nr, symmetric), C_l[n2][m] = c[n2, l^2 + m]
   // (nr x (2l + 1)); 16 x 16 output tiles, 4 n2 per MFMA. Lane L supplies
@@ -375,6 +417,16 @@ __global__ __launch_bounds__(64 * kAtomsPerBlock) void k_dEdc(B2Args a, const T 
   }
 }
 
+__device__ __forceinline__ void load4_lds(const float *p, float (&o)[4]) {
+  float4 v = *reinterpret_cast<const float4 *>(p);
+  o[0] = v.x, o[1] = v.y, o[2] = v.z, o[3] = v.w;
+}
+__device__ __forceinline__ void load4_lds(const double *p, double (&o)[4]) {
+  double2 v0 = *reinterpret_cast<const double2 *>(p);
+  double2 v1 = *reinterpret_cast<const double2 *>(p + 2);
+  o[0] = v0.x, o[1] = v0.y, o[2] = v1.x, o[3] = v1.y;
+}
+
 // Adjoint, second half: per bond f = dE_i/dr_ij = sum_{n,lm} u[s_j n, lm]
 // grad(g_n Y_lm) (pair_flare.cpp:141-150, the adjoint of b2.cpp:219-228).
 // One wavefront per atom, lanes over its bonds; each lane sweeps every
@@ -394,8 +446,10 @@ __global__ __launch_bounds__(64 * kAtomsPerBlock) void k_bond_forces(B2Args a,
   int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   int i = blockIdx.x * (blockDim.x >> 6) + w;
   if (i >= a.n) return;
-  int nc = a.nrad * a.H;
-  T *u = reinterpret_cast<T *>(smem) + (size_t)w * nc;
+  int nc = a.nrad * a.H, Np = pad4(a.N);
+  // u in LDS as [species][lm][n] with n padded to 4, so a lane reads the
+  // radial coefficients of one (species, lm) with 16-byte loads.
+  T *u = reinterpret_cast<T *>(smem) + (size_t)w * a.nos * a.H * Np;
   int row = a.row_of_atom[i];
   int b0 = a.bond_off[i], b1 = a.bond_off[i + 1];
   double nrm = norm[row];
@@ -410,7 +464,10 @@ __global__ __launch_bounds__(64 * kAtomsPerBlock) void k_bond_forces(B2Args a,
     if (lane < 6) vir[6 * (size_t)i + lane] = 0;
     return;
   }
-  for (int k = lane; k < nc; k += 64) u[k] = Ug[(size_t)i * nc + k];
+  for (int k = lane; k < nc; k += 64) {
+    int n1 = k / a.H, lm = k - n1 * a.H, sj = n1 / a.N, n = n1 - sj * a.N;
+    u[((size_t)sj * a.H + lm) * Np + n] = Ug[(size_t)i * nc + k];
+  }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -429,16 +486,22 @@ __global__ __launch_bounds__(64 * kAtomsPerBlock) void k_bond_forces(B2Args a,
       radial_basis<T>(r, rc, (T)a.r1, N, a.cutkind, g, dg);
       T rinv = T(1) / r;
       T ux = x * rinv, uy = y * rinv, uz = z * rinv;
-      const T *ub = u + (size_t)sj * N * H;
+      const T *ub = u + (size_t)sj * H * Np;
       T acc_a = 0, Px = 0, Py = 0, Pz = 0;
       ylm_sweep<T, true>(L, ux, uy, uz, [&](int lm, T Y, T px, T py, T pz) {
         T s0 = 0, s1 = 0;
+        const T *ul = ub + (size_t)lm * Np;
 #pragma unroll
-        for (int nn = 0; nn < kMaxN; nn++) {
-          if (nn < N) {
-            T uu = ub[nn * H + lm];
-            s0 += uu * g[nn];
-            s1 += uu * dg[nn];
+        for (int n4 = 0; n4 < kMaxN; n4 += 4) {
+          if (n4 < N) {
+            T uu[4];
+            load4_lds(ul + n4, uu);
+#pragma unroll
+            for (int e = 0; e < 4; e++)
+              if (n4 + e < N) {
+                s0 += uu[e] * g[n4 + e];
+                s1 += uu[e] * dg[n4 + e];
+              }
           }
         }
         acc_a += s1 * Y;
@@ -577,12 +640,12 @@ void launch_adjoint(fm_structure s, const void *G, int64_t g_ld, void *fbond,
   size_t nc = (size_t)a.nrad * a.H;
   DevBuf U(&ctx->pool, std::max<size_t>(1, (size_t)s->n * nc * esz));
   AdjArgs aa{project, empty_norm, empty_nn};
-  // dE/dc (B2): c, G and B rows in LDS per atom.
-  size_t per1 = (nc + (s->desc_kind == 2 ? 2 * (size_t)a.nd : 0)) * esz;
-  int apb1 = atoms_per_block(per1);
+  // dE/dc (B2): c, G and B rows in LDS per atom (each padded to 4 values).
+  size_t per1 = ((size_t)pad4((int)nc) + 2 * (size_t)pad4(a.nd)) * esz;
+  int apb1 = s->desc_kind == 2 ? atoms_per_block(per1) : 1;
   int nblk1 = (s->n + apb1 - 1) / apb1;
-  // Bond forces: u in LDS per atom.
-  size_t per2 = nc * esz;
+  // Bond forces: u in LDS per atom (radial index padded to 4).
+  size_t per2 = (size_t)a.nos * a.H * pad4(a.N) * esz;
   int apb2 = atoms_per_block(per2);
   int nblk2 = (s->n + apb2 - 1) / apb2;
   if (s->n == 0) return;
