@@ -129,75 +129,94 @@ __device__ __forceinline__ int mfma_row(int lane, int q) {
   return sizeof(T) == 8 ? (lane >> 4) + 4 * q : 4 * (lane >> 4) + q;
 }
 
-// Workspace (elements of T) of one atom of the descriptor kernel: c, then a
-// region holding the bond tables (Y 64 x H, g 64 x N, neighbour species 64)
-// and later the B row.
-__host__ __device__ inline size_t b2_work(int nc, int H, int N, int nd) {
-  size_t tables = 64 * (size_t)H + 64 * (size_t)N + 64;
-  return (size_t)nc + (tables > (size_t)nd ? tables : (size_t)nd);
+// Bonds per table chunk of the descriptor kernel (lanes [0, kBondChunk) write
+// the tables; a smaller chunk means a smaller LDS workspace per atom and so
+// more atoms resident per CU).
+constexpr int kBondChunk = 32;
+
+// Workspace (elements of T) of one atom of the descriptor kernel: c, then the
+// bond tables of one chunk (Y kBondChunk x H, g kBondChunk x N).
+__host__ __device__ inline size_t b2_work(int nc, int H, int N) {
+  return (size_t)nc + (size_t)kBondChunk * ((size_t)H + (size_t)N);
 }
 
 // c[(t N + n) H + lm] of atom i for all neighbour species t into LDS
-// (b2.cpp:367-395). For each chunk of up to 64 bonds, lane k takes bond k and
-// writes its radial basis g_n and every Y_lm to LDS tables; then, per species
-// t and 16 harmonics, c_t += G_t^T Y on the matrix cores (A = g_n of the
-// chunk's bonds of species t, B = their Y_lm).
+// (b2.cpp:367-395). For each chunk of up to kBondChunk bonds, lane k takes
+// bond k and writes its radial basis g_n and every Y_lm to LDS tables, the
+// rows grouped by neighbour species (list order kept within a species); then,
+// per species t and 16 harmonics, c_t += G_t^T Y_t on the matrix cores over
+// that species' rows only (A = g_n of the bonds, B = their Y_lm).
 template <typename T>
 __device__ void accumulate_c(const B2Args &a, int i, int lane, T *c, T *tab) {
   int si = a.species[i];
   int b0 = a.bond_off[i], b1 = a.bond_off[i + 1];
   int nc = a.nrad * a.H, H = a.H, N = a.N;
-  T *Yt = tab, *gt = tab + 64 * (size_t)H;
-  int *sjt = reinterpret_cast<int *>(gt + 64 * (size_t)N);
+  T *Yt = tab, *gt = tab + kBondChunk * (size_t)H;
   for (int k = lane; k < nc; k += 64) c[k] = 0;
   int r16 = lane & 15, k4 = lane >> 4;
-  for (int bb = b0; bb < b1; bb += 64) {
-    int nk = min(64, b1 - bb);
+  unsigned long long below = (1ull << lane) - 1ull;
+  for (int bb = b0; bb < b1; bb += kBondChunk) {
+    int nk = min(kBondChunk, b1 - bb);
+    int q = 0, sj = -1;
+    if (lane < nk) {
+      q = a.bond_q[bb + lane];
+      sj = a.nspec[q];
+    }
+    // Table row of this lane's bond: its rank among the chunk's bonds of
+    // its species, after the rows of the lower species.
+    int pos = 0, off = 0;
+    for (int t = 0; t < a.nos; t++) {
+      unsigned long long m = __ballot(sj == t);
+      if (sj == t) pos = off + __popcll(m & below);
+      off += __popcll(m);
+    }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     if (lane < nk) {
-      int q = a.bond_q[bb + lane];
-      int sj = a.nspec[q];
       const double *rp = a.rel + 4 * (size_t)q;
       T g[kMaxN], dg[kMaxN];
       radial_basis<T>((T)rp[0], (T)a.cutoffs[si * a.nos + sj], (T)a.r1, N, a.cutkind, g, dg);
 #pragma unroll
       for (int nn = 0; nn < kMaxN; nn++)
-        if (nn < N) gt[lane * N + nn] = g[nn];
-      sjt[lane] = sj;
+        if (nn < N) gt[pos * N + nn] = g[nn];
       T rinv = T(1) / (T)rp[0];
-      T *yrow = Yt + (size_t)lane * H;
+      T *yrow = Yt + (size_t)pos * H;
       ylm_sweep<T, false>(a.lmax, (T)rp[1] * rinv, (T)rp[2] * rinv, (T)rp[3] * rinv,
                           [&](int lm, T Y, T, T, T) { yrow[lm] = Y; });
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    int r0 = 0;
     for (int t = 0; t < a.nos; t++) {
-      for (int ct = 0; ct < H; ct += 16) {
-        int lm = ct + r16;
-        typename MfmaAcc<T>::type acc;
+      int r1 = r0 + __popcll(__ballot(sj == t));  // rows [r0, r1) are species t
+      if (r1 > r0) {
+        for (int ct = 0; ct < H; ct += 16) {
+          int lm = ct + r16;
+          typename MfmaAcc<T>::type acc;
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-          int n = mfma_row<T>(lane, q);
-          acc[q] = (n < N && lm < H) ? c[(t * N + n) * H + lm] : T(0);
-        }
-        for (int k0 = 0; k0 < nk; k0 += 4) {
-          int b = k0 + k4;
-          T av = 0, bv = 0;
-          if (b < nk) {
-            if (r16 < N && sjt[b] == t) av = gt[b * N + r16];
-            if (lm < H) bv = Yt[b * H + lm];
+          for (int qq = 0; qq < 4; qq++) {
+            int n = mfma_row<T>(lane, qq);
+            acc[qq] = (n < N && lm < H) ? c[(t * N + n) * H + lm] : T(0);
           }
-          acc = mfma_4(av, bv, acc);
-        }
+          for (int k0 = r0; k0 < r1; k0 += 4) {
+            int b = k0 + k4;
+            T av = 0, bv = 0;
+            if (b < r1) {
+              if (r16 < N) av = gt[b * N + r16];
+              if (lm < H) bv = Yt[b * H + lm];
+            }
+            acc = mfma_4(av, bv, acc);
+          }
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-          int n = mfma_row<T>(lane, q);
-          if (n < N && lm < H) c[(t * N + n) * H + lm] = acc[q];
+          for (int qq = 0; qq < 4; qq++) {
+            int n = mfma_row<T>(lane, qq);
+            if (n < N && lm < H) c[(t * N + n) * H + lm] = acc[qq];
+          }
         }
       }
+      r0 = r1;
     }
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -229,10 +248,13 @@ __global__ __launch_bounds__(64 * kAtomsPerBlock) void k_b2(B2Args a, T *B, doub
   for (int k = lane; k < nc; k += 64) Cg[(size_t)i * nc + k] = c[k];
   if (!want_b2) return;
   // B_l = C_l C_l^T on the matrix cores (C_l[n][m] = c[n, l^2 + m]); the
-  // upper triangle n2 >= n1 goes to the staged row at p(n1, n2) (L + 1) + l.
-  T *Bst = tab;
+  // upper triangle n2 >= n1 goes straight to the row at p(n1, n2) (L + 1) + l
+  // (each element is written once; the L2 merges a row's scattered stores).
   int L1 = a.lmax + 1;
   int r16 = lane & 15, k4 = lane >> 4;
+  int row = a.row_of_atom[i];
+  T *Brow = B + (size_t)row * a.nd;
+  T s2 = 0;
   for (int l = 0; l <= a.lmax; l++) {
     int nm = 2 * l + 1, lm0 = l * l;
     for (int rt = 0; rt < nr; rt += 16) {
@@ -249,32 +271,23 @@ __global__ __launch_bounds__(64 * kAtomsPerBlock) void k_b2(B2Args a, T *B, doub
         for (int q = 0; q < 4; q++) {
           int n1 = rt + mfma_row<T>(lane, q), n2 = n2b;
           if (n1 < nr && n2 < nr && n2 >= n1) {
-            int p = n1 * nr - (n1 * (n1 - 1)) / 2 + (n2 - n1);
-            Bst[p * L1 + l] = acc[q];
+            int k = (n1 * nr - (n1 * (n1 - 1)) / 2 + (n2 - n1)) * L1 + l;
+            T v = acc[q];
+            Brow[k] = v;
+            if constexpr (sizeof(T) == 4) {
+              float r = v;
+#pragma unroll
+              for (int pc = 0; pc < 2; pc++) {
+                __bf16 h = (__bf16)r;
+                Bs.p[pc][(size_t)row * ldp + k] = h;
+                r -= (float)h;
+              }
+            }
+            s2 += v * v;
           }
         }
       }
     }
-  }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  int row = a.row_of_atom[i];
-  T *Brow = B + (size_t)row * a.nd;
-  T s2 = 0;
-  for (int k = lane; k < a.nd; k += 64) {
-    T acc = Bst[k];
-    Brow[k] = acc;
-    if constexpr (sizeof(T) == 4) {
-      float r = acc;
-#pragma unroll
-      for (int pc = 0; pc < 2; pc++) {
-        __bf16 h = (__bf16)r;
-        Bs.p[pc][(size_t)row * ldp + k] = h;
-        r -= (float)h;
-      }
-    }
-    s2 += acc * acc;
   }
   if constexpr (sizeof(T) == 4)
     for (int k = a.nd + lane; k < ldp; k += 64)
@@ -386,31 +399,43 @@ __global__ __launch_bounds__(64 * kAtomsPerBlock) void k_dEdc(B2Args a, const T 
   // w[p(min, max), l] (nr x nr, symmetric), C_l[n2][m] = c[n2, l^2 + m]
   // (nr x (2l + 1)); 16 x 16 output tiles, 4 n2 per MFMA. Lane L supplies
   // A[L % 16][L / 16] and B[L / 16][L % 16]; it holds output rows
-  // tile_row(L, i), column L % 16.
+  // tile_row(L, i), column L % 16. The l loop is innermost (one accumulator
+  // per l) so the pair index of (n1, n2) is formed once for all l.
   int L1 = a.lmax + 1, nr = a.nrad;
   T *urow = Ug + (size_t)i * nc;
   int r16 = lane & 15, k4 = lane >> 4;
-  for (int l = 0; l <= a.lmax; l++) {
-    int nm = 2 * l + 1, lm0 = l * l;
+  for (int ct = 0; ct < 2 * a.lmax + 1; ct += 16) {
     for (int rt = 0; rt < nr; rt += 16) {
-      for (int ct = 0; ct < nm; ct += 16) {
-        typename MfmaAcc<T>::type acc = {};
-        int n1 = rt + r16, m = ct + r16;
-        for (int k0 = 0; k0 < nr; k0 += 4) {
-          int n2 = k0 + k4;
-          T av = 0, bv = 0;
-          if (n1 < nr && n2 < nr) {
-            int lo = n1 < n2 ? n1 : n2, hi = n1 < n2 ? n2 : n1;
-            int p = (lo * nr - (lo * (lo - 1)) / 2 + (hi - lo)) * L1 + l;
-            av = (n1 == n2 ? T(2) : T(1)) * (sg * wg[p] - sb * wb[p]);
-          }
-          if (n2 < nr && m < nm) bv = c[n2 * a.H + lm0 + m];
-          acc = mfma_4(av, bv, acc);
-        }
+      typename MfmaAcc<T>::type acc[kMaxL + 1];
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-          int row = rt + mfma_row<T>(lane, q);
-          if (row < nr && m < nm) urow[row * a.H + lm0 + m] = acc[q];
+      for (int l = 0; l <= kMaxL; l++) acc[l] = {};
+      int n1 = rt + r16, m = ct + r16;
+      for (int k0 = 0; k0 < nr; k0 += 4) {
+        int n2 = k0 + k4;
+        bool ok = n1 < nr && n2 < nr;
+        int lo = n1 < n2 ? n1 : n2, hi = n1 < n2 ? n2 : n1;
+        int p0 = ok ? pair_index(lo, hi, nr) * L1 : 0;
+        T coef = n1 == n2 ? T(2) : T(1);
+        const T *cr = c + (n2 < nr ? n2 : 0) * a.H;
+#pragma unroll
+        for (int l = 0; l <= kMaxL; l++) {
+          int nm = 2 * l + 1;
+          if (l <= a.lmax && ct < nm) {
+            T av = ok ? coef * (sg * wg[p0 + l] - sb * wb[p0 + l]) : T(0);
+            T bv = (n2 < nr && m < nm) ? cr[l * l + m] : T(0);
+            acc[l] = mfma_4(av, bv, acc[l]);
+          }
+        }
+      }
+#pragma unroll
+      for (int l = 0; l <= kMaxL; l++) {
+        int nm = 2 * l + 1;
+        if (l <= a.lmax && ct < nm) {
+#pragma unroll
+          for (int q = 0; q < 4; q++) {
+            int row = rt + mfma_row<T>(lane, q);
+            if (row < nr && m < nm) urow[row * a.H + l * l + m] = acc[l][q];
+          }
         }
       }
     }
@@ -607,7 +632,7 @@ void launch_b2(fm_structure s, bool want_b2) {
   upload_rcp(st);
   B2Args a = make_args(s);
   size_t esz = s->form == FM_FORM_FP64 ? sizeof(double) : sizeof(float);
-  size_t work = b2_work(a.nrad * a.H, a.H, a.N, want_b2 ? a.nd : 0);
+  size_t work = b2_work(a.nrad * a.H, a.H, a.N);
   size_t per = work * esz;
   int apb = atoms_per_block(per);
   size_t lds = apb * per;
