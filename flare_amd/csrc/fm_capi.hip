@@ -195,6 +195,8 @@ int fm_context_create(int device, fm_context *out) {
     auto *c = new fm_context_s();
     c->device = device;
     FM_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    FM_HIP(hipHostMalloc(reinterpret_cast<void **>(&c->pinned),
+                         fm_context_s::kPinnedWords * sizeof(int), hipHostMallocDefault));
     for (int s = 0; s < fm::ST_COUNT; s++) {
       FM_HIP(hipEventCreate(&c->ev_begin[s]));
       FM_HIP(hipEventCreate(&c->ev_end[s]));
@@ -213,6 +215,7 @@ int fm_context_destroy(fm_context ctx) {
       (void)hipEventDestroy(ctx->ev_end[s]);
     }
     hipStream_t st = ctx->stream;
+    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     delete ctx;
     (void)hipStreamDestroy(st);
   });
@@ -305,20 +308,24 @@ static int structure_create(fm_context ctx, int n_struct, const double *cells,
         FM_HIP(hipMemcpyAsync(s->species.as<int>(), species, (size_t)n * sizeof(int), kind, st));
         FM_HIP(hipMemcpyAsync(s->positions.as<double>(), positions, 3 * (size_t)n * sizeof(double),
                               kind, st));
+        // The position / species flags are read back with the neighbour
+        // build's size readback (one synchronisation); the neighbour kernels
+        // only bin (clamped) and copy species, so bad input cannot take
+        // them out of bounds before it is reported.
         fm::DevBuf flags(pool, 2 * sizeof(int));
         FM_HIP(hipMemsetAsync(flags.as<int>(), 0, 2 * sizeof(int), st));
         hipLaunchKernelGGL(k_check_positions, dim3((n + 255) / 256), dim3(256), 0, st, n,
                            s->positions.as<double>(), s->species.as<int>(), flags.as<int>());
-        int hf[2] = {0, 0};
-        FM_HIP(hipMemcpyAsync(hf, flags.as<int>(), 2 * sizeof(int), hipMemcpyDeviceToHost, st));
-        FM_HIP(hipStreamSynchronize(st));
-        FM_CHECK(!(hf[0] & 1), FM_ERR_INVALID, "positions contain NaN or Inf");
-        FM_CHECK(!(hf[0] & 2), FM_ERR_INVALID, "species must be non-negative");
-        s->max_species = hf[1];
+        FM_HIP(hipMemcpyAsync(ctx->pinned, flags.as<int>(), 2 * sizeof(int),
+                              hipMemcpyDeviceToHost, st));
       }
       ctx->mark_begin(fm::ST_NEIGH);
       if (n > 0) {
-        fm::build_neighbors(s);
+        fm::build_neighbors(s);  // synchronises: ctx->pinned[0, 2) is valid after
+        int hf0 = ctx->pinned[0], hf1 = ctx->pinned[1];
+        FM_CHECK(!(hf0 & 1), FM_ERR_INVALID, "positions contain NaN or Inf");
+        FM_CHECK(!(hf0 & 2), FM_ERR_INVALID, "species must be non-negative");
+        s->max_species = hf1;
       } else {
         s->cumulative.alloc(pool, sizeof(int));
         FM_HIP(hipMemsetAsync(s->cumulative.as<int>(), 0, sizeof(int), st));

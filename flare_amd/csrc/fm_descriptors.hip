@@ -200,14 +200,22 @@ __device__ void accumulate_c(const B2Args &a, int i, int lane, T *c, T *tab) {
             int n = mfma_row<T>(lane, qq);
             acc[qq] = (n < N && lm < H) ? c[(t * N + n) * H + lm] : T(0);
           }
-          for (int k0 = r0; k0 < r1; k0 += 4) {
-            int b = k0 + k4;
-            T av = 0, bv = 0;
-            if (b < r1) {
-              if (r16 < N) av = gt[b * N + r16];
-              if (lm < H) bv = Yt[b * H + lm];
+          // Four k-steps per iteration: the loads (clamped in bounds, then
+          // zeroed outside the species' rows) all issue before the MFMAs.
+          int rc = r16 < N ? r16 : N - 1, lc = lm < H ? lm : H - 1;
+          for (int k0 = r0; k0 < r1; k0 += 16) {
+            T av[4], bv[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+              int b = k0 + 4 * u + k4;
+              int bc = b < kBondChunk ? b : kBondChunk - 1;
+              T x = gt[bc * N + rc], y = Yt[bc * H + lc];
+              av[u] = (b < r1 && r16 < N) ? x : T(0);
+              bv[u] = (b < r1 && lm < H) ? y : T(0);
             }
-            acc = mfma_4(av, bv, acc);
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+              if (k0 + 4 * u < r1) acc = mfma_4(av[u], bv[u], acc);
           }
 #pragma unroll
           for (int qq = 0; qq < 4; qq++) {
@@ -261,11 +269,20 @@ __global__ __launch_bounds__(64 * kAtomsPerBlock) void k_b2(B2Args a, T *B, doub
       for (int cc = rt; cc < nr; cc += 16) {
         typename MfmaAcc<T>::type acc = {};
         int n1a = rt + r16, n2b = cc + r16;
-        for (int k0 = 0; k0 < nm; k0 += 4) {
-          int m = k0 + k4;
-          T av = (n1a < nr && m < nm) ? c[n1a * H + lm0 + m] : T(0);
-          T bv = (n2b < nr && m < nm) ? c[n2b * H + lm0 + m] : T(0);
-          acc = mfma_4(av, bv, acc);
+        const T *ca = c + (n1a < nr ? n1a : nr - 1) * H + lm0;
+        const T *cb = c + (n2b < nr ? n2b : nr - 1) * H + lm0;
+        for (int k0 = 0; k0 < nm; k0 += 8) {
+          T av[2], bv[2];
+#pragma unroll
+          for (int u = 0; u < 2; u++) {
+            int m = k0 + 4 * u + k4, mc = m < nm ? m : nm - 1;
+            T x = ca[mc], y = cb[mc];
+            av[u] = (n1a < nr && m < nm) ? x : T(0);
+            bv[u] = (n2b < nr && m < nm) ? y : T(0);
+          }
+#pragma unroll
+          for (int u = 0; u < 2; u++)
+            if (k0 + 4 * u < nm) acc = mfma_4(av[u], bv[u], acc);
         }
 #pragma unroll
         for (int q = 0; q < 4; q++) {
@@ -417,12 +434,15 @@ __global__ __launch_bounds__(64 * kAtomsPerBlock) void k_dEdc(B2Args a, const T 
         int p0 = ok ? pair_index(lo, hi, nr) * L1 : 0;
         T coef = n1 == n2 ? T(2) : T(1);
         const T *cr = c + (n2 < nr ? n2 : 0) * a.H;
+        // Loads clamped in bounds and zeroed by selects (no divergent branches).
 #pragma unroll
         for (int l = 0; l <= kMaxL; l++) {
           int nm = 2 * l + 1;
           if (l <= a.lmax && ct < nm) {
-            T av = ok ? coef * (sg * wg[p0 + l] - sb * wb[p0 + l]) : T(0);
-            T bv = (n2 < nr && m < nm) ? cr[l * l + m] : T(0);
+            int mc = m < nm ? m : nm - 1;
+            T g = wg[p0 + l], b = wb[p0 + l], x = cr[l * l + mc];
+            T av = ok ? coef * (sg * g - sb * b) : T(0);
+            T bv = (n2 < nr && m < nm) ? x : T(0);
             acc[l] = mfma_4(av, bv, acc[l]);
           }
         }

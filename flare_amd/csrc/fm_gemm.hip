@@ -243,7 +243,7 @@ __device__ __forceinline__ void run_epilogue(const EpArgs &e, const TileOut<Acc,
 // Base shape: the partial arrays (gemm_col_tiles / gemm_row_tiles) follow it.
 constexpr int F64_TM = 2, F64_TN = 2, F64_BM = 32 * F64_TM, F64_BN = 32 * F64_TN;
 constexpr int BK64 = 16;
-constexpr int PAD64 = 4;
+constexpr int PAD64 = 1;  // row stride 65 or 129 doubles: the transposed stores spread over the banks
 
 // Four consecutive k of one row, widened to FP64: one or two 16-byte loads
 // when `vec` (the row start is 16-byte aligned and k + 4 <= kend), else

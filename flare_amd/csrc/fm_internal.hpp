@@ -112,6 +112,11 @@ struct fm_context_s {
   int device = 0;
   hipStream_t stream = nullptr;
   fm::Pool pool;
+  // Pinned host words for small device-to-host readbacks (sizes, flags), so
+  // several can be queued and waited for with one stream synchronisation.
+  // Calls on one context are sequential (they share its stream).
+  static constexpr int kPinnedWords = 64;
+  int *pinned = nullptr;
   hipEvent_t ev_begin[fm::ST_COUNT], ev_end[fm::ST_COUNT];
   bool recorded[fm::ST_COUNT] = {};
   int launches[fm::ST_COUNT] = {};

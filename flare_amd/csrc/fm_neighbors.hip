@@ -458,10 +458,15 @@ void build_neighbors(fm_structure s) {
                      s->species.as<int>(), cas, ast, boffp, s->count.as<int>(), nullptr, nullptr,
                      nullptr, nullptr, maxc.as<int>());
   FM_HIP(hipGetLastError());
-  int total = scan_exclusive(ctx, s->count.as<int>(), s->cumulative.as<int>(), n, true);
-  int max_count = 0;
-  FM_HIP(hipMemcpyAsync(&max_count, maxc.as<int>(), sizeof(int), hipMemcpyDeviceToHost, st));
+  scan_exclusive(ctx, s->count.as<int>(), s->cumulative.as<int>(), n, false);
+  // Total and largest count in pinned words [2, 4), one synchronisation
+  // (which also completes any readback queued before, e.g. words [0, 2)).
+  int *hw = ctx->pinned;
+  FM_HIP(hipMemcpyAsync(hw + 2, s->cumulative.as<int>() + n, sizeof(int), hipMemcpyDeviceToHost,
+                        st));
+  FM_HIP(hipMemcpyAsync(hw + 3, maxc.as<int>(), sizeof(int), hipMemcpyDeviceToHost, st));
   FM_HIP(hipStreamSynchronize(st));
+  int total = hw[2], max_count = hw[3];
   s->n_nb = total;
   s->sidx.alloc(pool, std::max<size_t>(1, (size_t)total * sizeof(int)));
   s->nspecies.alloc(pool, std::max<size_t>(1, (size_t)total * sizeof(int)));
@@ -574,16 +579,44 @@ void build_bonds(fm_structure s) {
   hipStream_t st = ctx->stream;
   Pool *pool = &ctx->pool;
   int n = s->n, tpb = 256, nblk = (n + tpb - 1) / tpb;
+  FM_CHECK(s->nos + 1 <= fm_context_s::kPinnedWords, FM_ERR_INVALID, "too many species");
   s->d_cutoffs.alloc(pool, s->cutoffs.size() * sizeof(double));
   FM_HIP(hipMemcpyAsync(s->d_cutoffs.as<double>(), s->cutoffs.data(),
                         s->cutoffs.size() * sizeof(double), hipMemcpyHostToDevice, st));
+  // Species-sorted rows: one scan of species flags per species.
+  s->row_of_atom.alloc(pool, (size_t)n * sizeof(int));
+  s->atom_of_row.alloc(pool, (size_t)n * sizeof(int));
+  DevBuf flag(pool, (size_t)n * sizeof(int));
+  DevBuf rank(pool, (size_t)s->nos * ((size_t)n + 1) * sizeof(int));
+  for (int t = 0; t < s->nos; t++) {
+    hipLaunchKernelGGL(k_species_flag, dim3(nblk), dim3(tpb), 0, st, n, s->species.as<int>(), t,
+                       flag.as<int>());
+    scan_exclusive(ctx, flag.as<int>(), rank.as<int>() + (size_t)t * (n + 1), n, false);
+  }
   DevBuf bcount(pool, (size_t)n * sizeof(int));
   s->bond_off.alloc(pool, ((size_t)n + 1) * sizeof(int));
   hipLaunchKernelGGL(k_bond_count, dim3((n + 3) / 4), dim3(256), 0, st, n, s->species.as<int>(),
                      s->cumulative.as<int>(), s->nspecies.as<int>(), s->rel.as<double>(),
                      s->d_cutoffs.as<double>(), s->nos, bcount.as<int>());
-  int nbonds = scan_exclusive(ctx, bcount.as<int>(), s->bond_off.as<int>(), n, true);
+  scan_exclusive(ctx, bcount.as<int>(), s->bond_off.as<int>(), n, false);
+  // Bond total and per-species atom counts: one readback, one synchronisation.
+  int *hw = ctx->pinned;
+  FM_HIP(hipMemcpyAsync(hw, s->bond_off.as<int>() + n, sizeof(int), hipMemcpyDeviceToHost, st));
+  for (int t = 0; t < s->nos; t++)
+    FM_HIP(hipMemcpyAsync(hw + 1 + t, rank.as<int>() + (size_t)t * (n + 1) + n, sizeof(int),
+                          hipMemcpyDeviceToHost, st));
+  FM_HIP(hipStreamSynchronize(st));
+  int nbonds = hw[0];
   s->n_bonds = nbonds;
+  s->n_by_type.assign(s->nos, 0);
+  s->type_off.assign(s->nos + 1, 0);
+  for (int t = 0; t < s->nos; t++) {
+    s->n_by_type[t] = hw[1 + t];
+    s->type_off[t + 1] = s->type_off[t] + hw[1 + t];
+    hipLaunchKernelGGL(k_species_rows, dim3(nblk), dim3(tpb), 0, st, n, s->species.as<int>(), t,
+                       rank.as<int>() + (size_t)t * (n + 1), s->type_off[t],
+                       s->row_of_atom.as<int>(), s->atom_of_row.as<int>());
+  }
   s->bond_q.alloc(pool, (size_t)nbonds * sizeof(int));
   DevBuf in_count(pool, (size_t)n * sizeof(int));
   FM_HIP(hipMemsetAsync(in_count.as<int>(), 0, (size_t)n * sizeof(int), st));
@@ -603,31 +636,6 @@ void build_bonds(fm_structure s) {
                        cursor.as<int>(), unordered.as<int>());
     hipLaunchKernelGGL(k_in_rank, dim3((n + 3) / 4), dim3(256), 0, st, n, s->in_off.as<int>(),
                        unordered.as<int>(), s->in_bond.as<int>());
-  }
-  // Species-sorted rows: one scan of species flags per species, the
-  // per-species totals fetched with a single copy.
-  s->row_of_atom.alloc(pool, (size_t)n * sizeof(int));
-  s->atom_of_row.alloc(pool, (size_t)n * sizeof(int));
-  DevBuf flag(pool, (size_t)n * sizeof(int));
-  DevBuf rank(pool, (size_t)s->nos * ((size_t)n + 1) * sizeof(int));
-  for (int t = 0; t < s->nos; t++) {
-    hipLaunchKernelGGL(k_species_flag, dim3(nblk), dim3(tpb), 0, st, n, s->species.as<int>(), t,
-                       flag.as<int>());
-    scan_exclusive(ctx, flag.as<int>(), rank.as<int>() + (size_t)t * (n + 1), n, false);
-  }
-  std::vector<int> counts(s->nos);
-  for (int t = 0; t < s->nos; t++)
-    FM_HIP(hipMemcpyAsync(&counts[t], rank.as<int>() + (size_t)t * (n + 1) + n, sizeof(int),
-                          hipMemcpyDeviceToHost, st));
-  FM_HIP(hipStreamSynchronize(st));
-  s->n_by_type.assign(s->nos, 0);
-  s->type_off.assign(s->nos + 1, 0);
-  for (int t = 0; t < s->nos; t++) {
-    s->n_by_type[t] = counts[t];
-    s->type_off[t + 1] = s->type_off[t] + counts[t];
-    hipLaunchKernelGGL(k_species_rows, dim3(nblk), dim3(tpb), 0, st, n, s->species.as<int>(), t,
-                       rank.as<int>() + (size_t)t * (n + 1), s->type_off[t],
-                       s->row_of_atom.as<int>(), s->atom_of_row.as<int>());
   }
   FM_HIP(hipGetLastError());
 }
