@@ -117,3 +117,53 @@ def test_bad_files_raise(C, tmp_path):
         MappedPotential(path)
     with pytest.raises(ValueError, match="Cannot open"):
         MappedPotential(os.path.join(tmp_path, "missing.flare"))
+
+
+def _sgp_sigma_atom_order(sgp, s, species):
+    """sigma / |sigma_f| of the sparse GP in atom order (calculator.py:108-122)."""
+    sgp.predict_local_uncertainties(s)
+    var = s.local_uncertainties[0]
+    order = np.argsort(np.asarray(species), kind="stable")
+    v = np.zeros(len(species))
+    v[order] = var
+    sig = sgp.kernels[0].sigma
+    return np.sign(v) * np.sqrt(np.abs(v)) / sig
+
+
+@pytest.mark.parametrize("power", [1.0, 2.0])
+def test_std_atom_from_files(C, oracle, power, tmp_path):
+    """compute flare/std/atom from the L_inv and sparse-descriptor files
+    equals the sparse GP's sigma / |sigma_f| (tests/test_lammps.py:148-260)
+    and the oracle's local variance."""
+    from flare_amd.mapping import StdAtom, write_L_inverse, write_sparse_descriptors
+    cell, species, pos, cut, calc, s, sgp = setup(C, power)
+    p1, p2 = os.path.join(tmp_path, "L_inv.flare"), os.path.join(tmp_path, "sparse.flare")
+    write_L_inverse(sgp, p1, "tester", [calc])
+    write_sparse_descriptors(sgp, p2, "tester")
+    std = StdAtom(p1, p2).compute(cell, species, pos)
+    want = _sgp_sigma_atom_order(sgp, s, species)
+    assert np.max(np.abs(std ** 2 * np.sign(std) - want ** 2 * np.sign(want))) < 1e-10
+    cd = sgp.sparse_descriptors[0]
+    m = oracle.Model(1.5, power, cd.descriptors, cd.descriptor_norms, sgp.alpha, L_inv=sgp.L_inv)
+    ref = oracle.B2Values(oracle.Neighbors(cell, species, pos, float(cut.max()), "literal"),
+                          2, 4, 3, cut)
+    _, var_ref = m.predict(ref, "local")
+    order = np.argsort(np.asarray(species), kind="stable")
+    v = np.zeros(len(species))
+    v[order] = np.asarray(var_ref) / 1.5 ** 2
+    assert np.max(np.abs(std ** 2 * np.sign(std) - v)) < 1e-10
+
+
+def test_std_atom_varmap(C, tmp_path):
+    """The variance map (power 1) gives the sigma of the L_inv files."""
+    from flare_amd.mapping import (StdAtom, write_L_inverse, write_sparse_descriptors,
+                                   write_varmap_coefficients)
+    cell, species, pos, cut, calc, s, sgp = setup(C, 1.0)
+    p1, p2, p3 = (os.path.join(tmp_path, f) for f in ("L_inv.flare", "sparse.flare", "var.flare"))
+    write_L_inverse(sgp, p1, "tester", [calc])
+    write_sparse_descriptors(sgp, p2, "tester")
+    write_varmap_coefficients(sgp, p3, "tester", 0, calc)
+    a = StdAtom(p1, p2).compute(cell, species, pos)
+    b = StdAtom(p3).compute(cell, species, pos)
+    assert np.max(np.abs(a ** 2 * np.sign(a) - b ** 2 * np.sign(b))) < 1e-9
+    assert np.all(np.abs(b) <= 1.0 + 1e-12)
