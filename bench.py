@@ -64,7 +64,8 @@ def build_model(C_, cell, species, pos, n_sparse, seed_sparse=3, seed_alpha=4):
     return calc, sgp
 
 
-def run_form(lib, ctx, model_h, calc, cell, sp_dev, pos_dev, n, form, steps, warmup):
+def run_form(lib, ctx, model_h, calc, cell, sp_dev, pos_dev, n, form, steps, warmup,
+             cutoff=6.0):
     from flare_amd import _lib
     settings, keep = calc._settings()
     cellv = _lib.f64(cell.reshape(-1))
@@ -77,7 +78,7 @@ def run_form(lib, ctx, model_h, calc, cell, sp_dev, pos_dev, n, form, steps, war
     def step():
         h = C.c_void_p()
         _lib.check(lib.fm_structure_create_device(ctx.h, _lib.ptr(cellv), n, sp_dev, pos_dev,
-                                                  6.0, C.byref(h)))
+                                                  cutoff, C.byref(h)))
         _lib.check(lib.fm_structure_compute_b2(h, C.byref(settings), form))
         _lib.check(lib.fm_predict(model_h, h, flags, form, _lib.ptr(mean), _lib.ptr(var), None))
         lib.fm_structure_destroy(h)
@@ -177,6 +178,55 @@ def batched_leg(lib, ctx, hip, steps, warmup):
                             **sigma_errors(res["fp64"][1], res["chosen"][1],
                                            sgp.kernels[0].sigma)}
     return out, (sgp, calc)
+
+
+def large_leg(lib, ctx, hip, steps, warmup):
+    """Config 5 (B2 kernel): a 1,048,576-atom three-species BCC alloy
+    (a = 3.2 A, 128 x 64 x 64 cells, displacements 0.1 A, seed 8); B2
+    [3, 10, 6] rc 5.5 (d = 3255); NDP sigma 2, p 2; 4096 sparse environments
+    (seed 9) drawn from a 16^3-cell sample of the same alloy (seed 11; the
+    sparse rows of the full structure would be read back whole, 27 GB).
+    One step = one full evaluation of the 1M atoms; FP64 vs chosen."""
+    from flare_amd import _C_flare as CF
+    from flare_amd import _lib
+    from flare_amd.workloads import BCC_BASIS, random_alloy
+    cut = np.full((3, 3), 5.5)
+    calc = CF.B2("chebyshev", "quadratic", [0.0, 5.5], [], [3, 10, 6], cut)
+    cs, ss, ps = random_alloy((16, 16, 16), a=3.2, nos=3, sigma=0.1, seed=11, basis=BCC_BASIS)
+    small = CF.Structure(cs, ss, ps, 5.5, [calc])
+    sgp = CF.SparseGP([CF.NormalizedDotProduct(2.0, 2.0)], 0.01, 0.05, 0.005)
+    picks = np.random.default_rng(9).choice(len(ss), 4096, replace=False)
+    sgp.add_specific_environments(small, sorted(int(a) for a in picks))
+    del small
+    sgp.update_L_inverse()
+    sgp.set_alpha(np.random.default_rng(4).normal(scale=0.1, size=sgp.n_sparse))
+    cell, species, pos = random_alloy((128, 64, 64), a=3.2, nos=3, sigma=0.1, seed=8,
+                                      basis=BCC_BASIS)
+    n = len(pos)
+    pos_dev = hip.upload(np.ascontiguousarray(pos.reshape(-1)))
+    sp_dev = hip.upload(np.ascontiguousarray(species.astype(np.int32)))
+    hip.L.hipDeviceSynchronize()
+    model_h = sgp._model(0, ctx).h
+    out, res = {}, {}
+    for name, form in (("fp64", _lib.FORM_FP64), ("chosen", _lib.FORM_CHOSEN)):
+        log(f"  1M atoms, form {name} ...")
+        r = run_form(lib, ctx, model_h, calc, cell, sp_dev, pos_dev, n, form, steps, warmup,
+                     cutoff=5.5)
+        res[name] = r
+        out[name] = {"atom_evals_per_s": n * steps / r["total"],
+                     "ms_per_step": r["total"] / steps * 1e3,
+                     "stage_ms": {k: round(v[0] / steps, 3) for k, v in r["stage"].items()}}
+    m64, mch = res["fp64"]["mean"], res["chosen"]["mean"]
+    out["workload"] = (f"{n} atoms, 3-species BCC alloy a = 3.2 A (128x64x64 cells); "
+                       f"B2 [3,10,6] rc 5.5 (d=3255); {sgp.n_sparse} sparse; NDP sigma 2 p 2")
+    out["speedup_vs_fp64"] = out["chosen"]["atom_evals_per_s"] / out["fp64"]["atom_evals_per_s"]
+    out["error_vs_fp64"] = {"energy_meV_per_atom": abs(m64[0] - mch[0]) / n * 1e3,
+                            "force_max_meV_per_A": float(np.abs(m64[1:-6] - mch[1:-6]).max()) * 1e3,
+                            **sigma_errors(res["fp64"]["var"], res["chosen"]["var"],
+                                           sgp.kernels[0].sigma)}
+    hip.L.hipFree(pos_dev)
+    hip.L.hipFree(sp_dev)
+    return out
 
 
 def serving_leg(sgp, calc, batched, n_requests=2000, n_fp64=200):
@@ -283,10 +333,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--sparse", type=int, default=2000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workloads", default="all", choices=["single", "batched", "all"],
-                    help="single: the 32k-atom structure only; all: also the batched and "
-                         "served workloads; batched: the batched workload alone (its own "
-                         "JSON line, for profiling)")
+    ap.add_argument("--workloads", default="all", choices=["single", "batched", "large", "all"],
+                    help="single: the 32k-atom structure only; all: also the batched, "
+                         "served and 1M-atom workloads; batched / large: that workload "
+                         "alone (its own JSON line, for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a 512 MiB buffer three times (the known-size kernel "
                          "that FETCH_SIZE / WRITE_SIZE counters are scaled by)")
@@ -314,6 +364,10 @@ def main():
         log("batched workload ...")
         out, _ = batched_leg(lib, ctx, Hip(), max(3, args.steps // 2), 1)
         print(json.dumps({"batched": out}), flush=True)
+        return
+    if args.workloads == "large":
+        log("1M-atom workload ...")
+        print(json.dumps({"large_b2": large_leg(lib, ctx, Hip(), 2, 1)}), flush=True)
         return
     cell, species, pos = fcc_alloy_32k()
     n = len(pos)
@@ -397,6 +451,8 @@ def main():
                                                     1)
         log("serving workload ...")
         out["serving"] = serving_leg(bsgp, bcalc, out["batched"])
+        log("1M-atom workload ...")
+        out["large_b2"] = large_leg(lib, ctx, hip, 2, 1)
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
         log(f"cpu baseline on {threads} threads ...")

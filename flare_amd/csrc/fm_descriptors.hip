@@ -328,6 +328,11 @@ __global__ __launch_bounds__(64 * kAtomsPerBlock) void k_b2(B2Args a, T *B, doub
 struct AdjArgs {
   int project;
   double empty_norm, empty_nn;
+  // With project: G . B per row from the local energies when given,
+  // G . B = p |B| E (G = sum_j sigma^2 alpha_j p S_j^(p-1) x_hat_j and
+  // E = sum_j sigma^2 alpha_j S_j^p), else a pass over the G and B rows.
+  const double *energy;
+  double power;
 };
 
 __device__ __forceinline__ int pair_index(int lo, int hi, int nr) {
@@ -354,31 +359,45 @@ __device__ __forceinline__ void stage_row(const T *src, T *dst, int n, int lane)
   for (int k = done + lane; k < n; k += 64) dst[k] = src[k];
 }
 
-// Two rows of n elements each (the G and B rows of an atom) to LDS, their
-// loads interleaved so eight 16-byte vectors are in flight per lane.
+// w = sg G - sb B (the atom's dE/dB row) into LDS, n elements: 16-byte
+// loads of G and B when both rows are 16-byte aligned, four vectors of each
+// in flight per lane; B is not read when sb is zero.
 template <typename T>
-__device__ __forceinline__ void stage_rows2(const T *s0, T *d0, const T *s1, T *d1, int n,
-                                            int lane) {
+__device__ __forceinline__ void stage_w(const T *G, const T *B, T sg, T sb, T *w, int n,
+                                        int lane) {
   constexpr int E = 16 / sizeof(T);
-  if (((reinterpret_cast<uintptr_t>(s0) | reinterpret_cast<uintptr_t>(s1)) & 15) != 0) {
-    stage_row(s0, d0, n, lane);
-    stage_row(s1, d1, n, lane);
-    return;
-  }
-  using V = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
-  const V *a = reinterpret_cast<const V *>(s0), *b = reinterpret_cast<const V *>(s1);
-  V *da = reinterpret_cast<V *>(d0), *db = reinterpret_cast<V *>(d1);
-  int nv = n / E;
+  int done = 0;
+  bool withb = sb != T(0);
+  if (((reinterpret_cast<uintptr_t>(G) | (withb ? reinterpret_cast<uintptr_t>(B) : 0)) & 15) ==
+      0) {
+    using V = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
+    const V *gv = reinterpret_cast<const V *>(G), *bv = reinterpret_cast<const V *>(B);
+    V *wv = reinterpret_cast<V *>(w);
+    int nv = n / E;
+    if (withb) {
 #pragma unroll 4
-  for (int v = lane; v < nv; v += 64) {
-    V x = a[v], y = b[v];
-    da[v] = x;
-    db[v] = y;
+      for (int v = lane; v < nv; v += 64) {
+        V x = gv[v], y = bv[v], r;
+        const T *xs = reinterpret_cast<const T *>(&x), *ys = reinterpret_cast<const T *>(&y);
+        T *rs = reinterpret_cast<T *>(&r);
+#pragma unroll
+        for (int e = 0; e < E; e++) rs[e] = sg * xs[e] - sb * ys[e];
+        wv[v] = r;
+      }
+    } else {
+#pragma unroll 4
+      for (int v = lane; v < nv; v += 64) {
+        V x = gv[v], r;
+        const T *xs = reinterpret_cast<const T *>(&x);
+        T *rs = reinterpret_cast<T *>(&r);
+#pragma unroll
+        for (int e = 0; e < E; e++) rs[e] = sg * xs[e];
+        wv[v] = r;
+      }
+    }
+    done = nv * E;
   }
-  for (int k = nv * E + lane; k < n; k += 64) {
-    d0[k] = s0[k];
-    d1[k] = s1[k];
-  }
+  for (int k = done + lane; k < n; k += 64) w[k] = withb ? sg * G[k] - sb * B[k] : sg * G[k];
 }
 
 template <typename T>
@@ -392,26 +411,32 @@ __global__ __launch_bounds__(64 * kAtomsPerBlock) void k_dEdc(B2Args a, const T 
   int i = blockIdx.x * (blockDim.x >> 6) + w;
   if (i >= a.n) return;
   int nc = a.nrad * a.H, ncp = pad4(nc), ndp = pad4(a.nd);
-  T *c = reinterpret_cast<T *>(smem) + (size_t)w * ((size_t)ncp + 2 * (size_t)ndp);
-  T *wg = c + ncp, *wb = wg + ndp;  // G and B rows
+  T *c = reinterpret_cast<T *>(smem) + (size_t)w * ((size_t)ncp + (size_t)ndp);
+  T *wl = c + ncp;  // the w = dE/dB row
   int row = a.row_of_atom[i];
   double nrm = norm[row];
   if (nrm < aa.empty_norm || nnv[row] < aa.empty_nn) return;
-  // One pass over HBM with 16-byte loads: c, G and B rows to LDS.
-  stage_row(Cg + (size_t)i * nc, c, nc, lane);
-  stage_rows2(G + (size_t)row * g_ld, wg, B + (size_t)row * a.nd, wb, a.nd, lane);
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  // w = dE/dB = (G - (G . B_hat) B_hat) / |B| = sg G - sb B, formed when the
-  // MFMA operand is read.
+  const T *Brow = B + (size_t)row * a.nd;
+  const T *Grow = G + (size_t)row * g_ld;
+  // w = dE/dB = (G - (G . B_hat) B_hat) / |B| = sg G - sb B.
   T ninv = T(1) / (T)nrm;
   T sg = aa.project ? ninv : T(1), sb = 0;
   if (aa.project) {
-    T gb = 0;
-    for (int k = lane; k < a.nd; k += 64) gb += wg[k] * wb[k];
-    sb = wave_sum(gb) * ninv * ninv * ninv;
+    if (aa.energy) {
+      sb = (T)(aa.power * aa.energy[row] / (nrm * nrm));
+    } else {
+      T gb = 0;
+#pragma unroll 8
+      for (int k = lane; k < a.nd; k += 64) gb += Grow[k] * Brow[k];
+      sb = wave_sum(gb) * ninv * ninv * ninv;
+    }
   }
+  // One pass over HBM with 16-byte loads: c and w to LDS.
+  stage_row(Cg + (size_t)i * nc, c, nc, lane);
+  stage_w(Grow, Brow, sg, sb, wl, a.nd, lane);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   // u_l = W_l C_l per l on the matrix cores: W_l[n1][n2] = (1 + [n1 = n2])
   // w[p(min, max), l] (nr x nr, symmetric), C_l[n2][m] = c[n2, l^2 + m]
   // (nr x (2l + 1)); 16 x 16 output tiles, 4 n2 per MFMA. Lane L supplies
@@ -440,8 +465,8 @@ __global__ __launch_bounds__(64 * kAtomsPerBlock) void k_dEdc(B2Args a, const T 
           int nm = 2 * l + 1;
           if (l <= a.lmax && ct < nm) {
             int mc = m < nm ? m : nm - 1;
-            T g = wg[p0 + l], b = wb[p0 + l], x = cr[l * l + mc];
-            T av = ok ? coef * (sg * g - sb * b) : T(0);
+            T g = wl[p0 + l], x = cr[l * l + mc];
+            T av = ok ? coef * g : T(0);
             T bv = (n2 < nr && m < nm) ? x : T(0);
             acc[l] = mfma_4(av, bv, acc[l]);
           }
@@ -673,7 +698,8 @@ void launch_b2(fm_structure s, bool want_b2) {
 }
 
 void launch_adjoint(fm_structure s, const void *G, int64_t g_ld, void *fbond,
-                    double *virial_atom, int project, double empty_norm, double empty_nn) {
+                    double *virial_atom, int project, double empty_norm, double empty_nn,
+                    const double *energy, double power) {
   fm_context ctx = s->ctx;
   hipStream_t st = ctx->stream;
   B2Args a = make_args(s);
@@ -684,9 +710,9 @@ void launch_adjoint(fm_structure s, const void *G, int64_t g_ld, void *fbond,
   size_t esz = f64 ? sizeof(double) : sizeof(float);
   size_t nc = (size_t)a.nrad * a.H;
   DevBuf U(&ctx->pool, std::max<size_t>(1, (size_t)s->n * nc * esz));
-  AdjArgs aa{project, empty_norm, empty_nn};
-  // dE/dc (B2): c, G and B rows in LDS per atom (each padded to 4 values).
-  size_t per1 = ((size_t)pad4((int)nc) + 2 * (size_t)pad4(a.nd)) * esz;
+  AdjArgs aa{project, empty_norm, empty_nn, energy, power};
+  // dE/dc (B2): c and w rows in LDS per atom (each padded to 4 values).
+  size_t per1 = ((size_t)pad4((int)nc) + (size_t)pad4(a.nd)) * esz;
   int apb1 = s->desc_kind == 2 ? atoms_per_block(per1) : 1;
   int nblk1 = (s->n + apb1 - 1) / apb1;
   // Bond forces: u in LDS per atom (radial index padded to 4).

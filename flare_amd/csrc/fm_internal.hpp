@@ -226,8 +226,12 @@ void launch_b3(fm_structure s);
 void launch_dEdc_b3(fm_structure s, const void *G, int64_t g_ld, int project,
                     double empty_norm, double empty_nn, void *U);
 void free_b3_state(fm_structure s);
+// With project, G . B comes from `energy` (per species-sorted row; the
+// rows' local energies of a normalized dot product kernel of this power)
+// when given, else from a pass over the G and B rows.
 void launch_adjoint(fm_structure s, const void *G, int64_t g_ld, void *fbond,
-                    double *virial_atom, int project, double empty_norm, double empty_nn);
+                    double *virial_atom, int project, double empty_norm, double empty_nn,
+                    const double *energy = nullptr, double power = 0);
 // fm_predict.cpp
 void predict(fm_model m, fm_structure s, int flags, int form, double *mean_efs,
              double *local_var, double *local_energy);

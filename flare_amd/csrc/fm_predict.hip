@@ -270,8 +270,12 @@ void predict(fm_model m, fm_structure s, int flags, int form, double *mean_efs,
   double *o = out.as<double>();
   FM_HIP(hipMemsetAsync(o, 0, (size_t)n_out * sizeof(double), st));
   if (n > 0) {
+    // FP64: G . B from the local energies (G . B = p |B| E, exact to
+    // rounding). Split form: from the G and B rows, since G carries the
+    // BF16-pair rounding of w that E does not, and the projection
+    // G - (G . B_hat) B_hat must see the G it is applied to.
     launch_adjoint(s, G.as<void>(), nd, fbond.as<void>(), vir.as<double>(), 1, kEmptyThresh,
-                   -1.0);
+                   -1.0, f64 ? E_row.as<double>() : nullptr, m->power);
     DevBuf blk(pool, (size_t)s->n_chunks * 7 * sizeof(double));
     if (f64)
       hipLaunchKernelGGL(k_gather<double>, dim3(s->n_chunks), dim3(256), 0, st,
