@@ -65,7 +65,7 @@ def build_model(C_, cell, species, pos, n_sparse, seed_sparse=3, seed_alpha=4):
 
 
 def run_form(lib, ctx, model_h, calc, cell, sp_dev, pos_dev, n, form, steps, warmup,
-             cutoff=6.0):
+             cutoff=6.0, compute="fm_structure_compute_b2"):
     from flare_amd import _lib
     settings, keep = calc._settings()
     cellv = _lib.f64(cell.reshape(-1))
@@ -79,7 +79,7 @@ def run_form(lib, ctx, model_h, calc, cell, sp_dev, pos_dev, n, form, steps, war
         h = C.c_void_p()
         _lib.check(lib.fm_structure_create_device(ctx.h, _lib.ptr(cellv), n, sp_dev, pos_dev,
                                                   cutoff, C.byref(h)))
-        _lib.check(lib.fm_structure_compute_b2(h, C.byref(settings), form))
+        _lib.check(getattr(lib, compute)(h, C.byref(settings), form))
         _lib.check(lib.fm_predict(model_h, h, flags, form, _lib.ptr(mean), _lib.ptr(var), None))
         lib.fm_structure_destroy(h)
 
@@ -229,6 +229,54 @@ def large_leg(lib, ctx, hip, steps, warmup):
     return out
 
 
+def large_b3_leg(lib, ctx, hip, steps, warmup):
+    """Config 5 (B3 kernel), at the largest atom count whose descriptors fit
+    whole in HBM: a 65,536-atom three-species BCC alloy (a = 3.2 A, 32^3
+    cells, seed 8); B3 [3, 6, 4] rc 5.5 (d = 74,100); NDP sigma 2, p 2; 4096
+    sparse environments: every atom of a 16 x 16 x 8-cell sample (seed 11).
+    The 1M-atom B3 structure needs 296 GB of FP32 descriptors at once; atom
+    tiles are not built, so it is not run."""
+    from flare_amd import _C_flare as CF
+    from flare_amd import _lib
+    from flare_amd.workloads import BCC_BASIS, random_alloy
+    cut = np.full((3, 3), 5.5)
+    calc = CF.B3("chebyshev", "quadratic", [0.0, 5.5], [], [3, 6, 4], cut)
+    cs, ss, ps = random_alloy((16, 16, 8), a=3.2, nos=3, sigma=0.1, seed=11, basis=BCC_BASIS)
+    small = CF.Structure(cs, ss, ps, 5.5, [calc])
+    sgp = CF.SparseGP([CF.NormalizedDotProduct(2.0, 2.0)], 0.01, 0.05, 0.005)
+    sgp.add_specific_environments(small, list(range(len(ss))))
+    del small
+    sgp.update_L_inverse()
+    sgp.set_alpha(np.random.default_rng(4).normal(scale=0.1, size=sgp.n_sparse))
+    cell, species, pos = random_alloy((32, 32, 32), a=3.2, nos=3, sigma=0.1, seed=8,
+                                      basis=BCC_BASIS)
+    n = len(pos)
+    pos_dev = hip.upload(np.ascontiguousarray(pos.reshape(-1)))
+    sp_dev = hip.upload(np.ascontiguousarray(species.astype(np.int32)))
+    hip.L.hipDeviceSynchronize()
+    model_h = sgp._model(0, ctx).h
+    out, res = {}, {}
+    for name, form in (("fp64", _lib.FORM_FP64), ("chosen", _lib.FORM_CHOSEN)):
+        log(f"  B3, {n} atoms, form {name} ...")
+        r = run_form(lib, ctx, model_h, calc, cell, sp_dev, pos_dev, n, form, steps, warmup,
+                     cutoff=5.5, compute="fm_structure_compute_b3")
+        res[name] = r
+        out[name] = {"atom_evals_per_s": n * steps / r["total"],
+                     "ms_per_step": r["total"] / steps * 1e3,
+                     "stage_ms": {k: round(v[0] / steps, 3) for k, v in r["stage"].items()}}
+    m64, mch = res["fp64"]["mean"], res["chosen"]["mean"]
+    out["workload"] = (f"{n} atoms, 3-species BCC alloy a = 3.2 A (32^3 cells); B3 [3,6,4] "
+                       f"rc 5.5 (d={calc.n_descriptors}); {sgp.n_sparse} sparse; NDP sigma 2 p 2")
+    out["speedup_vs_fp64"] = out["chosen"]["atom_evals_per_s"] / out["fp64"]["atom_evals_per_s"]
+    out["error_vs_fp64"] = {"energy_meV_per_atom": abs(m64[0] - mch[0]) / n * 1e3,
+                            "force_max_meV_per_A": float(np.abs(m64[1:-6] - mch[1:-6]).max()) * 1e3,
+                            **sigma_errors(res["fp64"]["var"], res["chosen"]["var"],
+                                           sgp.kernels[0].sigma)}
+    hip.L.hipFree(pos_dev)
+    hip.L.hipFree(sp_dev)
+    return out
+
+
 def serving_leg(sgp, calc, batched, n_requests=2000, n_fp64=200):
     """Config 4: Poisson stream of requests (sizes log-uniform 32-8192
     atoms, the batched model), served with continuous batching in the
@@ -333,10 +381,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--sparse", type=int, default=2000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workloads", default="all", choices=["single", "batched", "large", "all"],
+    ap.add_argument("--workloads", default="all",
+                    choices=["single", "batched", "large", "large_b3", "all"],
                     help="single: the 32k-atom structure only; all: also the batched, "
-                         "served and 1M-atom workloads; batched / large: that workload "
-                         "alone (its own JSON line, for profiling)")
+                         "served and 1M-atom B2 workloads; batched / large / large_b3: that "
+                         "workload alone (its own JSON line); large_b3 (B3 at 65,536 "
+                         "atoms) runs only when named")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a 512 MiB buffer three times (the known-size kernel "
                          "that FETCH_SIZE / WRITE_SIZE counters are scaled by)")
@@ -368,6 +418,10 @@ def main():
     if args.workloads == "large":
         log("1M-atom workload ...")
         print(json.dumps({"large_b2": large_leg(lib, ctx, Hip(), 2, 1)}), flush=True)
+        return
+    if args.workloads == "large_b3":
+        log("B3 workload ...")
+        print(json.dumps({"large_b3": large_b3_leg(lib, ctx, Hip(), 1, 1)}), flush=True)
         return
     cell, species, pos = fcc_alloy_32k()
     n = len(pos)

@@ -175,12 +175,12 @@ void Pool::put(void *p, size_t cls) {
 
 }  // namespace fm
 
-void fm_context_s::mark_begin(int s) {
-  (void)hipEventRecord(ev_begin[s], stream);
+void fm_context_s::mark_begin(int s, hipStream_t on) {
+  (void)hipEventRecord(ev_begin[s], on ? on : stream);
   launches[s] = 0;
 }
-void fm_context_s::mark_end(int s) {
-  (void)hipEventRecord(ev_end[s], stream);
+void fm_context_s::mark_end(int s, hipStream_t on) {
+  (void)hipEventRecord(ev_end[s], on ? on : stream);
   recorded[s] = true;
 }
 
@@ -195,6 +195,9 @@ int fm_context_create(int device, fm_context *out) {
     auto *c = new fm_context_s();
     c->device = device;
     FM_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    FM_HIP(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+    FM_HIP(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+    FM_HIP(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
     FM_HIP(hipHostMalloc(reinterpret_cast<void **>(&c->pinned),
                          fm_context_s::kPinnedWords * sizeof(int), hipHostMallocDefault));
     for (int s = 0; s < fm::ST_COUNT; s++) {
@@ -214,9 +217,13 @@ int fm_context_destroy(fm_context ctx) {
       (void)hipEventDestroy(ctx->ev_begin[s]);
       (void)hipEventDestroy(ctx->ev_end[s]);
     }
-    hipStream_t st = ctx->stream;
+    hipStream_t st = ctx->stream, side = ctx->side;
+    (void)hipStreamSynchronize(side);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    (void)hipEventDestroy(ctx->ev_fork);
+    (void)hipEventDestroy(ctx->ev_join);
     delete ctx;
+    (void)hipStreamDestroy(side);
     (void)hipStreamDestroy(st);
   });
 }

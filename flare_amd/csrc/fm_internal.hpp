@@ -117,11 +117,16 @@ struct fm_context_s {
   // Calls on one context are sequential (they share its stream).
   static constexpr int kPinnedWords = 64;
   int *pinned = nullptr;
+  // Second stream (default priority) for work that overlaps the main one:
+  // the variance contraction runs there while the gradient contraction and
+  // the adjoint run on `stream`. fork / join order them with events.
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   hipEvent_t ev_begin[fm::ST_COUNT], ev_end[fm::ST_COUNT];
   bool recorded[fm::ST_COUNT] = {};
   int launches[fm::ST_COUNT] = {};
-  void mark_begin(int s);
-  void mark_end(int s);
+  void mark_begin(int s, hipStream_t on = nullptr);
+  void mark_end(int s, hipStream_t on = nullptr);
 };
 
 struct fm_structure_s {

@@ -19,6 +19,20 @@ namespace fm {
 // tiles; var = sigma^2 (B.B / |B|^2)^p - sum of the |Q|^2 partials over row
 // tiles (the diagonal of envs_envs(cluster, cluster) minus diag(Q1^T Q1),
 // sparse_gp.cpp:116-136). Partials are summed in tile order.
+__device__ __forceinline__ double row_variance(int r, const double *Qpart, int n_qtiles, int n_s,
+                                               double nrm, double nnr, double sig2,
+                                               double power) {
+  double v = 0;
+  if (nrm >= kEmptyThresh) {
+    double nd = nnr / (nrm * nrm);
+    double ks = sig2 * (power == 2.0 ? nd * nd : pow(nd, power));
+    double q = 0;
+    for (int t = 0; t < n_qtiles; t++) q += Qpart[(size_t)t * n_s + r];
+    v = ks - q;
+  }
+  return v;
+}
+
 __global__ void k_species_final(int n_s, int ro, const double *Epart, int n_etiles,
                                 const double *Qpart, int n_qtiles, const double *norm,
                                 const double *nn, double sig2, double power, double *E_row,
@@ -28,18 +42,17 @@ __global__ void k_species_final(int n_s, int ro, const double *Epart, int n_etil
   double e = 0;
   for (int t = 0; t < n_etiles; t++) e += Epart[(size_t)t * n_s + r];
   E_row[ro + r] = e;
-  if (var_row) {
-    double nrm = norm[ro + r];
-    double v = 0;
-    if (nrm >= kEmptyThresh) {
-      double nd = nn[ro + r] / (nrm * nrm);
-      double ks = sig2 * (power == 2.0 ? nd * nd : pow(nd, power));
-      double q = 0;
-      for (int t = 0; t < n_qtiles; t++) q += Qpart[(size_t)t * n_s + r];
-      v = ks - q;
-    }
-    var_row[ro + r] = v;
-  }
+  if (var_row)
+    var_row[ro + r] = row_variance(r, Qpart, n_qtiles, n_s, norm[ro + r], nn[ro + r], sig2, power);
+}
+
+// The variance half alone (run on the side stream).
+__global__ void k_var_final(int n_s, int ro, const double *Qpart, int n_qtiles,
+                            const double *norm, const double *nn, double sig2, double power,
+                            double *var_row) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n_s) return;
+  var_row[ro + r] = row_variance(r, Qpart, n_qtiles, n_s, norm[ro + r], nn[ro + r], sig2, power);
 }
 
 // Forces: F_i = sum_{bonds b of i} f_b - sum_{bonds b with neighbour i} f_b
@@ -232,9 +245,36 @@ void predict(fm_model m, fm_structure s, int flags, int form, double *mean_efs,
   }
   ctx->mark_end(ST_GGEMM);
 
-  // Variance contraction Q = L_s^-1 K^T (FP64 in both forms).
-  ctx->mark_begin(ST_VAR);
+  // Local energies from the energy partials (main stream).
+  for (int t = 0; t < nos; t++) {
+    int ns = s->n_by_type[t], Ms = m->M_by_type[t], ro = s->type_off[t];
+    if (ns == 0) continue;
+    int n_et = Ms == 0 ? 1 : gemm_col_tiles(form, Ms);
+    hipLaunchKernelGGL(k_species_final, dim3((ns + 255) / 256), dim3(256), 0, st, ns, ro,
+                       Epart.as<double>() + eo[t], n_et, nullptr, 0, s->norm.as<double>(),
+                       s->nn.as<double>(), sig2, m->power, E_row.as<double>(), nullptr);
+  }
+  FM_HIP(hipGetLastError());
+
+  // Variance contraction Q = L_s^-1 K^T (FP64 in both forms) on the side
+  // stream, after the kernel contraction that wrote k; it overlaps the
+  // gradient contraction and the adjoint, and is joined before the results
+  // are read back.
+  // Joins the side stream into the main one when predict leaves this scope
+  // by any path (declared after the buffers the side stream uses, so it runs
+  // before they return to the pool).
+  struct SideJoin {
+    fm_context c;
+    bool active = false;
+    ~SideJoin() {
+      if (active) (void)hipStreamWaitEvent(c->stream, c->ev_join, 0);
+    }
+  } join{ctx};
   if (want_var) {
+    hipStream_t sd = ctx->side;
+    FM_HIP(hipEventRecord(ctx->ev_fork, st));
+    FM_HIP(hipStreamWaitEvent(sd, ctx->ev_fork, 0));
+    ctx->mark_begin(ST_VAR, sd);
     for (int t = 0; t < nos; t++) {
       int ns = s->n_by_type[t], Ms = m->M_by_type[t];
       if (ns == 0 || Ms == 0 || m->L_rows[t] == 0) continue;
@@ -242,26 +282,26 @@ void predict(fm_model m, fm_structure s, int flags, int form, double *mean_efs,
       eq.part = Qpart.as<double>() + qo[t];
       eq.part_ld = ns;
       if (f64)
-        gemm_f64(st, EP_SUMSQ, m->L_rows[t], ns, Ms, m->L[t].as<double>(), Ms,
+        gemm_f64(sd, EP_SUMSQ, m->L_rows[t], ns, Ms, m->L[t].as<double>(), Ms,
                  Kb.as<double>() + ko[t], (long long)kld[t], m->L_lower[t], eq);
       else
-        gemm_f64(st, EP_SUMSQ, m->L_rows[t], ns, Ms, m->L[t].as<double>(), Ms,
+        gemm_f64(sd, EP_SUMSQ, m->L_rows[t], ns, Ms, m->L[t].as<double>(), Ms,
                  Kb.as<float>() + ko[t], (long long)kld[t], m->L_lower[t], eq);
       ctx->launches[ST_VAR]++;
     }
+    for (int t = 0; t < nos; t++) {
+      int ns = s->n_by_type[t], Ms = m->M_by_type[t], ro = s->type_off[t];
+      if (ns == 0) continue;
+      int n_qt = Ms > 0 ? gemm_row_tiles(FM_FORM_FP64, m->L_rows[t]) : 0;
+      hipLaunchKernelGGL(k_var_final, dim3((ns + 255) / 256), dim3(256), 0, sd, ns, ro,
+                         Qpart.as<double>() + qo[t], n_qt, s->norm.as<double>(),
+                         s->nn.as<double>(), sig2, m->power, var_row.as<double>());
+    }
+    FM_HIP(hipGetLastError());
+    ctx->mark_end(ST_VAR, sd);
+    FM_HIP(hipEventRecord(ctx->ev_join, sd));
+    join.active = true;
   }
-  for (int t = 0; t < nos; t++) {
-    int ns = s->n_by_type[t], Ms = m->M_by_type[t], ro = s->type_off[t];
-    if (ns == 0) continue;
-    int n_et = Ms == 0 ? 1 : gemm_col_tiles(form, Ms);
-    int n_qt = (want_var && Ms > 0) ? gemm_row_tiles(FM_FORM_FP64, m->L_rows[t]) : 0;
-    hipLaunchKernelGGL(k_species_final, dim3((ns + 255) / 256), dim3(256), 0, st, ns, ro,
-                       Epart.as<double>() + eo[t], n_et, Qpart.as<double>() + qo[t], n_qt,
-                       s->norm.as<double>(), s->nn.as<double>(), sig2, m->power,
-                       E_row.as<double>(), want_var ? var_row.as<double>() : nullptr);
-  }
-  FM_HIP(hipGetLastError());
-  ctx->mark_end(ST_VAR);
 
   // Adjoint bond forces and the deterministic gather.
   ctx->mark_begin(ST_ADJ);
@@ -298,6 +338,10 @@ void predict(fm_model m, fm_structure s, int flags, int form, double *mean_efs,
   }
   FM_HIP(hipGetLastError());
   ctx->mark_end(ST_ADJ);
+  if (join.active) {
+    FM_HIP(hipStreamWaitEvent(st, ctx->ev_join, 0));
+    join.active = false;
+  }
 
   if (mean_efs)
     FM_HIP(hipMemcpyAsync(mean_efs, o, (size_t)n_out * sizeof(double), hipMemcpyDeviceToHost, st));
