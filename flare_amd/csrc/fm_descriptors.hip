@@ -44,60 +44,89 @@ __constant__ double c_rcp[kMaxL + 2];
 // from the upward recursion in l. Calls f(lm, Y, px, py, pz); with kGrad,
 // (px, py, pz) is the gradient in u of the degree-0 extension
 // (dQ_l^m/du_z = Q_l^(m+1)), so grad Y_lm(r) = (p - (u . p) u) / r.
-template <typename T, bool kGrad, typename F>
-__device__ __forceinline__ void ylm_sweep(int L, T ux, T uy, T uz, F &&f) {
+// With LC >= 0 the degree is the compile-time LC (L is ignored) and the
+// sweep is fully unrolled: every index and constant-table offset is known
+// at compile time.
+template <typename T, bool kGrad>
+struct YlmState {
   T cm = 1, sm = 0, cm1 = 0, sm1 = 0;  // (u_x + i u_y)^mm and ^(mm - 1)
   T dfact = 1;                         // (2 mm - 1)!!
-  for (int mm = 0; mm <= L; mm++) {
-    if (mm > 0) {
-      cm1 = cm;
-      sm1 = sm;
-      cm = cm1 * ux - sm1 * uy;
-      sm = sm1 * ux + cm1 * uy;
-      dfact *= (T)(2 * mm - 1);
+  T q1 = 0, q2 = 0, p1 = 0, p2 = 0;
+};
+
+template <typename T, bool kGrad>
+__device__ __forceinline__ void ylm_m_begin(YlmState<T, kGrad> &y, int mm, T ux, T uy) {
+  if (mm > 0) {
+    y.cm1 = y.cm;
+    y.sm1 = y.sm;
+    y.cm = y.cm1 * ux - y.sm1 * uy;
+    y.sm = y.sm1 * ux + y.cm1 * uy;
+    y.dfact *= (T)(2 * mm - 1);
+  }
+  y.q1 = y.dfact;
+  y.q2 = 0;
+  y.p1 = 0;
+  y.p2 = 0;
+}
+
+template <typename T, bool kGrad, typename F>
+__device__ __forceinline__ void ylm_step(YlmState<T, kGrad> &y, int mm, int l, T uz, F &f) {
+  T ql, pl = 0;
+  if (l == mm) {
+    ql = y.dfact;
+  } else {
+    ql = ((T)(2 * l - 1) * uz * y.q1 - (T)(l + mm - 1) * y.q2) * (T)c_rcp[l - mm];
+    y.q2 = y.q1;
+    y.q1 = ql;
+    if (kGrad) {
+      if (l == mm + 1)
+        pl = y.dfact * (T)(2 * mm + 1);
+      else
+        pl = ((T)(2 * l - 1) * uz * y.p1 - (T)(l + mm) * y.p2) * (T)c_rcp[l - mm - 1];
+      y.p2 = y.p1;
+      y.p1 = pl;
     }
-    T q1 = dfact, q2 = 0, p1 = 0, p2 = 0;
-    for (int l = mm; l <= L; l++) {
-      T ql, pl = 0;
-      if (l == mm) {
-        ql = dfact;
-      } else {
-        ql = ((T)(2 * l - 1) * uz * q1 - (T)(l + mm - 1) * q2) * (T)c_rcp[l - mm];
-        q2 = q1;
-        q1 = ql;
-        if (kGrad) {
-          if (l == mm + 1)
-            pl = dfact * (T)(2 * mm + 1);
-          else
-            pl = ((T)(2 * l - 1) * uz * p1 - (T)(l + mm) * p2) * (T)c_rcp[l - mm - 1];
-          p2 = p1;
-          p1 = pl;
-        }
-      }
-      int base = l * l + l;
+  }
+  int base = l * l + l;
 #pragma unroll
-      for (int sgn = 0; sgn < 2; sgn++) {
-        if (sgn == 1 && mm == 0) break;
-        int lm = sgn ? base - mm : base + mm;
-        T K = (T)c_knorm[lm];
-        T trig, dtx = 0, dty = 0;
-        if (mm == 0) {
-          trig = 1;
-        } else if (sgn == 0) {
-          trig = cm;
-          dtx = (T)mm * cm1;
-          dty = -(T)mm * sm1;
-        } else {
-          trig = sm;
-          dtx = (T)mm * sm1;
-          dty = (T)mm * cm1;
-        }
-        T Kq = K * ql;
-        if (kGrad)
-          f(lm, Kq * trig, Kq * dtx, Kq * dty, K * pl * trig);
-        else
-          f(lm, Kq * trig, T(0), T(0), T(0));
-      }
+  for (int sgn = 0; sgn < 2; sgn++) {
+    if (sgn == 1 && mm == 0) break;
+    int lm = sgn ? base - mm : base + mm;
+    T K = (T)c_knorm[lm];
+    T trig, dtx = 0, dty = 0;
+    if (mm == 0) {
+      trig = 1;
+    } else if (sgn == 0) {
+      trig = y.cm;
+      dtx = (T)mm * y.cm1;
+      dty = -(T)mm * y.sm1;
+    } else {
+      trig = y.sm;
+      dtx = (T)mm * y.sm1;
+      dty = (T)mm * y.cm1;
+    }
+    T Kq = K * ql;
+    if (kGrad)
+      f(lm, Kq * trig, Kq * dtx, Kq * dty, K * pl * trig);
+    else
+      f(lm, Kq * trig, T(0), T(0), T(0));
+  }
+}
+
+template <typename T, bool kGrad, int LC = -1, typename F>
+__device__ __forceinline__ void ylm_sweep(int L, T ux, T uy, T uz, F &&f) {
+  YlmState<T, kGrad> y;
+  if constexpr (LC >= 0) {
+#pragma unroll
+    for (int mm = 0; mm <= LC; mm++) {
+      ylm_m_begin<T, kGrad>(y, mm, ux, uy);
+#pragma unroll
+      for (int l = mm; l <= LC; l++) ylm_step<T, kGrad>(y, mm, l, uz, f);
+    }
+  } else {
+    for (int mm = 0; mm <= L; mm++) {
+      ylm_m_begin<T, kGrad>(y, mm, ux, uy);
+      for (int l = mm; l <= L; l++) ylm_step<T, kGrad>(y, mm, l, uz, f);
     }
   }
 }
@@ -146,7 +175,7 @@ __host__ __device__ inline size_t b2_work(int nc, int H, int N) {
 // rows grouped by neighbour species (list order kept within a species); then,
 // per species t and 16 harmonics, c_t += G_t^T Y_t on the matrix cores over
 // that species' rows only (A = g_n of the bonds, B = their Y_lm).
-template <typename T>
+template <typename T, int LC>
 __device__ void accumulate_c(const B2Args &a, int i, int lane, T *c, T *tab) {
   int si = a.species[i];
   int b0 = a.bond_off[i], b1 = a.bond_off[i + 1];
@@ -182,7 +211,7 @@ __device__ void accumulate_c(const B2Args &a, int i, int lane, T *c, T *tab) {
         if (nn < N) gt[pos * N + nn] = g[nn];
       T rinv = T(1) / (T)rp[0];
       T *yrow = Yt + (size_t)pos * H;
-      ylm_sweep<T, false>(a.lmax, (T)rp[1] * rinv, (T)rp[2] * rinv, (T)rp[3] * rinv,
+      ylm_sweep<T, false, LC>(a.lmax, (T)rp[1] * rinv, (T)rp[2] * rinv, (T)rp[3] * rinv,
                           [&](int lm, T Y, T, T, T) { yrow[lm] = Y; });
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -241,7 +270,7 @@ struct SplitOut {
   __bf16 *p[2];
 };
 
-template <typename T>
+template <typename T, int LC>
 __global__ __launch_bounds__(64 * kAtomsPerBlock) void k_b2(B2Args a, T *B, double *norm,
                                                               double *nn, SplitOut Bs, int ldp,
                                                               T *Cg, int want_b2, int work) {
@@ -252,7 +281,7 @@ __global__ __launch_bounds__(64 * kAtomsPerBlock) void k_b2(B2Args a, T *B, doub
   int nc = a.nrad * a.H, H = a.H, nr = a.nrad;
   T *c = reinterpret_cast<T *>(smem) + (size_t)w * work;  // work: elements per atom
   T *tab = c + nc;
-  accumulate_c<T>(a, i, lane, c, tab);
+  accumulate_c<T, LC>(a, i, lane, c, tab);
   for (int k = lane; k < nc; k += 64) Cg[(size_t)i * nc + k] = c[k];
   if (!want_b2) return;
   // B_l = C_l C_l^T on the matrix cores (C_l[n][m] = c[n, l^2 + m]); the
@@ -506,7 +535,7 @@ __device__ __forceinline__ void load4_lds(const double *p, double (&o)[4]) {
 //   a = sum_lm (sum_n u dg_n) Y_lm,  P = sum_lm (sum_n u g_n) grad_u Y_lm,
 // with grad_u the gradient of the degree-0 extension in u. Writes f for every
 // bond and the atom's virial -sum_b f (x) r_b (xx, xy, xz, yy, yz, zz).
-template <typename T>
+template <typename T, int LC>
 __global__ __launch_bounds__(64 * kAtomsPerBlock) void k_bond_forces(B2Args a,
                                                                        const double *norm,
                                                                        const double *nnv,
@@ -558,7 +587,7 @@ __global__ __launch_bounds__(64 * kAtomsPerBlock) void k_bond_forces(B2Args a,
       T ux = x * rinv, uy = y * rinv, uz = z * rinv;
       const T *ub = u + (size_t)sj * H * Np;
       T acc_a = 0, Px = 0, Py = 0, Pz = 0;
-      ylm_sweep<T, true>(L, ux, uy, uz, [&](int lm, T Y, T px, T py, T pz) {
+      ylm_sweep<T, true, LC>(L, ux, uy, uz, [&](int lm, T Y, T px, T py, T pz) {
         T s0 = 0, s1 = 0;
         const T *ul = ub + (size_t)lm * Np;
 #pragma unroll
@@ -615,6 +644,19 @@ static int atoms_per_block(size_t lds_per_atom) {
     }
   }
   return best;
+}
+
+// Calls f(std::integral_constant<int, L>) for the l_max values with an
+// unrolled harmonic sweep (those of the common B2 / B3 settings), else with
+// L = -1 (the sweep loops over the run-time l_max).
+template <typename F>
+static void with_lmax(int lmax, F &&f) {
+  switch (lmax) {
+    case 3: f(std::integral_constant<int, 3>{}); break;
+    case 4: f(std::integral_constant<int, 4>{}); break;
+    case 6: f(std::integral_constant<int, 6>{}); break;
+    default: f(std::integral_constant<int, -1>{}); break;
+  }
 }
 
 static B2Args make_args(fm_structure s) {
@@ -686,13 +728,18 @@ void launch_b2(fm_structure s, bool want_b2) {
   if (nblk == 0) return;
   SplitOut none{{nullptr, nullptr}};
   SplitOut pieces{{s->Bs[0].as<__bf16>(), s->Bs[1].as<__bf16>()}};
+  // The unrolled sweep (compile-time l_max) is faster for the FP64 kernel;
+  // for FP32 its extra registers cost more occupancy than it saves.
   if (s->form == FM_FORM_FP64)
-    hipLaunchKernelGGL(k_b2<double>, dim3(nblk), dim3(64 * apb), lds, st, a, s->B.as<double>(),
-                       s->norm.as<double>(), s->nn.as<double>(), none, 0, s->C.as<double>(),
-                       (int)want_b2, (int)work);
+    with_lmax(a.lmax, [&](auto lc) {
+      constexpr int LC = decltype(lc)::value;
+      hipLaunchKernelGGL((k_b2<double, LC>), dim3(nblk), dim3(64 * apb), lds, st, a,
+                         s->B.as<double>(), s->norm.as<double>(), s->nn.as<double>(), none, 0,
+                         s->C.as<double>(), (int)want_b2, (int)work);
+    });
   else
-    hipLaunchKernelGGL(k_b2<float>, dim3(nblk), dim3(64 * apb), lds, st, a, s->B.as<float>(),
-                       s->norm.as<double>(), s->nn.as<double>(), pieces,
+    hipLaunchKernelGGL((k_b2<float, -1>), dim3(nblk), dim3(64 * apb), lds, st, a,
+                       s->B.as<float>(), s->norm.as<double>(), s->nn.as<double>(), pieces,
                        (int)split_ld(s->nd), s->C.as<float>(), (int)want_b2, (int)work);
   FM_HIP(hipGetLastError());
 }
@@ -728,8 +775,8 @@ void launch_adjoint(fm_structure s, const void *G, int64_t g_ld, void *fbond,
                          s->B.as<double>(), s->norm.as<double>(), s->nn.as<double>(),
                          (const double *)G, (long long)g_ld, aa, s->C.as<double>(),
                          U.as<double>());
-    hipLaunchKernelGGL(k_bond_forces<double>, dim3(nblk2), dim3(64 * apb2), apb2 * per2, st, a,
-                       s->norm.as<double>(), s->nn.as<double>(), aa, U.as<double>(),
+    hipLaunchKernelGGL((k_bond_forces<double, -1>), dim3(nblk2), dim3(64 * apb2), apb2 * per2,
+                       st, a, s->norm.as<double>(), s->nn.as<double>(), aa, U.as<double>(),
                        (double *)fbond, virial_atom);
   } else {
     if (s->desc_kind == 2)
@@ -737,8 +784,8 @@ void launch_adjoint(fm_structure s, const void *G, int64_t g_ld, void *fbond,
                          s->B.as<float>(), s->norm.as<double>(), s->nn.as<double>(),
                          (const float *)G, (long long)g_ld, aa, s->C.as<float>(),
                          U.as<float>());
-    hipLaunchKernelGGL(k_bond_forces<float>, dim3(nblk2), dim3(64 * apb2), apb2 * per2, st, a,
-                       s->norm.as<double>(), s->nn.as<double>(), aa, U.as<float>(),
+    hipLaunchKernelGGL((k_bond_forces<float, -1>), dim3(nblk2), dim3(64 * apb2), apb2 * per2,
+                       st, a, s->norm.as<double>(), s->nn.as<double>(), aa, U.as<float>(),
                        (float *)fbond, virial_atom);
   }
   FM_HIP(hipGetLastError());
