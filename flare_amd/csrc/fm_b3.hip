@@ -68,10 +68,12 @@ struct B3Tables {
   std::vector<int> term_off;   // n_ls + 1
   std::vector<int> term_lm;    // n_terms x 3 (lm of each factor)
   std::vector<double> term_w;  // n_terms
-  // Gradient lists: for each (lm, position p): the (ls, term) pairs whose
-  // factor p has that lm. Entries grouped by lm * 3 + p.
-  std::vector<int> grad_off;   // 3 H + 1
-  std::vector<int> grad_ls, grad_term;
+  // Gradient lists: for each l-set ls, harmonic lm and position p, the
+  // terms of ls whose factor p has that lm; entries grouped by
+  // (ls H + lm) 3 + p, terms in increasing order.
+  std::vector<int> gls_off;    // n_ls 3 H + 1
+  std::vector<int> gls_term;
+  std::vector<int> tri_pk;     // n_tri: n1 | n2 << 8 | n3 << 16
 };
 
 static void build_b3_tables(int nr, int lmax, B3Tables &t) {
@@ -102,17 +104,17 @@ static void build_b3_tables(int nr, int lmax, B3Tables &t) {
   }
   t.n_tri = (int)t.tri.size() / 3;
   t.nd = t.n_tri * t.n_ls;
-  std::vector<std::vector<std::pair<int, int>>> lists(3 * t.H);
-  for (int ls = 0; ls < t.n_ls; ls++)
+  for (int k = 0; k < t.n_tri; k++)
+    t.tri_pk.push_back(t.tri[3 * k] | t.tri[3 * k + 1] << 8 | t.tri[3 * k + 2] << 16);
+  t.gls_off.push_back(0);
+  for (int ls = 0; ls < t.n_ls; ls++) {
+    std::vector<std::vector<int>> lists(3 * t.H);
     for (int q = t.term_off[ls]; q < t.term_off[ls + 1]; q++)
-      for (int p = 0; p < 3; p++) lists[t.term_lm[3 * q + p] * 3 + p].push_back({ls, q});
-  t.grad_off.push_back(0);
-  for (auto &v : lists) {
-    for (auto &e : v) {
-      t.grad_ls.push_back(e.first);
-      t.grad_term.push_back(e.second);
+      for (int p = 0; p < 3; p++) lists[t.term_lm[3 * q + p] * 3 + p].push_back(q);
+    for (auto &v : lists) {
+      t.gls_term.insert(t.gls_term.end(), v.begin(), v.end());
+      t.gls_off.push_back((int)t.gls_term.size());
     }
-    t.grad_off.push_back((int)t.grad_ls.size());
   }
 }
 
@@ -123,8 +125,17 @@ int b3_size(int nos, int N, int lmax) {
 }
 
 // Device copies of the tables.
+//
+// Order of the descriptor on the device: l-set major, k' = ls n_tri + tri
+// (the reference's index is k = tri n_ls + ls, b3.cpp:230-242). Dot
+// products do not depend on the order, so the contractions run on rows and
+// sparse sets stored this way (the model's sparse set is permuted once, on
+// its first use with a B3 structure) and the descriptor read-back restores
+// the reference's order. With the l-set outermost, the forward kernel's
+// lanes share one term list, and the adjoint reads one contiguous slice of
+// dE/dB per l-set.
 struct B3Dev {
-  const int *tri, *ls_l, *term_off, *term_lm, *grad_off, *grad_ls, *grad_term, *tri_base;
+  const int *tri, *ls_l, *term_off, *term_lm, *gls_off, *gls_term, *tri_base, *tri_pk;
   const double *term_w;
   int nr, H, n_ls, n_tri, nd;
 };
@@ -157,7 +168,9 @@ __device__ void complex_c(const T *cr, T *cc, int nr, int H, int lane) {
   }
 }
 
-// B3 per atom: one wavefront, complex c in LDS, lanes over outputs.
+// B3 per atom: one wavefront, complex c in LDS. For each l-set (uniform
+// across the wave, so its term list is read once per wave), lanes take the
+// radial triples and write the row in l-set-major order.
 template <typename T>
 __global__ __launch_bounds__(256) void k_b3(int n, B3Dev t, const T *Cg, const int *row_of_atom,
                                              T *B, double *norm, double *nnv, __bf16 *B0,
@@ -166,7 +179,7 @@ __global__ __launch_bounds__(256) void k_b3(int n, B3Dev t, const T *Cg, const i
   int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   int i = blockIdx.x * (blockDim.x >> 6) + w;
   if (i >= n) return;
-  int nc = t.nr * t.H;
+  int nc = t.nr * t.H, H = t.H;
   T *cc = reinterpret_cast<T *>(smem) + (size_t)w * 2 * nc;
   complex_c<T>(Cg + (size_t)i * nc, cc, t.nr, t.H, lane);
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -175,26 +188,30 @@ __global__ __launch_bounds__(256) void k_b3(int n, B3Dev t, const T *Cg, const i
   int row = row_of_atom[i];
   T *Brow = B + (size_t)row * t.nd;
   T s2 = 0;
-  for (int k = lane; k < t.nd; k += 64) {
-    int tr = k / t.n_ls, ls = k - tr * t.n_ls;
-    int n1 = t.tri[3 * tr], n2 = t.tri[3 * tr + 1], n3 = t.tri[3 * tr + 2];
-    const T *c1 = cc + 2 * n1 * t.H, *c2 = cc + 2 * n2 * t.H, *c3 = cc + 2 * n3 * t.H;
-    T acc = 0;
-    for (int q = t.term_off[ls]; q < t.term_off[ls + 1]; q++) {
-      int a1 = t.term_lm[3 * q], a2 = t.term_lm[3 * q + 1], a3 = t.term_lm[3 * q + 2];
-      T xr = c1[2 * a1], xi = c1[2 * a1 + 1], yr = c2[2 * a2], yi = c2[2 * a2 + 1];
-      T zr = c3[2 * a3], zi = c3[2 * a3 + 1];
-      T pr = xr * yr - xi * yi, pi = xr * yi + xi * yr;
-      acc += (T)t.term_w[q] * (pr * zr - pi * zi);
+  for (int ls = 0; ls < t.n_ls; ls++) {
+    int q0 = t.term_off[ls], q1 = t.term_off[ls + 1];
+    for (int tr = lane; tr < t.n_tri; tr += 64) {
+      int pk = t.tri_pk[tr];
+      const T *c1 = cc + 2 * (pk & 255) * H, *c2 = cc + 2 * ((pk >> 8) & 255) * H;
+      const T *c3 = cc + 2 * (pk >> 16) * H;
+      T acc = 0;
+      for (int q = q0; q < q1; q++) {
+        int a1 = t.term_lm[3 * q], a2 = t.term_lm[3 * q + 1], a3 = t.term_lm[3 * q + 2];
+        T xr = c1[2 * a1], xi = c1[2 * a1 + 1], yr = c2[2 * a2], yi = c2[2 * a2 + 1];
+        T zr = c3[2 * a3], zi = c3[2 * a3 + 1];
+        T pr = xr * yr - xi * yi, pi = xr * yi + xi * yr;
+        acc += (T)t.term_w[q] * (pr * zr - pi * zi);
+      }
+      int k = ls * t.n_tri + tr;
+      Brow[k] = acc;
+      if (B0) {
+        float r = (float)acc;
+        __bf16 h = (__bf16)r;
+        B0[(size_t)row * ldp + k] = h;
+        B1[(size_t)row * ldp + k] = (__bf16)(r - (float)h);
+      }
+      s2 += acc * acc;
     }
-    Brow[k] = acc;
-    if (B0) {
-      float r = (float)acc;
-      __bf16 h = (__bf16)r;
-      B0[(size_t)row * ldp + k] = h;
-      B1[(size_t)row * ldp + k] = (__bf16)(r - (float)h);
-    }
-    s2 += acc * acc;
   }
   if (B0)
     for (int k = t.nd + lane; k < ldp; k += 64) {
@@ -213,12 +230,14 @@ __device__ __forceinline__ int tri_index(const B3Dev &t, int n1, int n2, int n3)
   return t.tri_base[n1] + (n2 - n1) * nr - (n2 * (n2 - 1) - n1 * (n1 - 1)) / 2 + (n3 - n2);
 }
 
-// dE/dc for B3 per atom. With `project`, w_k = (G_k - (G . B_hat) B_k /|B|)
-// / |B| (as for B2; read from HBM, the row is too long for LDS). Lanes over
-// the targets (n, lm): the complex gradient sums, over each factor position
-// p holding (n, lm), w_k w3j conj(product of the other two factors); the
-// real gradient u = dE/dc^r follows by the transpose of the complex map.
-// Each target's sum runs in a fixed order, so results are deterministic.
+// dE/dc for B3 per atom. With `project`, w_k = (G_k - (G . B_hat) B_k / |B|)
+// / |B| (as for B2). For each l-set the slice w[ls n_tri, (ls + 1) n_tri)
+// is formed from contiguous G and B reads into LDS; lanes then take the
+// targets (n, lm) and add, over each factor position p holding (n, lm) and
+// the terms of that l-set, w w3j conj(product of the other two factors)
+// (radial triples enumerated in a fixed order, so results are
+// deterministic). The real gradient u = dE/dc^r follows by the transpose of
+// the complex map.
 template <typename T>
 __global__ __launch_bounds__(256) void k_dEdc_b3(int n, B3Dev t, const T *Cg,
                                                   const int *row_of_atom, const T *B,
@@ -228,64 +247,73 @@ __global__ __launch_bounds__(256) void k_dEdc_b3(int n, B3Dev t, const T *Cg,
   extern __shared__ __align__(16) unsigned char smem[];
   int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   int i = blockIdx.x * (blockDim.x >> 6) + w;
+  int nc = t.nr * t.H, nr = t.nr, H = t.H;
+  int per = 4 * nc + t.n_tri;
+  // Block-shared: index of the first triple of each n1.
+  int *tb = reinterpret_cast<int *>(smem);
+  for (int k = threadIdx.x; k < nr; k += blockDim.x) tb[k] = t.tri_base[k];
+  __syncthreads();
   if (i >= n) return;
-  int nc = t.nr * t.H;
-  T *cc = reinterpret_cast<T *>(smem) + (size_t)w * 4 * nc;
-  T *gc = cc + 2 * nc;  // complex gradient
+  T *cc = reinterpret_cast<T *>(smem + 256) + (size_t)w * per;
+  T *gc = cc + 2 * nc;  // complex gradient, one (re, im) per target
+  T *wl = gc + 2 * nc;  // w slice of one l-set
   int row = row_of_atom[i];
   double nrm = norm[row];
   if (nrm < empty_norm || nnv[row] < empty_nn) return;
   complex_c<T>(Cg + (size_t)i * nc, cc, t.nr, t.H, lane);
+  for (int k = lane; k < 2 * nc; k += 64) gc[k] = 0;
   const T *Brow = B + (size_t)row * t.nd;
   const T *Grow = G + (size_t)row * g_ld;
   T ninv = T(1) / (T)nrm, gb = 0;
   if (project) {
+#pragma unroll 8
     for (int k = lane; k < t.nd; k += 64) gb += Grow[k] * Brow[k];
     gb = wave_sum(gb) * ninv;
   }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  auto wk = [&](int k) -> T {
-    return project ? (Grow[k] - gb * Brow[k] * ninv) * ninv : Grow[k];
-  };
-  int nr = t.nr, H = t.H;
-  for (int g = lane; g < nc; g += 64) {
-    int nn = g / H, lm = g - nn * H;
-    T ar = 0, ai = 0;
-    for (int p = 0; p < 3; p++) {
-      int e0 = t.grad_off[lm * 3 + p], e1 = t.grad_off[lm * 3 + p + 1];
-      for (int e = e0; e < e1; e++) {
-        int ls = t.grad_ls[e], q = t.grad_term[e];
-        T wq = (T)t.term_w[q];
-        int a[3] = {t.term_lm[3 * q], t.term_lm[3 * q + 1], t.term_lm[3 * q + 2]};
-        // The two other factors' lm.
-        int oa = p == 0 ? a[1] : a[0], ob = p == 2 ? a[1] : a[2];
-        // Triples with factor p = nn: (nn, x, y), (x, nn, y) or (x, y, nn).
-        int lo_x = p == 0 ? nn : 0, hi_x = p == 0 ? nr - 1 : nn;
-        for (int x = lo_x; x <= hi_x; x++) {
-          int lo_y = p == 0 ? x : (p == 1 ? nn : x);
-          int hi_y = p == 2 ? nn : nr - 1;
-          for (int y = lo_y; y <= hi_y; y++) {
-            int n1, n2, n3, fa, fb;  // fa, fb: radial index of the other factors
-            if (p == 0) {
-              n1 = nn; n2 = x; n3 = y; fa = x; fb = y;
-            } else if (p == 1) {
-              n1 = x; n2 = nn; n3 = y; fa = x; fb = y;
-            } else {
-              n1 = x; n2 = y; n3 = nn; fa = x; fb = y;
+  for (int ls = 0; ls < t.n_ls; ls++) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const T *gs = Grow + (size_t)ls * t.n_tri, *bs = Brow + (size_t)ls * t.n_tri;
+    for (int k = lane; k < t.n_tri; k += 64)
+      wl[k] = project ? (gs[k] - gb * bs[k] * ninv) * ninv : gs[k];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int g = lane; g < nc; g += 64) {
+      int nn = g / H, lm = g - nn * H;
+      T xr = 0, xi = 0;
+      for (int p = 0; p < 3; p++) {
+        int base = ((ls * H + lm) * 3 + p);
+        int e0 = t.gls_off[base], e1 = t.gls_off[base + 1];
+        for (int e = e0; e < e1; e++) {
+          int q = t.gls_term[e];
+          T wq = (T)t.term_w[q];
+          int a0 = t.term_lm[3 * q], a1 = t.term_lm[3 * q + 1], a2 = t.term_lm[3 * q + 2];
+          // The two other factors' lm.
+          int oa = p == 0 ? a1 : a0, ob = p == 2 ? a1 : a2;
+          // Triples with factor p = nn: (nn, x, y), (x, nn, y) or (x, y, nn).
+          int lo_x = p == 0 ? nn : 0, hi_x = p == 0 ? nr - 1 : nn;
+          for (int x = lo_x; x <= hi_x; x++) {
+            int lo_y = p == 0 ? x : (p == 1 ? nn : x);
+            int hi_y = p == 2 ? nn : nr - 1;
+            const T *u = cc + 2 * (x * H + oa);
+            for (int y = lo_y; y <= hi_y; y++) {
+              int n1 = p == 0 ? nn : x, n2 = p == 0 ? x : (p == 1 ? nn : y);
+              int n3 = p == 2 ? nn : y;
+              int tri = tb[n1] + (n2 - n1) * nr - (n2 * (n2 - 1) - n1 * (n1 - 1)) / 2 + (n3 - n2);
+              T wkv = wl[tri] * wq;
+              const T *v = cc + 2 * (y * H + ob);
+              T pr = u[0] * v[0] - u[1] * v[1], pi = u[0] * v[1] + u[1] * v[0];
+              xr += wkv * pr;
+              xi -= wkv * pi;
             }
-            T wkv = wk(tri_index(t, n1, n2, n3) * t.n_ls + ls) * wq;
-            const T *u = cc + 2 * (fa * H + oa), *v = cc + 2 * (fb * H + ob);
-            T pr = u[0] * v[0] - u[1] * v[1], pi = u[0] * v[1] + u[1] * v[0];
-            ar += wkv * pr;
-            ai -= wkv * pi;
           }
         }
       }
+      gc[2 * g] += xr;
+      gc[2 * g + 1] += xi;
     }
-    gc[2 * g] = ar;
-    gc[2 * g + 1] = ai;
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -314,16 +342,48 @@ __global__ __launch_bounds__(256) void k_dEdc_b3(int n, B3Dev t, const T *Cg,
   }
 }
 
+// Column / row permutations between the reference's B3 order and the
+// l-set-major one: out[r][k'] = in[r][src(k')] for k' < nd (columns past nd,
+// the padding, are copied), or out row k' = in row src(k'), with
+// src(ls n_tri + tri) = tri n_ls + ls (to_device) or its inverse.
+__device__ __forceinline__ long long b3_src(long long kp, int n_tri, int n_ls, bool to_device) {
+  if (to_device) {
+    long long ls = kp / n_tri, tri = kp - ls * n_tri;
+    return tri * n_ls + ls;
+  }
+  long long tri = kp / n_ls, ls = kp - tri * n_ls;
+  return ls * n_tri + tri;
+}
+
+template <typename E>
+__global__ void k_b3_permute_cols(long long rows, long long ld, int nd, int n_tri, int n_ls,
+                                  int to_device, const E *in, E *out) {
+  long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= rows * ld) return;
+  long long r = idx / ld, k = idx - r * ld;
+  long long src = k < nd ? b3_src(k, n_tri, n_ls, to_device) : k;
+  out[idx] = in[r * ld + src];
+}
+
+template <typename E>
+__global__ void k_b3_permute_rows(int nd, long long ld, int n_tri, int n_ls, int to_device,
+                                  const E *in, E *out) {
+  long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= (long long)nd * ld) return;
+  long long k = idx / ld, c = idx - k * ld;
+  out[idx] = in[b3_src(k, n_tri, n_ls, to_device) * ld + c];
+}
+
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
 struct B3State {
   B3Tables t;
-  DevBuf tri, tri_base, ls_l, term_off, term_lm, term_w, grad_off, grad_ls, grad_term;
+  DevBuf tri, tri_base, ls_l, term_off, term_lm, term_w, gls_off, gls_term, tri_pk;
   B3Dev dev() const {
-    return B3Dev{tri.as<int>(),      ls_l.as<int>(),    term_off.as<int>(),
-                 term_lm.as<int>(),  grad_off.as<int>(), grad_ls.as<int>(),
-                 grad_term.as<int>(), tri_base.as<int>(), term_w.as<double>(),
+    return B3Dev{tri.as<int>(),      ls_l.as<int>(),     term_off.as<int>(),
+                 term_lm.as<int>(),  gls_off.as<int>(),  gls_term.as<int>(),
+                 tri_base.as<int>(), tri_pk.as<int>(),   term_w.as<double>(),
                  t.nr, t.H, t.n_ls, t.n_tri, t.nd};
   }
 };
@@ -345,13 +405,85 @@ static B3State *b3_state(fm_structure s) {
     up(st->term_off, t.term_off.data(), t.term_off.size() * 4);
     up(st->term_lm, t.term_lm.data(), t.term_lm.size() * 4);
     up(st->term_w, t.term_w.data(), t.term_w.size() * 8);
-    up(st->grad_off, t.grad_off.data(), t.grad_off.size() * 4);
-    up(st->grad_ls, t.grad_ls.data(), t.grad_ls.size() * 4);
-    up(st->grad_term, t.grad_term.data(), t.grad_term.size() * 4);
+    up(st->gls_off, t.gls_off.data(), t.gls_off.size() * 4);
+    up(st->gls_term, t.gls_term.data(), t.gls_term.size() * 4);
+    up(st->tri_pk, t.tri_pk.data(), t.tri_pk.size() * 4);
     FM_HIP(hipStreamSynchronize(stream));
     s->b3 = st;
   }
   return s->b3;
+}
+
+namespace {
+
+template <typename E>
+void permute_cols(hipStream_t st, DevBuf &buf, Pool *pool, long long rows, long long ld, int nd,
+                  int n_tri, int n_ls, int to_device) {
+  long long total = rows * ld;
+  if (total == 0) return;
+  DevBuf out(pool, (size_t)total * sizeof(E));
+  hipLaunchKernelGGL(k_b3_permute_cols<E>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                     st, rows, ld, nd, n_tri, n_ls, to_device, buf.as<E>(), out.as<E>());
+  FM_HIP(hipGetLastError());
+  std::swap(buf, out);
+}
+
+template <typename E>
+void permute_rows(hipStream_t st, DevBuf &buf, Pool *pool, int nd, long long ld, int n_tri,
+                  int n_ls, int to_device) {
+  long long total = (long long)nd * ld;
+  if (total == 0) return;
+  DevBuf out(pool, (size_t)total * sizeof(E));
+  hipLaunchKernelGGL(k_b3_permute_rows<E>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                     st, nd, ld, n_tri, n_ls, to_device, buf.as<E>(), out.as<E>());
+  FM_HIP(hipGetLastError());
+  std::swap(buf, out);
+}
+
+}  // namespace
+
+void b3_order_model(fm_model m, fm_structure s) {
+  B3State *b = b3_state(s);
+  const B3Tables &t = b->t;
+  if (m->b3_ntri == t.n_tri && m->b3_nls == t.n_ls) return;
+  FM_CHECK(m->b3_ntri == 0, FM_ERR_INVALID,
+           "the model's sparse set is in the order of another B3 setting");
+  FM_CHECK(m->nd == t.nd, FM_ERR_INVALID, "model and B3 descriptor sizes disagree");
+  // The model lives on its own context's stream.
+  hipStream_t st = m->ctx->stream;
+  Pool *pool = &m->ctx->pool;
+  int nd = t.nd;
+  for (int sp = 0; sp < m->nos; sp++) {
+    long long Ms = m->M_by_type[sp];
+    if (Ms == 0) continue;
+    permute_cols<double>(st, m->X[sp], pool, Ms, nd, nd, t.n_tri, t.n_ls, 1);
+    permute_rows<double>(st, m->XT[sp], pool, nd, Ms, t.n_tri, t.n_ls, 1);
+    permute_cols<uint16_t>(st, m->Xs0[sp], pool, Ms, split_ld(nd), nd, t.n_tri, t.n_ls, 1);
+    permute_cols<uint16_t>(st, m->Xs1[sp], pool, Ms, split_ld(nd), nd, t.n_tri, t.n_ls, 1);
+    permute_rows<uint16_t>(st, m->XTs0[sp], pool, nd, split_ld(Ms), t.n_tri, t.n_ls, 1);
+    permute_rows<uint16_t>(st, m->XTs1[sp], pool, nd, split_ld(Ms), t.n_tri, t.n_ls, 1);
+  }
+  FM_HIP(hipStreamSynchronize(st));
+  m->b3_ntri = t.n_tri;
+  m->b3_nls = t.n_ls;
+}
+
+void b3_rows_to_reference(fm_structure s, int r0, int rows, void *out) {
+  B3State *b = b3_state(s);
+  const B3Tables &t = b->t;
+  hipStream_t st = s->ctx->stream;
+  long long total = (long long)rows * t.nd;
+  if (total == 0) return;
+  dim3 grid((unsigned)((total + 255) / 256));
+  if (s->form == FM_FORM_FP64)
+    hipLaunchKernelGGL(k_b3_permute_cols<double>, grid, dim3(256), 0, st, (long long)rows,
+                       (long long)t.nd, t.nd, t.n_tri, t.n_ls, 0,
+                       s->B.as<double>() + (size_t)r0 * t.nd, (double *)out);
+  else
+    hipLaunchKernelGGL(k_b3_permute_cols<float>, grid, dim3(256), 0, st, (long long)rows,
+                       (long long)t.nd, t.nd, t.n_tri, t.n_ls, 0,
+                       s->B.as<float>() + (size_t)r0 * t.nd, (float *)out);
+  FM_HIP(hipGetLastError());
 }
 
 void free_b3_state(fm_structure s) {
@@ -400,16 +532,19 @@ void launch_dEdc_b3(fm_structure s, const void *G, int64_t g_ld, int project,
   if (n == 0) return;
   bool f64 = s->form == FM_FORM_FP64;
   size_t esz = f64 ? 8 : 4;
-  size_t per = 4 * (size_t)d.nr * d.H * esz;
-  int apb = waves_per_block(per);
+  int nc = d.nr * d.H;
+  FM_CHECK(d.nr <= 64, FM_ERR_INVALID, "B3 adjoint: n_species n_max above 64");
+  size_t per = (4 * (size_t)nc + (size_t)d.n_tri) * esz;
+  int apb = waves_per_block(per + 256);
   int nblk = (n + apb - 1) / apb;
+  size_t lds = 256 + apb * per;
   if (f64)
-    hipLaunchKernelGGL(k_dEdc_b3<double>, dim3(nblk), dim3(64 * apb), apb * per, st, n, d,
+    hipLaunchKernelGGL(k_dEdc_b3<double>, dim3(nblk), dim3(64 * apb), lds, st, n, d,
                        s->C.as<double>(), s->row_of_atom.as<int>(), s->B.as<double>(),
                        s->norm.as<double>(), s->nn.as<double>(), (const double *)G,
                        (long long)g_ld, project, empty_norm, empty_nn, (double *)U);
   else
-    hipLaunchKernelGGL(k_dEdc_b3<float>, dim3(nblk), dim3(64 * apb), apb * per, st, n, d,
+    hipLaunchKernelGGL(k_dEdc_b3<float>, dim3(nblk), dim3(64 * apb), lds, st, n, d,
                        s->C.as<float>(), s->row_of_atom.as<int>(), s->B.as<float>(),
                        s->norm.as<double>(), s->nn.as<double>(), (const float *)G,
                        (long long)g_ld, project, empty_norm, empty_nn, (float *)U);

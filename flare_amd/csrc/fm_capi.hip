@@ -565,13 +565,24 @@ int fm_structure_get_b2(fm_structure s, int t, double *descriptors, double *norm
     int ns = s->n_by_type[t], ro = s->type_off[t], nd = s->nd;
     if (ns == 0) return;
     if (descriptors) {
+      // B3 rows are stored l-set major on the device: restore the
+      // reference's order (b3.cpp:230-242) in a device copy first.
+      size_t esz = s->form == FM_FORM_FP64 ? sizeof(double) : sizeof(float);
+      fm::DevBuf ref;
+      const void *src = (const char *)s->B.as<char>() + (size_t)ro * nd * esz;
+      if (s->desc_kind == 3) {
+        ref.alloc(&s->ctx->pool, (size_t)ns * nd * esz);
+        fm::b3_rows_to_reference(s, ro, ns, ref.as<void>());
+        src = ref.as<void>();
+      }
       if (s->form == FM_FORM_FP64) {
-        FM_HIP(hipMemcpyAsync(descriptors, s->B.as<double>() + (size_t)ro * nd,
-                              (size_t)ns * nd * sizeof(double), hipMemcpyDeviceToHost, st));
+        FM_HIP(hipMemcpyAsync(descriptors, src, (size_t)ns * nd * sizeof(double),
+                              hipMemcpyDeviceToHost, st));
+        FM_HIP(hipStreamSynchronize(st));
       } else {
         std::vector<float> tmp((size_t)ns * nd);
-        FM_HIP(hipMemcpyAsync(tmp.data(), s->B.as<float>() + (size_t)ro * nd,
-                              tmp.size() * sizeof(float), hipMemcpyDeviceToHost, st));
+        FM_HIP(hipMemcpyAsync(tmp.data(), src, tmp.size() * sizeof(float),
+                              hipMemcpyDeviceToHost, st));
         FM_HIP(hipStreamSynchronize(st));
         for (size_t k = 0; k < tmp.size(); k++) descriptors[k] = tmp[k];
       }
@@ -785,6 +796,8 @@ int fm_structure_get_b2_dervs(fm_structure s, int t, double *force_dervs, double
 }
 
 static void dense_check(fm_model m, fm_structure s) {
+  FM_CHECK(m && m->b3_ntri == 0, FM_ERR_INVALID,
+           "the model's sparse set is in B3 order (it was used with a B3 descriptor)");
   FM_CHECK(m && s, FM_ERR_INVALID, "null argument");
   FM_CHECK(m->ctx == s->ctx, FM_ERR_INVALID, "model and structure belong to different contexts");
   FM_CHECK(s->has_b2, FM_ERR_STATE, "descriptors have not been computed");
@@ -823,6 +836,8 @@ int fm_predict_dense(fm_model m, fm_structure s, int kind, double *mean_efs, dou
 int fm_model_mapping_coefficients(fm_model m, double *beta, int64_t *beta_size) {
   return guarded([&] {
     FM_CHECK(m && beta_size, FM_ERR_INVALID, "null argument");
+    FM_CHECK(m->b3_ntri == 0, FM_ERR_INVALID,
+             "mapping coefficients are for B2 models (this sparse set is in B3 order)");
     int d = m->nd;
     FM_CHECK(m->power == 1.0 || m->power == 2.0, FM_ERR_INVALID,
              "Mapping coefficients of the normalized dot product kernel are implemented for "

@@ -196,6 +196,9 @@ struct fm_model_s {
   // (L_rows x M_s, starting at the species' first row).
   std::vector<fm::DevBuf> X, XT, Xs0, Xs1, XTs0, XTs1, alpha, L;
   std::vector<double> host_alpha;
+  // Non-zero once the sparse set has been permuted to the device order of
+  // B3 descriptors (l-set major) for (n_tri, n_ls) = (b3_ntri, b3_nls).
+  int b3_ntri = 0, b3_nls = 0;
   // Dense (DTC / SOR) predictions: sparse norms per species, alpha in one
   // vector, R_inv^T (M x M) and whether it is lower triangular, and whether
   // L_inv is block diagonal over species.
@@ -228,6 +231,12 @@ void launch_b2(fm_structure s, bool want_b2 = true);
 // fm_b3.hip
 int b3_size(int nos, int N, int lmax);
 void launch_b3(fm_structure s);
+// The model's sparse set in the device order of the structure's B3 rows
+// (permuted once; an error if it is already in another order).
+void b3_order_model(fm_model m, fm_structure s);
+// Rows [r0, r0 + rows) of a B3 structure's descriptors in the reference's
+// order into `out` (device memory, rows x nd, the descriptor precision).
+void b3_rows_to_reference(fm_structure s, int r0, int rows, void *out);
 void launch_dEdc_b3(fm_structure s, const void *G, int64_t g_ld, int project,
                     double empty_norm, double empty_nn, void *U);
 void free_b3_state(fm_structure s);

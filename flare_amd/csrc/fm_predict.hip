@@ -125,6 +125,11 @@ void predict(fm_model m, fm_structure s, int flags, int form, double *mean_efs,
   FM_CHECK(s->form == form, FM_ERR_STATE, "structure descriptors were computed in another form");
   FM_CHECK(m->nos == s->nos && m->nd == s->nd, FM_ERR_INVALID,
            "model and descriptor settings disagree (species or descriptor size)");
+  if (s->desc_kind == 3)
+    b3_order_model(m, s);  // the sparse set in the rows' (l-set major) order
+  else
+    FM_CHECK(m->b3_ntri == 0, FM_ERR_INVALID,
+             "the model's sparse set is in B3 order (it was used with a B3 descriptor)");
   bool want_var = (flags & FM_PREDICT_LOCAL) != 0;
   FM_CHECK(!want_var || m->has_L, FM_ERR_STATE,
            "L_inv must be up to date to evaluate uncertainties");
