@@ -280,8 +280,9 @@ def large_b3_leg(lib, ctx, hip, steps, warmup):
 def serving_leg(sgp, calc, batched, n_requests=2000, n_fp64=200):
     """Config 4: Poisson stream of requests (sizes log-uniform 32-8192
     atoms, the batched model), served with continuous batching in the
-    chosen form at 0.7x the batched throughput; FP64 drop-in evaluated
-    request by request on a prefix of the same stream."""
+    chosen form at 0.7x the batched throughput (three workers, batches of
+    at most 16,384 atoms); FP64 drop-in evaluated request by request on a
+    prefix of the same stream."""
     from flare_amd import _C_flare as CF
     from flare_amd import _lib
     from flare_amd.serving import Server, latency_stats, poisson_run
@@ -289,7 +290,11 @@ def serving_leg(sgp, calc, batched, n_requests=2000, n_fp64=200):
     reqs_s = batch_structures(n_requests, seed=41, n_min=32, n_max=8192)
     mean_n = float(np.mean([len(s[1]) for s in reqs_s]))
     rate = 0.7 * batched["chosen"]["atom_evals_per_s"] / mean_n
-    srv = Server(sgp, calc, 5.5, form=_lib.FORM_CHOSEN, hbm_budget=24 << 30, n_workers=2)
+    # Three workers (contexts) and batches of at most 16,384 atoms: batches
+    # start sooner and three overlap on the GPU (the lowest p50/p99 of the
+    # worker counts and caps tried, at no loss of throughput).
+    srv = Server(sgp, calc, 5.5, form=_lib.FORM_CHOSEN, hbm_budget=24 << 30, n_workers=3,
+                 max_batch_atoms=16384)
     try:
         reqs, wall = poisson_run(srv, reqs_s, rate=rate, seed=3)
     finally:
@@ -311,7 +316,8 @@ def serving_leg(sgp, calc, batched, n_requests=2000, n_fp64=200):
     dt64 = time.perf_counter() - t0
     atoms64 = sum(len(s[1]) for s in reqs_s[:n_fp64])
     return {"workload": f"{n_requests} requests, 32-8192 atoms (mean {mean_n:.0f}), Poisson "
-                        f"rate {rate:.1f}/s (0.7x batched chosen throughput)",
+                        f"rate {rate:.1f}/s (0.7x batched chosen throughput); 3 workers, "
+                        f"batches of at most 16384 atoms",
             "atom_evals_per_s": atoms / wall, "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
             "batches": len(srv.batches),
             "mean_batch_atoms": float(bt[:, 0].mean()),

@@ -42,13 +42,16 @@ class Request:
 
 class Server:
     def __init__(self, sgp, calc, cutoff, form=_lib.FORM_CHOSEN, hbm_budget=16 << 30,
-                 n_workers=2, device=0, local=True):
+                 n_workers=2, device=0, local=True, max_batch_atoms=None):
         self.sgp = sgp
         self.calc = calc
         self.cutoff = cutoff
         self.form = form
         self.local = local
         self.budget = hbm_budget
+        # Optional cap on the atoms of one batch (latency against throughput:
+        # smaller batches start sooner; the HBM budget still applies).
+        self.max_batch_atoms = max_batch_atoms
         nd = calc.n_descriptors
         self.max_sparse = max(max(c.n_clusters_by_type) for c in sgp.sparse_descriptors)
         self._fp = lambda n: footprint_bytes(n, nd, self.max_sparse,
@@ -80,14 +83,16 @@ class Server:
 
     def _admit(self):
         """The oldest waiting requests that fit the budget (at least one)."""
-        batch, used = [], 0
+        batch, used, atoms = [], 0, 0
+        cap = self.max_batch_atoms
         while self._queue:
             n = len(self._queue[0].structure[1])
             need = self._fp(n)
-            if batch and used + need > self.budget:
+            if batch and (used + need > self.budget or (cap is not None and atoms + n > cap)):
                 break
             batch.append(self._queue.popleft())
             used += need
+            atoms += n
         return batch
 
     def _work(self, ctx):

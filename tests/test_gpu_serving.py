@@ -7,7 +7,8 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def test_serving_results_equal_alone():
+@pytest.mark.parametrize("workers,cap", [(2, None), (3, 1200)])
+def test_serving_results_equal_alone(workers, cap):
     from flare_amd import _C_flare as C
     from flare_amd.serving import Server, latency_stats, poisson_run
     from flare_amd.workloads import batch_structures
@@ -20,7 +21,8 @@ def test_serving_results_equal_alone():
     sgp.update_L_inverse()
     sgp.set_alpha(np.random.default_rng(6).normal(scale=0.1, size=sgp.n_sparse))
     # A small budget forces several batches per burst.
-    srv = Server(sgp, calc, 5.5, form=1, hbm_budget=200 << 20, n_workers=2)
+    srv = Server(sgp, calc, 5.5, form=1, hbm_budget=200 << 20, n_workers=workers,
+                 max_batch_atoms=cap)
     try:
         reqs, wall = poisson_run(srv, structs, rate=400.0, seed=1)
     finally:
@@ -29,6 +31,8 @@ def test_serving_results_equal_alone():
     print("serving:", stats, "batches", len(srv.batches), flush=True)
     assert stats["n"] == len(structs)
     assert len(srv.batches) >= 2
+    if cap is not None:  # a batch exceeds the cap only when it is a single request
+        assert all(b[1] <= cap or b[0] == 1 for b in srv.batches)
     for r, (cell, species, pos) in zip(reqs, structs):
         mean, var = r.future.result()
         s = C.Structure(cell, species, pos, 5.5, [calc], form=1)
