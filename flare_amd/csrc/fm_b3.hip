@@ -280,10 +280,16 @@ __global__ __launch_bounds__(256) void k_dEdc_b3(int n, B3Dev t, const T *Cg,
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    for (int g = lane; g < nc; g += 64) {
-      int nn = g / H, lm = g - nn * H;
-      T xr = 0, xi = 0;
-      for (int p = 0; p < 3; p++) {
+    // Per factor position p, only the targets whose lm lies in l_p's block
+    // have terms in this l-set: lanes take those (n, m) pairs. A target met
+    // at several positions is updated in consecutive passes (one wave, LDS
+    // operations in program order).
+    for (int p = 0; p < 3; p++) {
+      int lp = t.ls_l[3 * ls + p], nm = 2 * lp + 1;
+      for (int idx = lane; idx < nr * nm; idx += 64) {
+        int nn = idx / nm, lm = lp * lp + (idx - nn * nm);
+        int g = nn * H + lm;
+        T xr = 0, xi = 0;
         int base = ((ls * H + lm) * 3 + p);
         int e0 = t.gls_off[base], e1 = t.gls_off[base + 1];
         for (int e = e0; e < e1; e++) {
@@ -310,9 +316,12 @@ __global__ __launch_bounds__(256) void k_dEdc_b3(int n, B3Dev t, const T *Cg,
             }
           }
         }
+        gc[2 * g] += xr;
+        gc[2 * g + 1] += xi;
       }
-      gc[2 * g] += xr;
-      gc[2 * g + 1] += xi;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
