@@ -41,7 +41,7 @@ EXPORTS = [
     "fm_mapped_info", "fm_mapped_destroy", "fm_mapped_compute",
     "fm_batch_create", "fm_batch_create_device", "fm_batch_info",
     "fm_structure_get_b2_dervs", "fm_envs_struc", "fm_self_kernel_struc", "fm_predict_dense",
-    "fm_structure_compute_b3",
+    "fm_structure_compute_b3", "fm_context_set_row_tile",
 ]
 
 
@@ -90,6 +90,7 @@ def load():
         L.fm_context_create.argtypes = [ip, pvp]
         L.fm_context_destroy.argtypes = [vp]
         L.fm_context_synchronize.argtypes = [vp]
+        L.fm_context_set_row_tile.argtypes = [vp, C.c_int64]
         L.fm_context_stream.argtypes = [vp, pvp]
         L.fm_structure_create.argtypes = [vp, vp, ip, vp, vp, dp, pvp]
         L.fm_structure_create_device.argtypes = [vp, vp, ip, vp, vp, dp, pvp]
@@ -157,6 +158,11 @@ class Context:
 
     def synchronize(self):
         check(self.L.fm_context_synchronize(self.h))
+
+    def set_row_tile(self, rows):
+        """Row tiles for B3 descriptors computed on this context afterwards
+        (0: tile only structures whose descriptors would exceed 32 GB)."""
+        check(self.L.fm_context_set_row_tile(self.h, int(rows)))
 
     @property
     def stream(self):

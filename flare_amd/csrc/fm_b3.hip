@@ -171,22 +171,43 @@ __device__ void complex_c(const T *cr, T *cc, int nr, int H, int lane) {
 // B3 per atom: one wavefront, complex c in LDS. For each l-set (uniform
 // across the wave, so its term list is read once per wave), lanes take the
 // radial triples and write the row in l-set-major order.
+// Rows of one call: every atom (aor null: wave i takes atom i and writes its
+// species-sorted row), or the species-sorted rows [r0, r0 + n) (aor =
+// atom_of_row: wave k takes row r0 + k and writes row k of B, a row tile).
+struct RowRange {
+  int r0;
+  const int *aor;
+  __device__ __forceinline__ void map(int k, const int *row_of_atom, int &atom, int &row,
+                                      int &brow) const {
+    if (aor) {
+      row = r0 + k;
+      atom = aor[row];
+      brow = k;
+    } else {
+      atom = k;
+      row = row_of_atom[k];
+      brow = row;
+    }
+  }
+};
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_b3(int n, B3Dev t, const T *Cg, const int *row_of_atom,
-                                             T *B, double *norm, double *nnv, __bf16 *B0,
-                                             __bf16 *B1, int ldp) {
+                                             RowRange rr, T *B, double *norm, double *nnv,
+                                             __bf16 *B0, __bf16 *B1, int ldp) {
   extern __shared__ __align__(16) unsigned char smem[];
   int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  int i = blockIdx.x * (blockDim.x >> 6) + w;
-  if (i >= n) return;
+  int k = blockIdx.x * (blockDim.x >> 6) + w;
+  if (k >= n) return;
+  int i, row, brow;
+  rr.map(k, row_of_atom, i, row, brow);
   int nc = t.nr * t.H, H = t.H;
   T *cc = reinterpret_cast<T *>(smem) + (size_t)w * 2 * nc;
   complex_c<T>(Cg + (size_t)i * nc, cc, t.nr, t.H, lane);
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  int row = row_of_atom[i];
-  T *Brow = B + (size_t)row * t.nd;
+  T *Brow = B + (size_t)brow * t.nd;
   T s2 = 0;
   for (int ls = 0; ls < t.n_ls; ls++) {
     int q0 = t.term_off[ls], q1 = t.term_off[ls + 1];
@@ -202,21 +223,21 @@ __global__ __launch_bounds__(256) void k_b3(int n, B3Dev t, const T *Cg, const i
         T pr = xr * yr - xi * yi, pi = xr * yi + xi * yr;
         acc += (T)t.term_w[q] * (pr * zr - pi * zi);
       }
-      int k = ls * t.n_tri + tr;
-      Brow[k] = acc;
+      int kd = ls * t.n_tri + tr;
+      Brow[kd] = acc;
       if (B0) {
         float r = (float)acc;
         __bf16 h = (__bf16)r;
-        B0[(size_t)row * ldp + k] = h;
-        B1[(size_t)row * ldp + k] = (__bf16)(r - (float)h);
+        B0[(size_t)brow * ldp + kd] = h;
+        B1[(size_t)brow * ldp + kd] = (__bf16)(r - (float)h);
       }
       s2 += acc * acc;
     }
   }
   if (B0)
-    for (int k = t.nd + lane; k < ldp; k += 64) {
-      B0[(size_t)row * ldp + k] = (__bf16)0.0f;
-      B1[(size_t)row * ldp + k] = (__bf16)0.0f;
+    for (int kd = t.nd + lane; kd < ldp; kd += 64) {
+      B0[(size_t)brow * ldp + kd] = (__bf16)0.0f;
+      B1[(size_t)brow * ldp + kd] = (__bf16)0.0f;
     }
   s2 = wave_sum(s2);
   if (lane == 0) {
@@ -240,30 +261,32 @@ __device__ __forceinline__ int tri_index(const B3Dev &t, int n1, int n2, int n3)
 // the complex map.
 template <typename T>
 __global__ __launch_bounds__(256) void k_dEdc_b3(int n, B3Dev t, const T *Cg,
-                                                  const int *row_of_atom, const T *B,
-                                                  const double *norm, const double *nnv,
-                                                  const T *G, long long g_ld, int project,
-                                                  double empty_norm, double empty_nn, T *Ug) {
+                                                  const int *row_of_atom, RowRange rr,
+                                                  const T *B, const double *norm,
+                                                  const double *nnv, const T *G, long long g_ld,
+                                                  int project, double empty_norm,
+                                                  double empty_nn, T *Ug) {
   extern __shared__ __align__(16) unsigned char smem[];
   int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  int i = blockIdx.x * (blockDim.x >> 6) + w;
+  int kw = blockIdx.x * (blockDim.x >> 6) + w;
   int nc = t.nr * t.H, nr = t.nr, H = t.H;
   int per = 4 * nc + t.n_tri;
   // Block-shared: index of the first triple of each n1.
   int *tb = reinterpret_cast<int *>(smem);
   for (int k = threadIdx.x; k < nr; k += blockDim.x) tb[k] = t.tri_base[k];
   __syncthreads();
-  if (i >= n) return;
+  if (kw >= n) return;
+  int i, row, brow;
+  rr.map(kw, row_of_atom, i, row, brow);
   T *cc = reinterpret_cast<T *>(smem + 256) + (size_t)w * per;
   T *gc = cc + 2 * nc;  // complex gradient, one (re, im) per target
   T *wl = gc + 2 * nc;  // w slice of one l-set
-  int row = row_of_atom[i];
   double nrm = norm[row];
   if (nrm < empty_norm || nnv[row] < empty_nn) return;
   complex_c<T>(Cg + (size_t)i * nc, cc, t.nr, t.H, lane);
   for (int k = lane; k < 2 * nc; k += 64) gc[k] = 0;
-  const T *Brow = B + (size_t)row * t.nd;
-  const T *Grow = G + (size_t)row * g_ld;
+  const T *Brow = B + (size_t)brow * t.nd;
+  const T *Grow = G + (size_t)brow * g_ld;
   T ninv = T(1) / (T)nrm, gb = 0;
   if (project) {
 #pragma unroll 8
@@ -509,11 +532,15 @@ static int waves_per_block(size_t lds_per_atom) {
 }
 
 void launch_b3(fm_structure s) {
+  launch_b3_rows(s, 0, s->n, false, s->B.as<void>(), s->Bs[0].as<void>(), s->Bs[1].as<void>());
+}
+
+void launch_b3_rows(fm_structure s, int r0, int n, bool tile, void *Bout, void *B0, void *B1) {
   hipStream_t st = s->ctx->stream;
   B3State *b = b3_state(s);
   B3Dev d = b->dev();
-  int n = s->n;
   if (n == 0) return;
+  RowRange rr{r0, tile ? s->atom_of_row.as<int>() : nullptr};
   bool f64 = s->form == FM_FORM_FP64;
   size_t esz = f64 ? 8 : 4;
   size_t per = 2 * (size_t)d.nr * d.H * esz;
@@ -521,24 +548,31 @@ void launch_b3(fm_structure s) {
   int nblk = (n + apb - 1) / apb;
   if (f64)
     hipLaunchKernelGGL(k_b3<double>, dim3(nblk), dim3(64 * apb), apb * per, st, n, d,
-                       s->C.as<double>(), s->row_of_atom.as<int>(), s->B.as<double>(),
+                       s->C.as<double>(), s->row_of_atom.as<int>(), rr, (double *)Bout,
                        s->norm.as<double>(), s->nn.as<double>(), (__bf16 *)nullptr,
                        (__bf16 *)nullptr, 0);
   else
     hipLaunchKernelGGL(k_b3<float>, dim3(nblk), dim3(64 * apb), apb * per, st, n, d,
-                       s->C.as<float>(), s->row_of_atom.as<int>(), s->B.as<float>(),
-                       s->norm.as<double>(), s->nn.as<double>(), s->Bs[0].as<__bf16>(),
-                       s->Bs[1].as<__bf16>(), (int)split_ld(s->nd));
+                       s->C.as<float>(), s->row_of_atom.as<int>(), rr, (float *)Bout,
+                       s->norm.as<double>(), s->nn.as<double>(), (__bf16 *)B0, (__bf16 *)B1,
+                       (int)split_ld(s->nd));
   FM_HIP(hipGetLastError());
 }
 
 void launch_dEdc_b3(fm_structure s, const void *G, int64_t g_ld, int project,
                     double empty_norm, double empty_nn, void *U) {
+  launch_dEdc_b3_rows(s, 0, s->n, false, s->B.as<void>(), G, g_ld, project, empty_norm,
+                      empty_nn, U);
+}
+
+void launch_dEdc_b3_rows(fm_structure s, int r0, int n, bool tile, const void *B,
+                         const void *G, int64_t g_ld, int project, double empty_norm,
+                         double empty_nn, void *U) {
   hipStream_t st = s->ctx->stream;
   B3State *b = b3_state(s);
   B3Dev d = b->dev();
-  int n = s->n;
   if (n == 0) return;
+  RowRange rr{r0, tile ? s->atom_of_row.as<int>() : nullptr};
   bool f64 = s->form == FM_FORM_FP64;
   size_t esz = f64 ? 8 : 4;
   int nc = d.nr * d.H;
@@ -549,12 +583,12 @@ void launch_dEdc_b3(fm_structure s, const void *G, int64_t g_ld, int project,
   size_t lds = 256 + apb * per;
   if (f64)
     hipLaunchKernelGGL(k_dEdc_b3<double>, dim3(nblk), dim3(64 * apb), lds, st, n, d,
-                       s->C.as<double>(), s->row_of_atom.as<int>(), s->B.as<double>(),
+                       s->C.as<double>(), s->row_of_atom.as<int>(), rr, (const double *)B,
                        s->norm.as<double>(), s->nn.as<double>(), (const double *)G,
                        (long long)g_ld, project, empty_norm, empty_nn, (double *)U);
   else
     hipLaunchKernelGGL(k_dEdc_b3<float>, dim3(nblk), dim3(64 * apb), lds, st, n, d,
-                       s->C.as<float>(), s->row_of_atom.as<int>(), s->B.as<float>(),
+                       s->C.as<float>(), s->row_of_atom.as<int>(), rr, (const float *)B,
                        s->norm.as<double>(), s->nn.as<double>(), (const float *)G,
                        (long long)g_ld, project, empty_norm, empty_nn, (float *)U);
   FM_HIP(hipGetLastError());

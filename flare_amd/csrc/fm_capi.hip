@@ -228,6 +228,14 @@ int fm_context_destroy(fm_context ctx) {
   });
 }
 
+int fm_context_set_row_tile(fm_context ctx, int64_t rows) {
+  return guarded([&] {
+    FM_CHECK(ctx, FM_ERR_INVALID, "null context");
+    FM_CHECK(rows >= 0 && rows <= (1ll << 30), FM_ERR_INVALID, "row tile out of range");
+    ctx->row_tile = rows;
+  });
+}
+
 int fm_context_synchronize(fm_context ctx) {
   return guarded([&] { FM_HIP(hipStreamSynchronize(ctx->stream)); });
 }
@@ -474,8 +482,20 @@ static void compute_descriptor(fm_structure s, const fm_b2_settings *st_in, int 
       s->n_bonds = 0;
     }
     size_t esz = form == FM_FORM_FP64 ? sizeof(double) : sizeof(float);
-    s->B.alloc(pool, std::max<size_t>(1, (size_t)s->n * s->nd * esz));
-    size_t split_bytes = (size_t)s->n * fm::split_ld(s->nd) * sizeof(uint16_t);
+    // B3 rows too large to keep are formed in row tiles inside fm_predict.
+    s->row_tile = 0;
+    if (kind == 3 && s->n > 0) {
+      size_t per_row = (size_t)s->nd * (esz + (form == FM_FORM_CHOSEN ? 4 : 0));
+      if (ctx->row_tile > 0) {
+        s->row_tile = ctx->row_tile;
+      } else if ((size_t)s->n * per_row > (32ull << 30)) {
+        size_t tile_row = (size_t)s->nd * (2 * esz + 4);  // B, G (and pieces) per row
+        s->row_tile = std::max<int64_t>(256, (int64_t)((8ull << 30) / tile_row) / 256 * 256);
+      }
+    }
+    size_t rows_kept = s->row_tile > 0 ? 0 : (size_t)s->n;
+    s->B.alloc(pool, std::max<size_t>(1, rows_kept * s->nd * esz));
+    size_t split_bytes = rows_kept * fm::split_ld(s->nd) * sizeof(uint16_t);
     for (auto &piece : s->Bs) {
       if (form == FM_FORM_CHOSEN)
         piece.alloc(pool, std::max<size_t>(1, split_bytes));
@@ -486,7 +506,7 @@ static void compute_descriptor(fm_structure s, const fm_b2_settings *st_in, int 
     s->nn.alloc(pool, std::max<size_t>(1, (size_t)s->n * sizeof(double)));
     if (s->n > 0) {
       fm::launch_b2(s, kind == 2);
-      if (kind == 3) fm::launch_b3(s);
+      if (kind == 3 && s->row_tile == 0) fm::launch_b3(s);
     }
     ctx->mark_end(fm::ST_DESC);
     // Per-structure species-sorted position of every global row (the
@@ -560,6 +580,9 @@ int fm_structure_get_b2(fm_structure s, int t, double *descriptors, double *norm
   return guarded([&] {
     FM_CHECK(s && s->has_b2, FM_ERR_STATE, "descriptors have not been computed");
     FM_CHECK(s->n_struct == 1, FM_ERR_INVALID, "descriptor export is per single structure");
+    FM_CHECK(s->row_tile == 0 || (!descriptors && !norms), FM_ERR_STATE,
+             "this B3 structure forms its descriptors in row tiles during prediction; they "
+             "are not kept (see fm_context_set_row_tile)");
     FM_CHECK(t >= 0 && t < s->nos, FM_ERR_INVALID, "species out of range");
     hipStream_t st = s->ctx->stream;
     int ns = s->n_by_type[t], ro = s->type_off[t], nd = s->nd;

@@ -540,11 +540,14 @@ __global__ __launch_bounds__(64 * kAtomsPerBlock) void k_bond_forces(B2Args a,
                                                                        const double *norm,
                                                                        const double *nnv,
                                                                        AdjArgs aa, const T *Ug,
-                                                                       T *fbond, double *vir) {
+                                                                       T *fbond, double *vir,
+                                                                       const int *atom_list,
+                                                                       int n_list) {
   extern __shared__ __align__(16) unsigned char smem[];
   int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  int i = blockIdx.x * (blockDim.x >> 6) + w;
-  if (i >= a.n) return;
+  int k = blockIdx.x * (blockDim.x >> 6) + w;
+  if (k >= n_list) return;
+  int i = atom_list ? atom_list[k] : k;  // all atoms, or those of a row tile
   int nc = a.nrad * a.H, Np = pad4(a.N);
   // u in LDS as [species][lm][n] with n padded to 4, so a lane reads the
   // radial coefficients of one (species, lm) with 16-byte loads.
@@ -762,10 +765,6 @@ void launch_adjoint(fm_structure s, const void *G, int64_t g_ld, void *fbond,
   size_t per1 = ((size_t)pad4((int)nc) + (size_t)pad4(a.nd)) * esz;
   int apb1 = s->desc_kind == 2 ? atoms_per_block(per1) : 1;
   int nblk1 = (s->n + apb1 - 1) / apb1;
-  // Bond forces: u in LDS per atom (radial index padded to 4).
-  size_t per2 = (size_t)a.nos * a.H * pad4(a.N) * esz;
-  int apb2 = atoms_per_block(per2);
-  int nblk2 = (s->n + apb2 - 1) / apb2;
   if (s->n == 0) return;
   if (s->desc_kind == 3)
     launch_dEdc_b3(s, G, g_ld, project, empty_norm, empty_nn, U.as<void>());
@@ -775,19 +774,39 @@ void launch_adjoint(fm_structure s, const void *G, int64_t g_ld, void *fbond,
                          s->B.as<double>(), s->norm.as<double>(), s->nn.as<double>(),
                          (const double *)G, (long long)g_ld, aa, s->C.as<double>(),
                          U.as<double>());
-    hipLaunchKernelGGL((k_bond_forces<double, -1>), dim3(nblk2), dim3(64 * apb2), apb2 * per2,
-                       st, a, s->norm.as<double>(), s->nn.as<double>(), aa, U.as<double>(),
-                       (double *)fbond, virial_atom);
   } else {
     if (s->desc_kind == 2)
       hipLaunchKernelGGL(k_dEdc<float>, dim3(nblk1), dim3(64 * apb1), apb1 * per1, st, a,
                          s->B.as<float>(), s->norm.as<double>(), s->nn.as<double>(),
                          (const float *)G, (long long)g_ld, aa, s->C.as<float>(),
                          U.as<float>());
-    hipLaunchKernelGGL((k_bond_forces<float, -1>), dim3(nblk2), dim3(64 * apb2), apb2 * per2,
-                       st, a, s->norm.as<double>(), s->nn.as<double>(), aa, U.as<float>(),
-                       (float *)fbond, virial_atom);
   }
+  FM_HIP(hipGetLastError());
+  launch_bond_forces(s, U.as<void>(), fbond, virial_atom, nullptr, s->n, empty_norm, empty_nn);
+}
+
+void launch_bond_forces(fm_structure s, const void *U, void *fbond, double *virial_atom,
+                        const int *atom_list, int n_list, double empty_norm, double empty_nn) {
+  if (n_list == 0) return;
+  hipStream_t st = s->ctx->stream;
+  B2Args a = make_args(s);
+  upload_knorm(st, s->lmax);
+  upload_rcp(st);
+  bool f64 = s->form == FM_FORM_FP64;
+  size_t esz = f64 ? sizeof(double) : sizeof(float);
+  AdjArgs aa{1, empty_norm, empty_nn, nullptr, 0};
+  // u in LDS per atom (radial index padded to 4).
+  size_t per2 = (size_t)a.nos * a.H * pad4(a.N) * esz;
+  int apb2 = atoms_per_block(per2);
+  int nblk2 = (n_list + apb2 - 1) / apb2;
+  if (f64)
+    hipLaunchKernelGGL((k_bond_forces<double, -1>), dim3(nblk2), dim3(64 * apb2), apb2 * per2,
+                       st, a, s->norm.as<double>(), s->nn.as<double>(), aa, (const double *)U,
+                       (double *)fbond, virial_atom, atom_list, n_list);
+  else
+    hipLaunchKernelGGL((k_bond_forces<float, -1>), dim3(nblk2), dim3(64 * apb2), apb2 * per2,
+                       st, a, s->norm.as<double>(), s->nn.as<double>(), aa, (const float *)U,
+                       (float *)fbond, virial_atom, atom_list, n_list);
   FM_HIP(hipGetLastError());
 }
 

@@ -122,6 +122,8 @@ struct fm_context_s {
   // the adjoint run on `stream`. fork / join order them with events.
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // Row tile for B3 descriptors computed on this context (0: automatic).
+  int64_t row_tile = 0;
   hipEvent_t ev_begin[fm::ST_COUNT], ev_end[fm::ST_COUNT];
   bool recorded[fm::ST_COUNT] = {};
   int launches[fm::ST_COUNT] = {};
@@ -132,6 +134,9 @@ struct fm_context_s {
 struct fm_structure_s {
   fm_context ctx = nullptr;
   int n = 0;
+  // B3 only: > 0 when the descriptor rows are not kept but formed in tiles
+  // of this many species-sorted rows inside fm_predict.
+  int64_t row_tile = 0;
   // A structure object holds a batch of n_struct periodic structures; atoms
   // of structure k are [atom_off[k], atom_off[k+1]). Single structures have
   // n_struct = 1 and the fields below (cell ... sweep) describe structure 0.
@@ -231,15 +236,27 @@ void launch_b2(fm_structure s, bool want_b2 = true);
 // fm_b3.hip
 int b3_size(int nos, int N, int lmax);
 void launch_b3(fm_structure s);
+// B3 rows of one call: all atoms (tile false: atom order, each written at its
+// species-sorted row of Bout) or the species-sorted rows [r0, r0 + n) (tile
+// true: written as rows 0.. n - 1 of Bout, a row tile). B0 / B1: the BF16
+// pieces (chosen form), same row layout with split_ld(nd).
+void launch_b3_rows(fm_structure s, int r0, int n, bool tile, void *Bout, void *B0, void *B1);
 // The model's sparse set in the device order of the structure's B3 rows
 // (permuted once; an error if it is already in another order).
 void b3_order_model(fm_model m, fm_structure s);
 // Rows [r0, r0 + rows) of a B3 structure's descriptors in the reference's
 // order into `out` (device memory, rows x nd, the descriptor precision).
 void b3_rows_to_reference(fm_structure s, int r0, int rows, void *out);
+void launch_dEdc_b3_rows(fm_structure s, int r0, int n, bool tile, const void *B,
+                         const void *G, int64_t g_ld, int project, double empty_norm,
+                         double empty_nn, void *U);
 void launch_dEdc_b3(fm_structure s, const void *G, int64_t g_ld, int project,
                     double empty_norm, double empty_nn, void *U);
 void free_b3_state(fm_structure s);
+// Bond forces and per-atom virials from u = dE/dc (n x n_rad H, atom
+// order) for all atoms (atom_list null, n_list = n) or the listed atoms.
+void launch_bond_forces(fm_structure s, const void *U, void *fbond, double *virial_atom,
+                        const int *atom_list, int n_list, double empty_norm, double empty_nn);
 // With project, G . B comes from `energy` (per species-sorted row; the
 // rows' local energies of a normalized dot product kernel of this power)
 // when given, else from a pass over the G and B rows.

@@ -19,40 +19,43 @@ namespace fm {
 // tiles; var = sigma^2 (B.B / |B|^2)^p - sum of the |Q|^2 partials over row
 // tiles (the diagonal of envs_envs(cluster, cluster) minus diag(Q1^T Q1),
 // sparse_gp.cpp:116-136). Partials are summed in tile order.
-__device__ __forceinline__ double row_variance(int r, const double *Qpart, int n_qtiles, int n_s,
-                                               double nrm, double nnr, double sig2,
-                                               double power) {
+// Rows [ro, ro + n_rows); partial t of row r at t * part_ld + r.
+__device__ __forceinline__ double row_variance(int r, const double *Qpart, int n_qtiles,
+                                               long long part_ld, double nrm, double nnr,
+                                               double sig2, double power) {
   double v = 0;
   if (nrm >= kEmptyThresh) {
     double nd = nnr / (nrm * nrm);
     double ks = sig2 * (power == 2.0 ? nd * nd : pow(nd, power));
     double q = 0;
-    for (int t = 0; t < n_qtiles; t++) q += Qpart[(size_t)t * n_s + r];
+    for (int t = 0; t < n_qtiles; t++) q += Qpart[(size_t)t * part_ld + r];
     v = ks - q;
   }
   return v;
 }
 
-__global__ void k_species_final(int n_s, int ro, const double *Epart, int n_etiles,
-                                const double *Qpart, int n_qtiles, const double *norm,
-                                const double *nn, double sig2, double power, double *E_row,
-                                double *var_row) {
+__global__ void k_species_final(int n_rows, int ro, const double *Epart, int n_etiles,
+                                const double *Qpart, int n_qtiles, long long part_ld,
+                                const double *norm, const double *nn, double sig2, double power,
+                                double *E_row, double *var_row) {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n_s) return;
+  if (r >= n_rows) return;
   double e = 0;
-  for (int t = 0; t < n_etiles; t++) e += Epart[(size_t)t * n_s + r];
+  for (int t = 0; t < n_etiles; t++) e += Epart[(size_t)t * part_ld + r];
   E_row[ro + r] = e;
   if (var_row)
-    var_row[ro + r] = row_variance(r, Qpart, n_qtiles, n_s, norm[ro + r], nn[ro + r], sig2, power);
+    var_row[ro + r] =
+        row_variance(r, Qpart, n_qtiles, part_ld, norm[ro + r], nn[ro + r], sig2, power);
 }
 
 // The variance half alone (run on the side stream).
-__global__ void k_var_final(int n_s, int ro, const double *Qpart, int n_qtiles,
-                            const double *norm, const double *nn, double sig2, double power,
-                            double *var_row) {
+__global__ void k_var_final(int n_rows, int ro, const double *Qpart, int n_qtiles,
+                            long long part_ld, const double *norm, const double *nn, double sig2,
+                            double power, double *var_row) {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n_s) return;
-  var_row[ro + r] = row_variance(r, Qpart, n_qtiles, n_s, norm[ro + r], nn[ro + r], sig2, power);
+  if (r >= n_rows) return;
+  var_row[ro + r] =
+      row_variance(r, Qpart, n_qtiles, part_ld, norm[ro + r], nn[ro + r], sig2, power);
 }
 
 // Forces: F_i = sum_{bonds b of i} f_b - sum_{bonds b with neighbour i} f_b
@@ -116,6 +119,221 @@ __global__ void k_fill(double *p, long long n, double v) {
   if (i < n) p[i] = v;
 }
 
+// Per-atom forces (own bonds minus incoming bonds), energies and virials
+// into `o` (one mean_efs block per structure).
+static void gather_forces(fm_structure s, bool f64, DevBuf &fbond, DevBuf &vir, DevBuf &E_row,
+                          double *o) {
+  hipStream_t st = s->ctx->stream;
+  Pool *pool = &s->ctx->pool;
+  DevBuf blk(pool, (size_t)s->n_chunks * 7 * sizeof(double));
+  if (f64)
+    hipLaunchKernelGGL(k_gather<double>, dim3(s->n_chunks), dim3(256), 0, st,
+                       s->d_chunk_begin.as<int>(), s->d_chunk_end.as<int>(),
+                       s->atom_struct.as<int>(), s->d_atom_off.as<int>(),
+                       s->d_mean_off.as<long long>(), s->bond_off.as<int>(), s->in_off.as<int>(),
+                       s->in_bond.as<int>(), fbond.as<double>(), s->row_of_atom.as<int>(),
+                       E_row.as<double>(), vir.as<double>(), o, blk.as<double>());
+  else
+    hipLaunchKernelGGL(k_gather<float>, dim3(s->n_chunks), dim3(256), 0, st,
+                       s->d_chunk_begin.as<int>(), s->d_chunk_end.as<int>(),
+                       s->atom_struct.as<int>(), s->d_atom_off.as<int>(),
+                       s->d_mean_off.as<long long>(), s->bond_off.as<int>(), s->in_off.as<int>(),
+                       s->in_bond.as<int>(), fbond.as<float>(), s->row_of_atom.as<int>(),
+                       E_row.as<double>(), vir.as<double>(), o, blk.as<double>());
+  hipLaunchKernelGGL(k_final_sum, dim3((s->n_struct + 63) / 64), dim3(64), 0, st, s->n_struct,
+                     s->d_struct_chunk_off.as<int>(), blk.as<double>(), s->d_vol_inv.as<double>(),
+                     s->d_mean_off.as<long long>(), s->d_atom_off.as<int>(), o);
+}
+
+// Results to the caller's host buffers (any may be null).
+static void read_outputs(fm_structure s, const double *o, size_t n_out, const double *var_row,
+                         const double *E_row, double *mean_efs, double *local_var,
+                         double *local_energy) {
+  hipStream_t st = s->ctx->stream;
+  int n = s->n;
+  if (mean_efs)
+    FM_HIP(hipMemcpyAsync(mean_efs, o, (size_t)n_out * sizeof(double), hipMemcpyDeviceToHost, st));
+  if (local_var && var_row && n > 0) {
+    // Species-sorted rows of the batch to each structure's species-sorted
+    // order (compute_cluster_uncertainties order).
+    std::vector<double> vr(n);
+    FM_HIP(hipMemcpyAsync(vr.data(), var_row, (size_t)n * sizeof(double),
+                          hipMemcpyDeviceToHost, st));
+    FM_HIP(hipStreamSynchronize(st));
+    for (int r = 0; r < n; r++) local_var[s->var_pos[r]] = vr[r];
+  }
+  if (local_energy && n > 0) {
+    // E_row is in species-sorted order; map to atom order on the host.
+    std::vector<double> er(n);
+    std::vector<int> aor(n);
+    FM_HIP(hipMemcpyAsync(er.data(), E_row, (size_t)n * sizeof(double),
+                          hipMemcpyDeviceToHost, st));
+    FM_HIP(hipMemcpyAsync(aor.data(), s->atom_of_row.as<int>(), (size_t)n * sizeof(int),
+                          hipMemcpyDeviceToHost, st));
+    FM_HIP(hipStreamSynchronize(st));
+    for (int r = 0; r < n; r++) local_energy[aor[r]] = er[r];
+  }
+}
+
+
+// B3 structures too large to keep their descriptors (s->row_tile > 0): each
+// species' rows are processed in tiles of row_tile rows. Per tile: B3 rows
+// (from the coefficients c kept for every atom), the kernel contraction with
+// its energy partials, the variance contraction, the gradient contraction
+// and the adjoint of the tile's atoms (their u and bond forces); the force
+// gather runs once at the end. Every output element sums its terms in the
+// order of the untiled path, so the results are the same bit for bit.
+static void predict_tiled(fm_model m, fm_structure s, int flags, int form, double *mean_efs,
+                          double *local_var, double *local_energy) {
+  fm_context ctx = s->ctx;
+  hipStream_t st = ctx->stream;
+  Pool *pool = &ctx->pool;
+  bool want_var = (flags & FM_PREDICT_LOCAL) != 0;
+  bool f64 = form == FM_FORM_FP64;
+  size_t esz = f64 ? sizeof(double) : sizeof(float);
+  int n = s->n, nd = s->nd, nos = s->nos;
+  size_t n_out = (size_t)s->mean_off[s->n_struct];
+  int T = (int)std::min<int64_t>(s->row_tile, std::max(1, n));
+  std::vector<size_t> eo(nos + 1, 0), qo(nos + 1, 0);
+  int Mmax = 0;
+  for (int t = 0; t < nos; t++) {
+    size_t ns = s->n_by_type[t];
+    int Ms = m->M_by_type[t];
+    Mmax = std::max(Mmax, Ms);
+    eo[t + 1] = eo[t] + ns * (size_t)std::max(1, gemm_col_tiles(form, Ms));
+    int Lr = want_var ? m->L_rows[t] : 0;
+    qo[t + 1] = qo[t] + ns * (size_t)gemm_row_tiles(FM_FORM_FP64, Lr);
+  }
+  // Whole-structure buffers.
+  DevBuf Epart(pool, std::max<size_t>(1, eo[nos] * sizeof(double)));
+  DevBuf Qpart(pool, std::max<size_t>(1, qo[nos] * sizeof(double)));
+  DevBuf E_row(pool, std::max<size_t>(1, (size_t)n * sizeof(double)));
+  DevBuf var_row(pool, std::max<size_t>(1, (size_t)n * sizeof(double)));
+  DevBuf out(pool, (size_t)n_out * sizeof(double));
+  size_t nc = (size_t)s->nos * s->N * (s->lmax + 1) * (s->lmax + 1);
+  DevBuf U(pool, std::max<size_t>(1, (size_t)n * nc * esz));
+  DevBuf fbond(pool, std::max<size_t>(1, (size_t)3 * s->n_bonds * esz));
+  DevBuf vir(pool, std::max<size_t>(1, (size_t)6 * n * sizeof(double)));
+  // Tile buffers.
+  long long ldd = split_ld(nd), ldm = split_ld(std::max(Mmax, 1));
+  size_t kld = f64 ? (size_t)Mmax : ((size_t)Mmax + 3) / 4 * 4;
+  DevBuf Bt(pool, std::max<size_t>(1, (size_t)T * nd * esz));
+  DevBuf Gt(pool, std::max<size_t>(1, (size_t)T * nd * esz));
+  DevBuf Bt0, Bt1, Wt, Wh, Wl, Kt;
+  if (f64) {
+    Wt.alloc(pool, std::max<size_t>(1, (size_t)T * Mmax * sizeof(double)));
+  } else {
+    Bt0.alloc(pool, std::max<size_t>(1, (size_t)T * ldd * sizeof(uint16_t)));
+    Bt1.alloc(pool, std::max<size_t>(1, (size_t)T * ldd * sizeof(uint16_t)));
+    Wh.alloc(pool, std::max<size_t>(1, (size_t)T * ldm * sizeof(uint16_t)));
+    Wl.alloc(pool, std::max<size_t>(1, (size_t)T * ldm * sizeof(uint16_t)));
+  }
+  if (want_var) Kt.alloc(pool, std::max<size_t>(1, (size_t)T * kld * esz));
+  double sig2 = m->sigma * m->sigma;
+  ctx->mark_begin(ST_SGEMM);
+  for (int t = 0; t < nos; t++) {
+    int ns = s->n_by_type[t], Ms = m->M_by_type[t], ro = s->type_off[t];
+    int n_et = Ms == 0 ? 1 : gemm_col_tiles(form, Ms);
+    int n_qt = (want_var && Ms > 0) ? gemm_row_tiles(FM_FORM_FP64, m->L_rows[t]) : 0;
+    for (int c0 = 0; c0 < ns; c0 += T) {
+      int cnt = std::min(T, ns - c0), r0 = ro + c0;
+      launch_b3_rows(s, r0, cnt, true, Bt.as<void>(), f64 ? nullptr : Bt0.as<void>(),
+                     f64 ? nullptr : Bt1.as<void>());
+      double *ep = Epart.as<double>() + eo[t] + c0;
+      if (Ms == 0) {
+        hipLaunchKernelGGL(k_fill, dim3((cnt + 255) / 256), dim3(256), 0, st, ep,
+                           (long long)cnt, 0.0);
+        FM_HIP(hipMemsetAsync(Gt.as<void>(), 0, (size_t)cnt * nd * esz, st));
+      } else {
+        // Kernel contraction of the tile.
+        EpArgs ek;
+        ek.sig2 = sig2;
+        ek.power = m->power;
+        ek.row_norm = s->norm.as<double>() + r0;
+        ek.alpha = m->alpha[t].as<double>();
+        if (f64) {
+          ek.W = Wt.as<double>();
+          ek.ldw = Ms;
+        } else {
+          ek.Wh = Wh.as<uint16_t>();
+          ek.Wl = Wl.as<uint16_t>();
+          ek.ldw = split_ld(Ms);
+        }
+        size_t kl = f64 ? (size_t)Ms : ((size_t)Ms + 3) / 4 * 4;
+        if (want_var) {
+          ek.K = Kt.as<void>();
+          ek.ldk = (long long)kl;
+        }
+        ek.part = ep;
+        ek.part_ld = ns;
+        if (f64) {
+          gemm_f64(st, EP_KERNEL, cnt, Ms, nd, Bt.as<double>(), nd, m->X[t].as<double>(), nd, 0,
+                   ek);
+        } else {
+          SplitOperand A, Bx;
+          A.piece[0] = Bt0.as<uint16_t>();
+          A.piece[1] = Bt1.as<uint16_t>();
+          A.ld = ldd;
+          Bx.piece[0] = m->Xs0[t].as<uint16_t>();
+          Bx.piece[1] = m->Xs1[t].as<uint16_t>();
+          Bx.ld = ldd;
+          gemm_split(st, EP_KERNEL, 2, cnt, Ms, (int)ldd, A, Bx, ek);
+        }
+        // Variance contraction of the tile.
+        if (want_var && m->L_rows[t] > 0) {
+          EpArgs eq;
+          eq.part = Qpart.as<double>() + qo[t] + c0;
+          eq.part_ld = ns;
+          if (f64)
+            gemm_f64(st, EP_SUMSQ, m->L_rows[t], cnt, Ms, m->L[t].as<double>(), Ms,
+                     Kt.as<double>(), (long long)kl, m->L_lower[t], eq);
+          else
+            gemm_f64(st, EP_SUMSQ, m->L_rows[t], cnt, Ms, m->L[t].as<double>(), Ms,
+                     Kt.as<float>(), (long long)kl, m->L_lower[t], eq);
+        }
+        // Gradient contraction of the tile.
+        EpArgs eg;
+        eg.W = Gt.as<void>();
+        eg.ldw = nd;
+        if (f64) {
+          gemm_f64(st, EP_STORE, cnt, nd, Ms, Wt.as<double>(), Ms, m->XT[t].as<double>(), Ms, 0,
+                   eg);
+        } else {
+          long long lm = split_ld(Ms);
+          SplitOperand A, Bx;
+          A.piece[0] = Wh.as<uint16_t>();
+          A.piece[1] = Wl.as<uint16_t>();
+          A.ld = lm;
+          Bx.piece[0] = m->XTs0[t].as<uint16_t>();
+          Bx.piece[1] = m->XTs1[t].as<uint16_t>();
+          Bx.ld = lm;
+          gemm_split(st, EP_STORE, 2, cnt, nd, (int)lm, A, Bx, eg);
+        }
+      }
+      hipLaunchKernelGGL(k_species_final, dim3((cnt + 255) / 256), dim3(256), 0, st, cnt, r0, ep,
+                         n_et, Qpart.as<double>() + qo[t] + c0, n_qt, (long long)ns,
+                         s->norm.as<double>(), s->nn.as<double>(), sig2, m->power,
+                         E_row.as<double>(), want_var ? var_row.as<double>() : nullptr);
+      // Adjoint of the tile's atoms: u, then their bonds' forces.
+      launch_dEdc_b3_rows(s, r0, cnt, true, Bt.as<void>(), Gt.as<void>(), nd, 1, kEmptyThresh,
+                          -1.0, U.as<void>());
+      launch_bond_forces(s, U.as<void>(), fbond.as<void>(), vir.as<double>(),
+                         s->atom_of_row.as<int>() + r0, cnt, kEmptyThresh, -1.0);
+      FM_HIP(hipGetLastError());
+    }
+  }
+  ctx->mark_end(ST_SGEMM);
+  ctx->mark_begin(ST_ADJ);
+  double *o = out.as<double>();
+  FM_HIP(hipMemsetAsync(o, 0, (size_t)n_out * sizeof(double), st));
+  if (n > 0) gather_forces(s, f64, fbond, vir, E_row, o);
+  FM_HIP(hipGetLastError());
+  ctx->mark_end(ST_ADJ);
+  read_outputs(s, o, n_out, want_var ? var_row.as<double>() : nullptr, E_row.as<double>(),
+               mean_efs, local_var, local_energy);
+  FM_HIP(hipStreamSynchronize(st));
+}
+
 void predict(fm_model m, fm_structure s, int flags, int form, double *mean_efs,
              double *local_var, double *local_energy) {
   fm_context ctx = s->ctx;
@@ -133,6 +351,8 @@ void predict(fm_model m, fm_structure s, int flags, int form, double *mean_efs,
   bool want_var = (flags & FM_PREDICT_LOCAL) != 0;
   FM_CHECK(!want_var || m->has_L, FM_ERR_STATE,
            "L_inv must be up to date to evaluate uncertainties");
+  if (s->desc_kind == 3 && s->row_tile > 0)
+    return predict_tiled(m, s, flags, form, mean_efs, local_var, local_energy);
   bool f64 = form == FM_FORM_FP64;
   size_t esz = f64 ? sizeof(double) : sizeof(float);
   int n = s->n, nd = s->nd, nos = s->nos;
@@ -256,8 +476,9 @@ void predict(fm_model m, fm_structure s, int flags, int form, double *mean_efs,
     if (ns == 0) continue;
     int n_et = Ms == 0 ? 1 : gemm_col_tiles(form, Ms);
     hipLaunchKernelGGL(k_species_final, dim3((ns + 255) / 256), dim3(256), 0, st, ns, ro,
-                       Epart.as<double>() + eo[t], n_et, nullptr, 0, s->norm.as<double>(),
-                       s->nn.as<double>(), sig2, m->power, E_row.as<double>(), nullptr);
+                       Epart.as<double>() + eo[t], n_et, nullptr, 0, (long long)ns,
+                       s->norm.as<double>(), s->nn.as<double>(), sig2, m->power,
+                       E_row.as<double>(), nullptr);
   }
   FM_HIP(hipGetLastError());
 
@@ -299,7 +520,7 @@ void predict(fm_model m, fm_structure s, int flags, int form, double *mean_efs,
       if (ns == 0) continue;
       int n_qt = Ms > 0 ? gemm_row_tiles(FM_FORM_FP64, m->L_rows[t]) : 0;
       hipLaunchKernelGGL(k_var_final, dim3((ns + 255) / 256), dim3(256), 0, sd, ns, ro,
-                         Qpart.as<double>() + qo[t], n_qt, s->norm.as<double>(),
+                         Qpart.as<double>() + qo[t], n_qt, (long long)ns, s->norm.as<double>(),
                          s->nn.as<double>(), sig2, m->power, var_row.as<double>());
     }
     FM_HIP(hipGetLastError());
@@ -321,24 +542,7 @@ void predict(fm_model m, fm_structure s, int flags, int form, double *mean_efs,
     // G - (G . B_hat) B_hat must see the G it is applied to.
     launch_adjoint(s, G.as<void>(), nd, fbond.as<void>(), vir.as<double>(), 1, kEmptyThresh,
                    -1.0, f64 ? E_row.as<double>() : nullptr, m->power);
-    DevBuf blk(pool, (size_t)s->n_chunks * 7 * sizeof(double));
-    if (f64)
-      hipLaunchKernelGGL(k_gather<double>, dim3(s->n_chunks), dim3(256), 0, st,
-                         s->d_chunk_begin.as<int>(), s->d_chunk_end.as<int>(),
-                         s->atom_struct.as<int>(), s->d_atom_off.as<int>(),
-                         s->d_mean_off.as<long long>(), s->bond_off.as<int>(), s->in_off.as<int>(),
-                         s->in_bond.as<int>(), fbond.as<double>(), s->row_of_atom.as<int>(),
-                         E_row.as<double>(), vir.as<double>(), o, blk.as<double>());
-    else
-      hipLaunchKernelGGL(k_gather<float>, dim3(s->n_chunks), dim3(256), 0, st,
-                         s->d_chunk_begin.as<int>(), s->d_chunk_end.as<int>(),
-                         s->atom_struct.as<int>(), s->d_atom_off.as<int>(),
-                         s->d_mean_off.as<long long>(), s->bond_off.as<int>(), s->in_off.as<int>(),
-                         s->in_bond.as<int>(), fbond.as<float>(), s->row_of_atom.as<int>(),
-                         E_row.as<double>(), vir.as<double>(), o, blk.as<double>());
-    hipLaunchKernelGGL(k_final_sum, dim3((s->n_struct + 63) / 64), dim3(64), 0, st, s->n_struct,
-                       s->d_struct_chunk_off.as<int>(), blk.as<double>(), s->d_vol_inv.as<double>(),
-                       s->d_mean_off.as<long long>(), s->d_atom_off.as<int>(), o);
+    gather_forces(s, f64, fbond, vir, E_row, o);
     ctx->launches[ST_ADJ] += 3;
   }
   FM_HIP(hipGetLastError());
@@ -348,28 +552,8 @@ void predict(fm_model m, fm_structure s, int flags, int form, double *mean_efs,
     join.active = false;
   }
 
-  if (mean_efs)
-    FM_HIP(hipMemcpyAsync(mean_efs, o, (size_t)n_out * sizeof(double), hipMemcpyDeviceToHost, st));
-  if (local_var && want_var && n > 0) {
-    // Species-sorted rows of the batch to each structure's species-sorted
-    // order (compute_cluster_uncertainties order).
-    std::vector<double> vr(n);
-    FM_HIP(hipMemcpyAsync(vr.data(), var_row.as<double>(), (size_t)n * sizeof(double),
-                          hipMemcpyDeviceToHost, st));
-    FM_HIP(hipStreamSynchronize(st));
-    for (int r = 0; r < n; r++) local_var[s->var_pos[r]] = vr[r];
-  }
-  if (local_energy && n > 0) {
-    // E_row is in species-sorted order; map to atom order on the host.
-    std::vector<double> er(n);
-    std::vector<int> aor(n);
-    FM_HIP(hipMemcpyAsync(er.data(), E_row.as<double>(), (size_t)n * sizeof(double),
-                          hipMemcpyDeviceToHost, st));
-    FM_HIP(hipMemcpyAsync(aor.data(), s->atom_of_row.as<int>(), (size_t)n * sizeof(int),
-                          hipMemcpyDeviceToHost, st));
-    FM_HIP(hipStreamSynchronize(st));
-    for (int r = 0; r < n; r++) local_energy[aor[r]] = er[r];
-  }
+  read_outputs(s, o, n_out, want_var ? var_row.as<double>() : nullptr, E_row.as<double>(),
+               mean_efs, local_var, local_energy);
   FM_HIP(hipStreamSynchronize(st));
 }
 

@@ -246,6 +246,16 @@ int fm_mapped_compute(fm_mapped mp, fm_structure s, double *atom_energy,
 #define FM_N_STAGES 6
 int fm_last_timings(fm_context ctx, double *ms, int32_t *launches);
 
+/* Row tiles for B3 structures whose descriptors are too large to keep
+ * (B3 at 10^6 atoms is 296 GB in FP32): for B3 descriptors computed on this
+ * context afterwards, rows > 0 forces tiles of that many species-sorted rows;
+ * 0 (the default) tiles only when the descriptor rows would exceed 32 GB,
+ * with tiles of about 8 GB of working buffers. A tiled structure keeps its
+ * coefficients c; its B3 rows are formed tile by tile inside fm_predict
+ * (results equal the untiled ones bit for bit) and cannot be read back with
+ * fm_structure_get_b2. No reference counterpart (b3.cpp keeps every row). */
+int fm_context_set_row_tile(fm_context ctx, int64_t rows);
+
 #ifdef __cplusplus
 }
 #endif

@@ -104,3 +104,25 @@ def test_b3_limits(C, system):
     s = C.Structure(system["cell"], system["species"], system["pos"], 4.6, [system["calc"]])
     with pytest.raises(ValueError, match="B2"):
         s.descriptors[0].descriptor_force_dervs
+
+
+@pytest.mark.parametrize("form", [0, 1])
+def test_b3_row_tiles(C, oracle, system, form):
+    """Prediction over B3 row tiles (descriptors formed tile by tile, never
+    held whole) equals the untiled prediction bit for bit, for tiles that
+    split species and atoms unevenly."""
+    say("B3 row tiles, form", form)
+    from flare_amd import _lib
+    m, sgp = model(C, oracle, system, 2.0)
+    args = (system["cell"], system["species"], system["pos"], 4.6, [system["calc"]])
+    whole = C.Structure(*args, form=form)
+    sgp.predict_local_uncertainties(whole)
+    for rows in (7, 16):
+        ctx = _lib.Context(0)
+        ctx.set_row_tile(rows)
+        s = C.Structure(*args, form=form, context=ctx)
+        sgp.predict_local_uncertainties(s)
+        assert np.array_equal(s.mean_efs, whole.mean_efs), rows
+        assert np.array_equal(s.local_uncertainties[0], whole.local_uncertainties[0]), rows
+        with pytest.raises(RuntimeError, match="row tiles"):
+            s.descriptors[0].descriptors
