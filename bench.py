@@ -230,12 +230,12 @@ def large_leg(lib, ctx, hip, steps, warmup):
 
 
 def large_b3_leg(lib, ctx, hip, steps, warmup):
-    """Config 5 (B3 kernel), at the largest atom count whose descriptors fit
-    whole in HBM: a 65,536-atom three-species BCC alloy (a = 3.2 A, 32^3
-    cells, seed 8); B3 [3, 6, 4] rc 5.5 (d = 74,100); NDP sigma 2, p 2; 4096
-    sparse environments: every atom of a 16 x 16 x 8-cell sample (seed 11).
-    The 1M-atom B3 structure needs 296 GB of FP32 descriptors at once; atom
-    tiles are not built, so it is not run."""
+    """Config 5 (B3 kernel): the 1,048,576-atom three-species BCC alloy of
+    the B2 leg (a = 3.2 A, 128 x 64 x 64 cells, displacements 0.1 A, seed
+    8); B3 [3, 6, 4] rc 5.5 (d = 74,100); NDP sigma 2, p 2; 4096 sparse
+    environments: every atom of a 16 x 16 x 8-cell sample (seed 11). Its
+    FP32 descriptors alone would take 296 GB, so prediction forms them in
+    row tiles (automatic above 32 GB of descriptor rows)."""
     from flare_amd import _C_flare as CF
     from flare_amd import _lib
     from flare_amd.workloads import BCC_BASIS, random_alloy
@@ -248,7 +248,7 @@ def large_b3_leg(lib, ctx, hip, steps, warmup):
     del small
     sgp.update_L_inverse()
     sgp.set_alpha(np.random.default_rng(4).normal(scale=0.1, size=sgp.n_sparse))
-    cell, species, pos = random_alloy((32, 32, 32), a=3.2, nos=3, sigma=0.1, seed=8,
+    cell, species, pos = random_alloy((128, 64, 64), a=3.2, nos=3, sigma=0.1, seed=8,
                                       basis=BCC_BASIS)
     n = len(pos)
     pos_dev = hip.upload(np.ascontiguousarray(pos.reshape(-1)))
@@ -265,7 +265,7 @@ def large_b3_leg(lib, ctx, hip, steps, warmup):
                      "ms_per_step": r["total"] / steps * 1e3,
                      "stage_ms": {k: round(v[0] / steps, 3) for k, v in r["stage"].items()}}
     m64, mch = res["fp64"]["mean"], res["chosen"]["mean"]
-    out["workload"] = (f"{n} atoms, 3-species BCC alloy a = 3.2 A (32^3 cells); B3 [3,6,4] "
+    out["workload"] = (f"{n} atoms, 3-species BCC alloy a = 3.2 A (128x64x64 cells); B3 [3,6,4] "
                        f"rc 5.5 (d={calc.n_descriptors}); {sgp.n_sparse} sparse; NDP sigma 2 p 2")
     out["speedup_vs_fp64"] = out["chosen"]["atom_evals_per_s"] / out["fp64"]["atom_evals_per_s"]
     out["error_vs_fp64"] = {"energy_meV_per_atom": abs(m64[0] - mch[0]) / n * 1e3,

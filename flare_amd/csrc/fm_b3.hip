@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "fm_device.hpp"
@@ -74,7 +75,19 @@ struct B3Tables {
   std::vector<int> gls_off;    // n_ls 3 H + 1
   std::vector<int> gls_term;
   std::vector<int> tri_pk;     // n_tri: n1 | n2 << 8 | n3 << 16
+  // Radial pairs n1 <= n2, ordered by n2 then n1 (pair n2 (n2 + 1) / 2 + n1,
+  // so the pairs with n2 <= n form a prefix): n1 | n2 << 8, and the index
+  // of the triple (n1, n2, n2) (the triples (n1, n2, n3 >= n2) follow it).
+  int npair = 0;
+  std::vector<int> pair_pk, pair_tb;
+  // Dense weights per l-set, kLM x kLM each: wA[m1][m3] and wB[m2][m3] hold
+  // w3j of the term (m1, m2, m3) with m1 + m2 + m3 = l1 + l2 + l3 (0-based
+  // m within each l block), zero where there is none.
+  std::vector<double> wA, wB;
 };
+
+// Largest l block (2 l_max + 1, l_max <= 4 for B3).
+constexpr int kLM = 9;
 
 static void build_b3_tables(int nr, int lmax, B3Tables &t) {
   t.nr = nr;
@@ -106,6 +119,25 @@ static void build_b3_tables(int nr, int lmax, B3Tables &t) {
   t.nd = t.n_tri * t.n_ls;
   for (int k = 0; k < t.n_tri; k++)
     t.tri_pk.push_back(t.tri[3 * k] | t.tri[3 * k + 1] << 8 | t.tri[3 * k + 2] << 16);
+  t.npair = nr * (nr + 1) / 2;
+  for (int n2 = 0; n2 < nr; n2++)
+    for (int n1 = 0; n1 <= n2; n1++) {
+      t.pair_pk.push_back(n1 | n2 << 8);
+      // Triples of n1 are (n1, a, b), n1 <= a <= b < nr, a-major.
+      int k = t.tri_base[n1];
+      for (int a = n1; a < n2; a++) k += nr - a;
+      t.pair_tb.push_back(k);
+    }
+  t.wA.assign((size_t)t.n_ls * kLM * kLM, 0.0);
+  t.wB.assign((size_t)t.n_ls * kLM * kLM, 0.0);
+  for (int ls = 0; ls < t.n_ls; ls++)
+    for (int q = t.term_off[ls]; q < t.term_off[ls + 1]; q++) {
+      int l1 = t.ls_l[3 * ls], l2 = t.ls_l[3 * ls + 1], l3 = t.ls_l[3 * ls + 2];
+      int m1 = t.term_lm[3 * q] - l1 * l1, m2 = t.term_lm[3 * q + 1] - l2 * l2;
+      int m3 = t.term_lm[3 * q + 2] - l3 * l3;
+      t.wA[((size_t)ls * kLM + m1) * kLM + m3] = t.term_w[q];
+      t.wB[((size_t)ls * kLM + m2) * kLM + m3] = t.term_w[q];
+    }
   t.gls_off.push_back(0);
   for (int ls = 0; ls < t.n_ls; ls++) {
     std::vector<std::vector<int>> lists(3 * t.H);
@@ -138,6 +170,8 @@ struct B3Dev {
   const int *tri, *ls_l, *term_off, *term_lm, *gls_off, *gls_term, *tri_base, *tri_pk;
   const double *term_w;
   int nr, H, n_ls, n_tri, nd;
+  const int *pair_pk, *pair_tb;
+  int npair;
 };
 
 // Complex c of one atom in LDS from the real coefficients (interleaved
@@ -374,6 +408,350 @@ __global__ __launch_bounds__(256) void k_dEdc_b3(int n, B3Dev t, const T *Cg,
   }
 }
 
+// ---------------------------------------------------------------------------
+// Pair-factorized B3 (one workgroup per atom, one thread per radial pair).
+//
+// With the l-set fixed and m2 = l1 + l2 + l3 - m1 - m3 (0-based m), a term
+// is fixed by (m1, m3), so for each pair n1 <= n2
+//   P[m3] = sum_m1 wA[m1][m3] c[n1, l1 m1] c[n2, l2 m2]
+// and B(n1, n2, n3) = Re sum_m3 c[n3, l3 m3] P[m3] for n3 >= n2: the triple
+// sum over terms becomes a pair sum plus one dot product per triple. The
+// adjoint follows the same factors backwards: with w = dE/dB of the l-set,
+//   D[m3]   = sum_{n3 >= n2} w(n1, n2, n3) c[n3, l3 m3],
+//   dc[n3, l3 m3] += sum_{n1 <= n2 <= n3} w(n1, n2, n3) conj P(n1, n2)[m3],
+//   dc[n1, l1 m1] += sum_{n2 >= n1} sum_m3 wA[m1][m3] conj(c[n2, l2 m2] D[m3]),
+//   dc[n2, l2 m2] += sum_{n1 <= n2} sum_m3 wB[m2][m3] conj(c[n1, l1 m1] D[m3]).
+// The per-pair loops over (m1, m3) are unrolled over the largest l block
+// with the weight tables read at uniform addresses, so P, D and the partial
+// gradients stay in registers; the sums over pairs that end on a target
+// (n, m) run with one thread per target after a barrier. Summation orders
+// are fixed, so results are deterministic.
+// ---------------------------------------------------------------------------
+
+// Complex c of one atom into LDS, threads of the block striding.
+template <typename T>
+__device__ void complex_c_block(const T *cr, T *cc, int nr, int H, int tid, int nt) {
+  const T s2 = T(0.70710678118654752440);
+  for (int k = tid; k < nr * H; k += nt) {
+    int n = k / H, lm = k - n * H;
+    int l = 0;
+    while ((l + 1) * (l + 1) <= lm) l++;
+    int m = lm - l * l - l;
+    const T *row = cr + n * H + l * l + l;
+    T re, im;
+    if (m == 0) {
+      re = row[0];
+      im = 0;
+    } else if (m > 0) {
+      T sg = (m & 1) ? T(-1) : T(1);
+      re = sg * s2 * row[m];
+      im = sg * s2 * row[-m];
+    } else {
+      re = s2 * row[-m];
+      im = -s2 * row[m];
+    }
+    cc[2 * k] = re;
+    cc[2 * k + 1] = im;
+  }
+}
+
+// Sum over the block in a fixed order (red: one slot per wave).
+template <typename T>
+__device__ T block_sum(T v, T *red) {
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  v = wave_sum(v);
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  T s = 0;
+  for (int k = 0; k < nw; k++) s += red[k];
+  __syncthreads();
+  return s;
+}
+
+// P[m3] of one pair: x = c[n1, l1 block], y = c[n2, l2 block] (complex).
+template <typename T>
+__device__ __forceinline__ void pair_product(const T *__restrict__ wt, int S, int n1m,
+                                             const T *x, const T *y, T *pr, T *pi) {
+#pragma unroll
+  for (int m3 = 0; m3 < kLM; m3++) pr[m3] = pi[m3] = 0;
+#pragma unroll
+  for (int m1 = 0; m1 < kLM; m1++) {
+    if (m1 < n1m) {
+      T xr = x[2 * m1], xi = x[2 * m1 + 1];
+#pragma unroll
+      for (int m3 = 0; m3 < kLM; m3++) {
+        T w = wt[m1 * kLM + m3];
+        if (w != T(0)) {
+          int m2 = S - m1 - m3;
+          T yr = y[2 * m2], yi = y[2 * m2 + 1];
+          pr[m3] += w * (xr * yr - xi * yi);
+          pi[m3] += w * (xr * yi + xi * yr);
+        }
+      }
+    }
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(512) void k_b3p(int n, B3Dev t, const T *__restrict__ wA,
+                                              const T *Cg, const int *row_of_atom, RowRange rr,
+                                              T *B, double *norm, double *nnv, __bf16 *B0,
+                                              __bf16 *B1, int ldp) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  int k = blockIdx.x;
+  if (k >= n) return;
+  int tid = threadIdx.x, nt = blockDim.x;
+  int i, row, brow;
+  rr.map(k, row_of_atom, i, row, brow);
+  int nr = t.nr, H = t.H, nc = nr * H, n_tri = t.n_tri;
+  T *cc = reinterpret_cast<T *>(smem);
+  T *bl = cc + 2 * nc;    // one l-set's slice of the row
+  T *red = bl + n_tri;    // 16 wave slots
+  complex_c_block<T>(Cg + (size_t)i * nc, cc, nr, H, tid, nt);
+  bool act = tid < t.npair;
+  int n1 = 0, n2 = 0, tb = 0;
+  if (act) {
+    int pk = t.pair_pk[tid];
+    n1 = pk & 255;
+    n2 = pk >> 8;
+    tb = t.pair_tb[tid];
+  }
+  __syncthreads();
+  T s2 = 0;
+  T *Brow = B + (size_t)brow * t.nd;
+  for (int ls = 0; ls < t.n_ls; ls++) {
+    int l1 = t.ls_l[3 * ls], l2 = t.ls_l[3 * ls + 1], l3 = t.ls_l[3 * ls + 2];
+    if (act) {
+      T pr[kLM], pi[kLM];
+      pair_product<T>(wA + (size_t)ls * kLM * kLM, l1 + l2 + l3, 2 * l1 + 1,
+                      cc + 2 * (n1 * H + l1 * l1), cc + 2 * (n2 * H + l2 * l2), pr, pi);
+      int n3m = 2 * l3 + 1;
+      for (int n3 = n2; n3 < nr; n3++) {
+        const T *z = cc + 2 * (n3 * H + l3 * l3);
+        T acc = 0;
+#pragma unroll
+        for (int m3 = 0; m3 < kLM; m3++)
+          if (m3 < n3m) acc += z[2 * m3] * pr[m3] - z[2 * m3 + 1] * pi[m3];
+        bl[tb + n3 - n2] = acc;
+      }
+    }
+    __syncthreads();
+    for (int kk = tid; kk < n_tri; kk += nt) {
+      T v = bl[kk];
+      size_t kd = (size_t)ls * n_tri + kk;
+      Brow[kd] = v;
+      if (B0) {
+        float r = (float)v;
+        __bf16 h = (__bf16)r;
+        B0[(size_t)brow * ldp + kd] = h;
+        B1[(size_t)brow * ldp + kd] = (__bf16)(r - (float)h);
+      }
+      s2 += v * v;
+    }
+    __syncthreads();
+  }
+  if (B0)
+    for (int kd = t.nd + tid; kd < ldp; kd += nt) {
+      B0[(size_t)brow * ldp + kd] = (__bf16)0.0f;
+      B1[(size_t)brow * ldp + kd] = (__bf16)0.0f;
+    }
+  s2 = block_sum(s2, red);
+  if (tid == 0) {
+    norm[row] = (double)sqrt(s2);
+    nnv[row] = (double)s2;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(512) void k_dEdc_b3p(int n, B3Dev t, const T *__restrict__ wA,
+                                                   const T *__restrict__ wB, const T *Cg,
+                                                   const int *row_of_atom, RowRange rr,
+                                                   const T *B, const double *norm,
+                                                   const double *nnv, const T *G,
+                                                   long long g_ld, int project,
+                                                   double empty_norm, double empty_nn, T *Ug) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  int k = blockIdx.x;
+  if (k >= n) return;
+  int tid = threadIdx.x, nt = blockDim.x;
+  int i, row, brow;
+  rr.map(k, row_of_atom, i, row, brow);
+  int nr = t.nr, H = t.H, nc = nr * H, n_tri = t.n_tri, npair = t.npair;
+  T *urow = Ug + (size_t)i * nc;
+  double nrm = norm[row];
+  if (nrm < empty_norm || nnv[row] < empty_nn) {
+    for (int g = tid; g < nc; g += nt) urow[g] = 0;
+    return;
+  }
+  T *cc = reinterpret_cast<T *>(smem);
+  T *gc = cc + 2 * nc;                       // complex gradient per (n, lm)
+  T *wl = gc + 2 * nc;                       // w of one l-set
+  T *Pl = wl + n_tri;                        // P[m3][pair]
+  T *El = Pl + 2 * (size_t)kLM * npair;      // partial gradients [m][pair]
+  T *red = El + 2 * (size_t)kLM * npair;     // 16 wave slots
+  complex_c_block<T>(Cg + (size_t)i * nc, cc, nr, H, tid, nt);
+  for (int g = tid; g < 2 * nc; g += nt) gc[g] = 0;
+  const T *Brow = B + (size_t)brow * t.nd;
+  const T *Grow = G + (size_t)brow * g_ld;
+  T ninv = T(1) / (T)nrm, gb = 0;
+  if (project) {
+    T part = 0;
+    for (int kk = tid; kk < t.nd; kk += nt) part += Grow[kk] * Brow[kk];
+    gb = block_sum(part, red) * ninv;
+  }
+  bool act = tid < npair;
+  int n1 = 0, n2 = 0, tb = 0;
+  if (act) {
+    int pk = t.pair_pk[tid];
+    n1 = pk & 255;
+    n2 = pk >> 8;
+    tb = t.pair_tb[tid];
+  }
+  for (int ls = 0; ls < t.n_ls; ls++) {
+    int l1 = t.ls_l[3 * ls], l2 = t.ls_l[3 * ls + 1], l3 = t.ls_l[3 * ls + 2];
+    int S = l1 + l2 + l3, n1m = 2 * l1 + 1, n2m = 2 * l2 + 1, n3m = 2 * l3 + 1;
+    const T *gs = Grow + (size_t)ls * n_tri, *bs = Brow + (size_t)ls * n_tri;
+    for (int kk = tid; kk < n_tri; kk += nt)
+      wl[kk] = project ? (gs[kk] - gb * bs[kk] * ninv) * ninv : gs[kk];
+    __syncthreads();
+    const T *wa = wA + (size_t)ls * kLM * kLM, *wb = wB + (size_t)ls * kLM * kLM;
+    const T *x = cc + 2 * (n1 * H + l1 * l1), *y = cc + 2 * (n2 * H + l2 * l2);
+    T er2[kLM], ei2[kLM];
+    if (act) {
+      T pr[kLM], pi[kLM];
+      pair_product<T>(wa, S, n1m, x, y, pr, pi);
+#pragma unroll
+      for (int m3 = 0; m3 < kLM; m3++)
+        if (m3 < n3m) {
+          Pl[2 * (m3 * npair + tid)] = pr[m3];
+          Pl[2 * (m3 * npair + tid) + 1] = pi[m3];
+        }
+      T dr[kLM], di[kLM];
+#pragma unroll
+      for (int m3 = 0; m3 < kLM; m3++) dr[m3] = di[m3] = 0;
+      for (int n3 = n2; n3 < nr; n3++) {
+        T wv = wl[tb + n3 - n2];
+        const T *z = cc + 2 * (n3 * H + l3 * l3);
+#pragma unroll
+        for (int m3 = 0; m3 < kLM; m3++)
+          if (m3 < n3m) {
+            dr[m3] += wv * z[2 * m3];
+            di[m3] += wv * z[2 * m3 + 1];
+          }
+      }
+      // Factor 1 (target c[n1, l1 m1]): conj(c[n2, l2 m2] D[m3]).
+#pragma unroll
+      for (int m1 = 0; m1 < kLM; m1++) {
+        if (m1 < n1m) {
+          T er = 0, ei = 0;
+#pragma unroll
+          for (int m3 = 0; m3 < kLM; m3++) {
+            T w = wa[m1 * kLM + m3];
+            if (w != T(0)) {
+              int m2 = S - m1 - m3;
+              T yr = y[2 * m2], yi = y[2 * m2 + 1];
+              er += w * (yr * dr[m3] - yi * di[m3]);
+              ei -= w * (yr * di[m3] + yi * dr[m3]);
+            }
+          }
+          El[2 * (m1 * npair + tid)] = er;
+          El[2 * (m1 * npair + tid) + 1] = ei;
+        }
+      }
+      // Factor 2 (target c[n2, l2 m2]): conj(c[n1, l1 m1] D[m3]), kept in
+      // registers until the factor-1 sums have read El.
+#pragma unroll
+      for (int m2 = 0; m2 < kLM; m2++) {
+        er2[m2] = ei2[m2] = 0;
+        if (m2 < n2m) {
+#pragma unroll
+          for (int m3 = 0; m3 < kLM; m3++) {
+            T w = wb[m2 * kLM + m3];
+            if (w != T(0)) {
+              int m1 = S - m2 - m3;
+              T xr = x[2 * m1], xi = x[2 * m1 + 1];
+              er2[m2] += w * (xr * dr[m3] - xi * di[m3]);
+              ei2[m2] -= w * (xr * di[m3] + xi * dr[m3]);
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+    // Factor 1 targets (n, l1 m1): pairs (n, n2 >= n).
+    for (int tg = tid; tg < nr * n1m; tg += nt) {
+      int nn = tg / n1m, m1 = tg - nn * n1m;
+      T ar = 0, ai = 0;
+      for (int b = nn; b < nr; b++) {
+        const T *e = El + 2 * (m1 * npair + b * (b + 1) / 2 + nn);
+        ar += e[0];
+        ai += e[1];
+      }
+      gc[2 * (nn * H + l1 * l1 + m1)] += ar;
+      gc[2 * (nn * H + l1 * l1 + m1) + 1] += ai;
+    }
+    __syncthreads();
+    // Factor 3 targets (n, l3 m3): the pairs with n2 <= n, a prefix.
+    for (int tg = tid; tg < nr * n3m; tg += nt) {
+      int nn = tg / n3m, m3 = tg - nn * n3m;
+      int np = (nn + 1) * (nn + 2) / 2;
+      T ar = 0, ai = 0;
+      for (int q = 0; q < np; q++) {
+        int b = t.pair_pk[q] >> 8;
+        T wv = wl[t.pair_tb[q] + nn - b];
+        const T *p = Pl + 2 * (m3 * npair + q);
+        ar += wv * p[0];
+        ai -= wv * p[1];
+      }
+      gc[2 * (nn * H + l3 * l3 + m3)] += ar;
+      gc[2 * (nn * H + l3 * l3 + m3) + 1] += ai;
+    }
+    if (act) {
+#pragma unroll
+      for (int m2 = 0; m2 < kLM; m2++)
+        if (m2 < n2m) {
+          El[2 * (m2 * npair + tid)] = er2[m2];
+          El[2 * (m2 * npair + tid) + 1] = ei2[m2];
+        }
+    }
+    __syncthreads();
+    // Factor 2 targets (n, l2 m2): pairs (n1 <= n, n), contiguous.
+    for (int tg = tid; tg < nr * n2m; tg += nt) {
+      int nn = tg / n2m, m2 = tg - nn * n2m;
+      T ar = 0, ai = 0;
+      const T *e = El + 2 * (m2 * npair + nn * (nn + 1) / 2);
+      for (int a = 0; a <= nn; a++) {
+        ar += e[2 * a];
+        ai += e[2 * a + 1];
+      }
+      gc[2 * (nn * H + l2 * l2 + m2)] += ar;
+      gc[2 * (nn * H + l2 * l2 + m2) + 1] += ai;
+    }
+    __syncthreads();
+  }
+  // u[n, l, m] = d E / d c^r: transpose of the complex map.
+  const T s2 = T(0.70710678118654752440);
+  for (int g = tid; g < nc; g += nt) {
+    int nn = g / H, lm = g - nn * H;
+    int l = 0;
+    while ((l + 1) * (l + 1) <= lm) l++;
+    int m = lm - l * l - l;
+    const T *gr = gc + 2 * (nn * H + l * l + l);
+    T u;
+    if (m == 0) {
+      u = gr[0];
+    } else {
+      int mm = m > 0 ? m : -m;
+      T sg = (mm & 1) ? T(-1) : T(1);
+      if (m > 0)
+        u = s2 * (sg * gr[2 * mm] + gr[-2 * mm]);
+      else
+        u = s2 * (sg * gr[2 * mm + 1] - gr[-2 * mm + 1]);
+    }
+    urow[g] = u;
+  }
+}
+
 // Column / row permutations between the reference's B3 order and the
 // l-set-major one: out[r][k'] = in[r][src(k')] for k' < nd (columns past nd,
 // the padding, are copied), or out row k' = in row src(k'), with
@@ -412,11 +790,24 @@ __global__ void k_b3_permute_rows(int nd, long long ld, int n_tri, int n_ls, int
 struct B3State {
   B3Tables t;
   DevBuf tri, tri_base, ls_l, term_off, term_lm, term_w, gls_off, gls_term, tri_pk;
+  DevBuf pair_pk, pair_tb, wA64, wB64, wA32, wB32;
   B3Dev dev() const {
     return B3Dev{tri.as<int>(),      ls_l.as<int>(),     term_off.as<int>(),
                  term_lm.as<int>(),  gls_off.as<int>(),  gls_term.as<int>(),
                  tri_base.as<int>(), tri_pk.as<int>(),   term_w.as<double>(),
-                 t.nr, t.H, t.n_ls, t.n_tri, t.nd};
+                 t.nr,               t.H,                t.n_ls,
+                 t.n_tri,            t.nd,               pair_pk.as<int>(),
+                 pair_tb.as<int>(),  t.npair};
+  }
+  template <typename T>
+  const T *wA() const {
+    return std::is_same<T, double>::value ? (const T *)wA64.as<double>()
+                                          : (const T *)wA32.as<float>();
+  }
+  template <typename T>
+  const T *wB() const {
+    return std::is_same<T, double>::value ? (const T *)wB64.as<double>()
+                                          : (const T *)wB32.as<float>();
   }
 };
 
@@ -440,6 +831,13 @@ static B3State *b3_state(fm_structure s) {
     up(st->gls_off, t.gls_off.data(), t.gls_off.size() * 4);
     up(st->gls_term, t.gls_term.data(), t.gls_term.size() * 4);
     up(st->tri_pk, t.tri_pk.data(), t.tri_pk.size() * 4);
+    up(st->pair_pk, t.pair_pk.data(), t.pair_pk.size() * 4);
+    up(st->pair_tb, t.pair_tb.data(), t.pair_tb.size() * 4);
+    up(st->wA64, t.wA.data(), t.wA.size() * 8);
+    up(st->wB64, t.wB.data(), t.wB.size() * 8);
+    std::vector<float> wa(t.wA.begin(), t.wA.end()), wb(t.wB.begin(), t.wB.end());
+    up(st->wA32, wa.data(), wa.size() * 4);
+    up(st->wB32, wb.data(), wb.size() * 4);
     FM_HIP(hipStreamSynchronize(stream));
     s->b3 = st;
   }
@@ -531,6 +929,19 @@ static int waves_per_block(size_t lds_per_atom) {
   return apb;
 }
 
+// Workgroup size and LDS bytes of the pair kernels, or 0 when the pairs do
+// not fit one workgroup of 512 threads or the workspace does not fit LDS
+// (then the wave-per-atom kernels run).
+static size_t b3_pair_lds(const B3Dev &d, size_t esz, bool adjoint, int &nt) {
+  if (d.npair > 512) return 0;
+  nt = std::max(64, (d.npair + 63) / 64 * 64);
+  size_t nc = (size_t)d.nr * d.H;
+  size_t el = 2 * nc + d.n_tri + 16;
+  if (adjoint) el += 2 * nc + 4 * (size_t)kLM * d.npair;
+  size_t bytes = el * esz;
+  return bytes <= 160 * 1024 ? bytes : 0;
+}
+
 void launch_b3(fm_structure s) {
   launch_b3_rows(s, 0, s->n, false, s->B.as<void>(), s->Bs[0].as<void>(), s->Bs[1].as<void>());
 }
@@ -543,6 +954,22 @@ void launch_b3_rows(fm_structure s, int r0, int n, bool tile, void *Bout, void *
   RowRange rr{r0, tile ? s->atom_of_row.as<int>() : nullptr};
   bool f64 = s->form == FM_FORM_FP64;
   size_t esz = f64 ? 8 : 4;
+  int nt = 0;
+  size_t lds = b3_pair_lds(d, esz, false, nt);
+  if (lds) {
+    if (f64)
+      hipLaunchKernelGGL(k_b3p<double>, dim3(n), dim3(nt), lds, st, n, d, b->wA<double>(),
+                         s->C.as<double>(), s->row_of_atom.as<int>(), rr, (double *)Bout,
+                         s->norm.as<double>(), s->nn.as<double>(), (__bf16 *)nullptr,
+                         (__bf16 *)nullptr, 0);
+    else
+      hipLaunchKernelGGL(k_b3p<float>, dim3(n), dim3(nt), lds, st, n, d, b->wA<float>(),
+                         s->C.as<float>(), s->row_of_atom.as<int>(), rr, (float *)Bout,
+                         s->norm.as<double>(), s->nn.as<double>(), (__bf16 *)B0, (__bf16 *)B1,
+                         (int)split_ld(s->nd));
+    FM_HIP(hipGetLastError());
+    return;
+  }
   size_t per = 2 * (size_t)d.nr * d.H * esz;
   int apb = waves_per_block(per);
   int nblk = (n + apb - 1) / apb;
@@ -575,6 +1002,24 @@ void launch_dEdc_b3_rows(fm_structure s, int r0, int n, bool tile, const void *B
   RowRange rr{r0, tile ? s->atom_of_row.as<int>() : nullptr};
   bool f64 = s->form == FM_FORM_FP64;
   size_t esz = f64 ? 8 : 4;
+  int nt = 0;
+  size_t plds = b3_pair_lds(d, esz, true, nt);
+  if (plds) {
+    if (f64)
+      hipLaunchKernelGGL(k_dEdc_b3p<double>, dim3(n), dim3(nt), plds, st, n, d,
+                         b->wA<double>(), b->wB<double>(), s->C.as<double>(),
+                         s->row_of_atom.as<int>(), rr, (const double *)B, s->norm.as<double>(),
+                         s->nn.as<double>(), (const double *)G, (long long)g_ld, project,
+                         empty_norm, empty_nn, (double *)U);
+    else
+      hipLaunchKernelGGL(k_dEdc_b3p<float>, dim3(n), dim3(nt), plds, st, n, d, b->wA<float>(),
+                         b->wB<float>(), s->C.as<float>(), s->row_of_atom.as<int>(), rr,
+                         (const float *)B, s->norm.as<double>(), s->nn.as<double>(),
+                         (const float *)G, (long long)g_ld, project, empty_norm, empty_nn,
+                         (float *)U);
+    FM_HIP(hipGetLastError());
+    return;
+  }
   int nc = d.nr * d.H;
   FM_CHECK(d.nr <= 64, FM_ERR_INVALID, "B3 adjoint: n_species n_max above 64");
   size_t per = (4 * (size_t)nc + (size_t)d.n_tri) * esz;

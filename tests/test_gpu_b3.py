@@ -126,3 +126,41 @@ def test_b3_row_tiles(C, oracle, system, form):
         assert np.array_equal(s.local_uncertainties[0], whole.local_uncertainties[0]), rows
         with pytest.raises(RuntimeError, match="row tiles"):
             s.descriptors[0].descriptors
+
+
+@pytest.mark.parametrize("nos,nmax,n_atoms", [(3, 6, 14), (4, 7, 9)])
+def test_b3_lmax4_settings(C, oracle, nos, nmax, n_atoms):
+    """l_max = 4 B3 against the oracle: [3, 6, 4] is the setting of the
+    million-atom alloy (18 radial channels); [4, 7, 4] (28 channels) is wide
+    enough that the FP64 adjoint's workspace exceeds LDS, so that form runs
+    the wave-per-atom adjoint while FP32 keeps the pair-factorized one (its
+    second species has no atoms)."""
+    say("B3 l_max 4 settings", nos, nmax, n_atoms)
+    rc = 4.4
+    cut = np.full((nos, nos), rc)
+    cell, species, pos = random_cell(n_atoms, 7 + nos, nos=nos, scale=7.0, skew=0.3)
+    calc = C.B3("chebyshev", "quadratic", [0.0, rc], [], [nos, nmax, 4], cut)
+    ref = oracle.B3Values(oracle.Neighbors(cell, species, pos, rc, "literal"), nos, nmax, 4, cut)
+    picks = [list(range(min(3, ref.n_clusters_by_type[t]))) for t in range(nos)]
+    X, N, alpha, Linv = model_state(ref, picks, 1.1, 2.0, seed=5)
+    m = oracle.Model(1.1, 2.0, X, N, alpha, L_inv=Linv)
+    s0 = C.Structure(cell, species, pos, rc, [calc])
+    sgp = C.SparseGP([C.NormalizedDotProduct(1.1, 2.0)], 0.01, 0.05, 0.005)
+    sgp.add_specific_environments(s0, [int(ref.atom_indices[t][k]) for t in range(nos)
+                                       for k in picks[t]])
+    sgp.set_state(alpha, L_inv=Linv)
+    for t in range(nos):
+        assert rel_err(s0.descriptors[0].descriptors[t], ref.descriptors[t]) < TOL
+    s = C.Structure(cell, species, pos, rc, [calc])
+    sgp.predict_local_uncertainties(s)
+    mean, var = m.predict(ref, "local")
+    n = s.noa
+    assert abs(s.mean_efs[0] - mean[0]) / abs(mean[0]) < TOL
+    assert rel_err(s.mean_efs[1:1 + 3 * n], mean[1:1 + 3 * n]) < TOL
+    assert rel_err(s.mean_efs[-6:], mean[-6:]) < TOL
+    assert var_err(s.local_uncertainties[0], var, 1.1 ** 2) < TOL
+    s32 = C.Structure(cell, species, pos, rc, [calc], form=1)
+    sgp.predict_mean(s32)
+    dF = np.abs(s32.mean_efs[1:1 + 3 * n] - s.mean_efs[1:1 + 3 * n]).max()
+    say(f"  chosen form dF {dF * 1e3:.3e} meV/A")
+    assert dF < 1e-3
