@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <complex>
 #include <type_traits>
 #include <vector>
 
@@ -80,14 +81,22 @@ struct B3Tables {
   // of the triple (n1, n2, n2) (the triples (n1, n2, n3 >= n2) follow it).
   int npair = 0;
   std::vector<int> pair_pk, pair_tb;
-  // Dense weights per l-set, kLM x kLM each: wA[m1][m3] and wB[m2][m3] hold
-  // w3j of the term (m1, m2, m3) with m1 + m2 + m3 = l1 + l2 + l3 (0-based
-  // m within each l block), zero where there is none.
-  std::vector<double> wA, wB;
+  // Couplings of the real coefficients per l-set: with c = U r the complex
+  // coefficients of the real ones r (the map in the header comment),
+  //   B3 = sum C[mu1, mu2, mu3] r[n1, l1 mu1] r[n2, l2 mu2] r[n3, l3 mu3],
+  //   C = Re sum_terms w3j U_l1[m1, mu1] U_l2[m2, mu2] U_l3[m3, mu3]
+  // (B3 is the real part and r is real). Nonzero entries of l-set ls are
+  // [rt_off[ls], rt_off[ls + 1]): mu1 | mu2 << 4 | mu3 << 8 (0-based within
+  // each l block) and the value.
+  std::vector<int> rt_off, rt_pk;
+  std::vector<double> rt_w;
 };
 
 // Largest l block (2 l_max + 1, l_max <= 4 for B3).
 constexpr int kLM = 9;
+// Triples per thread of one l-set in the pair kernels: n_tri / n_pair =
+// (n + 2) / 3 <= 11 for the n <= 31 radial channels they take.
+constexpr int kPre = 12;
 
 static void build_b3_tables(int nr, int lmax, B3Tables &t) {
   t.nr = nr;
@@ -128,16 +137,48 @@ static void build_b3_tables(int nr, int lmax, B3Tables &t) {
       for (int a = n1; a < n2; a++) k += nr - a;
       t.pair_tb.push_back(k);
     }
-  t.wA.assign((size_t)t.n_ls * kLM * kLM, 0.0);
-  t.wB.assign((size_t)t.n_ls * kLM * kLM, 0.0);
-  for (int ls = 0; ls < t.n_ls; ls++)
-    for (int q = t.term_off[ls]; q < t.term_off[ls + 1]; q++) {
-      int l1 = t.ls_l[3 * ls], l2 = t.ls_l[3 * ls + 1], l3 = t.ls_l[3 * ls + 2];
-      int m1 = t.term_lm[3 * q] - l1 * l1, m2 = t.term_lm[3 * q + 1] - l2 * l2;
-      int m3 = t.term_lm[3 * q + 2] - l3 * l3;
-      t.wA[((size_t)ls * kLM + m1) * kLM + m3] = t.term_w[q];
-      t.wB[((size_t)ls * kLM + m2) * kLM + m3] = t.term_w[q];
+  // Row k = m + l of U_l: the complex c_l^m as a combination of the real
+  // r_l,mu (at most two entries; index mu + l).
+  auto urow = [](int l, int k, int *mu, std::complex<double> *v) {
+    const double s2 = 0.70710678118654752440;
+    int m = k - l;
+    if (m == 0) {
+      mu[0] = l, v[0] = 1.0, mu[1] = -1;
+    } else if (m > 0) {
+      double sg = (m & 1) ? -1.0 : 1.0;
+      mu[0] = m + l, v[0] = sg * s2;
+      mu[1] = -m + l, v[1] = std::complex<double>(0.0, sg * s2);
+    } else {
+      mu[0] = -m + l, v[0] = s2;
+      mu[1] = m + l, v[1] = std::complex<double>(0.0, -s2);
     }
+  };
+  t.rt_off.push_back(0);
+  for (int ls = 0; ls < t.n_ls; ls++) {
+    int l[3] = {t.ls_l[3 * ls], t.ls_l[3 * ls + 1], t.ls_l[3 * ls + 2]};
+    std::vector<std::complex<double>> T((size_t)kLM * kLM * kLM, 0.0);
+    for (int q = t.term_off[ls]; q < t.term_off[ls + 1]; q++) {
+      int mu[3][2];
+      std::complex<double> v[3][2];
+      for (int p = 0; p < 3; p++) urow(l[p], t.term_lm[3 * q + p] - l[p] * l[p], mu[p], v[p]);
+      for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 2; j++)
+          for (int k = 0; k < 2; k++) {
+            if (mu[0][i] < 0 || mu[1][j] < 0 || mu[2][k] < 0) continue;
+            T[((size_t)mu[0][i] * kLM + mu[1][j]) * kLM + mu[2][k]] +=
+                t.term_w[q] * v[0][i] * v[1][j] * v[2][k];
+          }
+    }
+    for (int a = 0; a < kLM; a++)
+      for (int b = 0; b < kLM; b++)
+        for (int c = 0; c < kLM; c++) {
+          double w = T[((size_t)a * kLM + b) * kLM + c].real();
+          if (std::abs(w) < 1e-13) continue;  // cancelled to rounding
+          t.rt_pk.push_back(a | b << 4 | c << 8);
+          t.rt_w.push_back(w);
+        }
+    t.rt_off.push_back((int)t.rt_w.size());
+  }
   t.gls_off.push_back(0);
   for (int ls = 0; ls < t.n_ls; ls++) {
     std::vector<std::vector<int>> lists(3 * t.H);
@@ -409,51 +450,23 @@ __global__ __launch_bounds__(256) void k_dEdc_b3(int n, B3Dev t, const T *Cg,
 }
 
 // ---------------------------------------------------------------------------
-// Pair-factorized B3 (one workgroup per atom, one thread per radial pair).
+// Pair-factorized B3 on the real coefficients (one workgroup per atom, one
+// thread per radial pair n1 <= n2).
 //
-// With the l-set fixed and m2 = l1 + l2 + l3 - m1 - m3 (0-based m), a term
-// is fixed by (m1, m3), so for each pair n1 <= n2
-//   P[m3] = sum_m1 wA[m1][m3] c[n1, l1 m1] c[n2, l2 m2]
-// and B(n1, n2, n3) = Re sum_m3 c[n3, l3 m3] P[m3] for n3 >= n2: the triple
-// sum over terms becomes a pair sum plus one dot product per triple. The
-// adjoint follows the same factors backwards: with w = dE/dB of the l-set,
-//   D[m3]   = sum_{n3 >= n2} w(n1, n2, n3) c[n3, l3 m3],
-//   dc[n3, l3 m3] += sum_{n1 <= n2 <= n3} w(n1, n2, n3) conj P(n1, n2)[m3],
-//   dc[n1, l1 m1] += sum_{n2 >= n1} sum_m3 wA[m1][m3] conj(c[n2, l2 m2] D[m3]),
-//   dc[n2, l2 m2] += sum_{n1 <= n2} sum_m3 wB[m2][m3] conj(c[n1, l1 m1] D[m3]).
-// The per-pair loops over (m1, m3) are unrolled over the largest l block
-// with the weight tables read at uniform addresses, so P, D and the partial
-// gradients stay in registers; the sums over pairs that end on a target
-// (n, m) run with one thread per target after a barrier. Summation orders
-// are fixed, so results are deterministic.
+// With C the real couplings of the l-set (B3Tables::rt_*), for each pair
+//   P[mu3] = sum_{(mu1, mu2, mu3) in C} C r[n1, l1 mu1] r[n2, l2 mu2]
+// and B(n1, n2, n3) = sum_mu3 r[n3, l3 mu3] P[mu3] for n3 >= n2: the triple
+// sum over couplings becomes a pair sum plus one dot product per triple.
+// The adjoint runs the same factors backwards: with w = dE/dB of the l-set,
+//   D[mu3] = sum_{n3 >= n2} w(n1, n2, n3) r[n3, l3 mu3],
+//   u[n3, l3 mu3] += sum_{n1 <= n2 <= n3} w(n1, n2, n3) P(n1, n2)[mu3],
+//   u[n1, l1 mu1] += sum_{n2 >= n1} sum_C C r[n2, l2 mu2] D[mu3],
+//   u[n2, l2 mu2] += sum_{n1 <= n2} sum_C C r[n1, l1 mu1] D[mu3].
+// The coupling list is read at uniform addresses and indexes small register
+// arrays (uniform indices, so no scratch); the sums over pairs that end on
+// one target (n, mu) run with one thread per target after a barrier.
+// Summation orders are fixed, so results are deterministic.
 // ---------------------------------------------------------------------------
-
-// Complex c of one atom into LDS, threads of the block striding.
-template <typename T>
-__device__ void complex_c_block(const T *cr, T *cc, int nr, int H, int tid, int nt) {
-  const T s2 = T(0.70710678118654752440);
-  for (int k = tid; k < nr * H; k += nt) {
-    int n = k / H, lm = k - n * H;
-    int l = 0;
-    while ((l + 1) * (l + 1) <= lm) l++;
-    int m = lm - l * l - l;
-    const T *row = cr + n * H + l * l + l;
-    T re, im;
-    if (m == 0) {
-      re = row[0];
-      im = 0;
-    } else if (m > 0) {
-      T sg = (m & 1) ? T(-1) : T(1);
-      re = sg * s2 * row[m];
-      im = sg * s2 * row[-m];
-    } else {
-      re = s2 * row[-m];
-      im = -s2 * row[m];
-    }
-    cc[2 * k] = re;
-    cc[2 * k + 1] = im;
-  }
-}
 
 // Sum over the block in a fixed order (red: one slot per wave).
 template <typename T>
@@ -468,34 +481,19 @@ __device__ T block_sum(T v, T *red) {
   return s;
 }
 
-// P[m3] of one pair: x = c[n1, l1 block], y = c[n2, l2 block] (complex).
+// One l block of r (2 l + 1 values) into registers.
 template <typename T>
-__device__ __forceinline__ void pair_product(const T *__restrict__ wt, int S, int n1m,
-                                             const T *x, const T *y, T *pr, T *pi) {
+__device__ __forceinline__ void load_block(const T *r, int nm, T *v) {
 #pragma unroll
-  for (int m3 = 0; m3 < kLM; m3++) pr[m3] = pi[m3] = 0;
-#pragma unroll
-  for (int m1 = 0; m1 < kLM; m1++) {
-    if (m1 < n1m) {
-      T xr = x[2 * m1], xi = x[2 * m1 + 1];
-#pragma unroll
-      for (int m3 = 0; m3 < kLM; m3++) {
-        T w = wt[m1 * kLM + m3];
-        if (w != T(0)) {
-          int m2 = S - m1 - m3;
-          T yr = y[2 * m2], yi = y[2 * m2 + 1];
-          pr[m3] += w * (xr * yr - xi * yi);
-          pi[m3] += w * (xr * yi + xi * yr);
-        }
-      }
-    }
-  }
+  for (int k = 0; k < kLM; k++) v[k] = k < nm ? r[k] : T(0);
 }
 
 template <typename T>
-__global__ __launch_bounds__(512) void k_b3p(int n, B3Dev t, const T *__restrict__ wA,
-                                              const T *Cg, const int *row_of_atom, RowRange rr,
-                                              T *B, double *norm, double *nnv, __bf16 *B0,
+__global__ __launch_bounds__(512) void k_b3p(int n, B3Dev t, const int *__restrict__ rt_off,
+                                              const int *__restrict__ rt_pk,
+                                              const T *__restrict__ rt_w, const T *Cg,
+                                              const int *row_of_atom, RowRange rr, T *B,
+                                              double *norm, double *nnv, __bf16 *B0,
                                               __bf16 *B1, int ldp) {
   extern __shared__ __align__(16) unsigned char smem[];
   int k = blockIdx.x;
@@ -504,10 +502,10 @@ __global__ __launch_bounds__(512) void k_b3p(int n, B3Dev t, const T *__restrict
   int i, row, brow;
   rr.map(k, row_of_atom, i, row, brow);
   int nr = t.nr, H = t.H, nc = nr * H, n_tri = t.n_tri;
-  T *cc = reinterpret_cast<T *>(smem);
-  T *bl = cc + 2 * nc;    // one l-set's slice of the row
-  T *red = bl + n_tri;    // 16 wave slots
-  complex_c_block<T>(Cg + (size_t)i * nc, cc, nr, H, tid, nt);
+  T *rc = reinterpret_cast<T *>(smem);  // r of the atom
+  T *bl = rc + nc;                        // one l-set's slice of the row
+  T *red = bl + n_tri;                    // 16 wave slots
+  for (int g = tid; g < nc; g += nt) rc[g] = Cg[(size_t)i * nc + g];
   bool act = tid < t.npair;
   int n1 = 0, n2 = 0, tb = 0;
   if (act) {
@@ -522,16 +520,24 @@ __global__ __launch_bounds__(512) void k_b3p(int n, B3Dev t, const T *__restrict
   for (int ls = 0; ls < t.n_ls; ls++) {
     int l1 = t.ls_l[3 * ls], l2 = t.ls_l[3 * ls + 1], l3 = t.ls_l[3 * ls + 2];
     if (act) {
-      T pr[kLM], pi[kLM];
-      pair_product<T>(wA + (size_t)ls * kLM * kLM, l1 + l2 + l3, 2 * l1 + 1,
-                      cc + 2 * (n1 * H + l1 * l1), cc + 2 * (n2 * H + l2 * l2), pr, pi);
+      T x[kLM], y[kLM], p[kLM];
+      load_block<T>(rc + n1 * H + l1 * l1, 2 * l1 + 1, x);
+      load_block<T>(rc + n2 * H + l2 * l2, 2 * l2 + 1, y);
+#pragma unroll
+      for (int m = 0; m < kLM; m++) p[m] = 0;
+      int q1 = rt_off[ls + 1];
+      for (int q = rt_off[ls]; q < q1; q++) {
+        int c = rt_pk[q];
+        T w = rt_w[q];
+        p[c >> 8] += w * x[c & 15] * y[(c >> 4) & 15];
+      }
       int n3m = 2 * l3 + 1;
       for (int n3 = n2; n3 < nr; n3++) {
-        const T *z = cc + 2 * (n3 * H + l3 * l3);
+        const T *z = rc + n3 * H + l3 * l3;
         T acc = 0;
 #pragma unroll
-        for (int m3 = 0; m3 < kLM; m3++)
-          if (m3 < n3m) acc += z[2 * m3] * pr[m3] - z[2 * m3 + 1] * pi[m3];
+        for (int m = 0; m < kLM; m++)
+          if (m < n3m) acc += z[m] * p[m];
         bl[tb + n3 - n2] = acc;
       }
     }
@@ -563,8 +569,9 @@ __global__ __launch_bounds__(512) void k_b3p(int n, B3Dev t, const T *__restrict
 }
 
 template <typename T>
-__global__ __launch_bounds__(512) void k_dEdc_b3p(int n, B3Dev t, const T *__restrict__ wA,
-                                                   const T *__restrict__ wB, const T *Cg,
+__global__ __launch_bounds__(512) void k_dEdc_b3p(int n, B3Dev t, const int *__restrict__ rt_off,
+                                                   const int *__restrict__ rt_pk,
+                                                   const T *__restrict__ rt_w, const T *Cg,
                                                    const int *row_of_atom, RowRange rr,
                                                    const T *B, const double *norm,
                                                    const double *nnv, const T *G,
@@ -583,22 +590,42 @@ __global__ __launch_bounds__(512) void k_dEdc_b3p(int n, B3Dev t, const T *__res
     for (int g = tid; g < nc; g += nt) urow[g] = 0;
     return;
   }
-  T *cc = reinterpret_cast<T *>(smem);
-  T *gc = cc + 2 * nc;                       // complex gradient per (n, lm)
-  T *wl = gc + 2 * nc;                       // w of one l-set
-  T *Pl = wl + n_tri;                        // P[m3][pair]
-  T *El = Pl + 2 * (size_t)kLM * npair;      // partial gradients [m][pair]
-  T *red = El + 2 * (size_t)kLM * npair;     // 16 wave slots
-  complex_c_block<T>(Cg + (size_t)i * nc, cc, nr, H, tid, nt);
-  for (int g = tid; g < 2 * nc; g += nt) gc[g] = 0;
+  T *rc = reinterpret_cast<T *>(smem);  // r of the atom
+  T *gc = rc + nc;                        // dE/dr
+  T *wl = gc + nc;                        // w of one l-set
+  T *Pl = wl + n_tri;                     // P, then Q: [mu][pair]
+  T *El = Pl + (size_t)kLM * npair;       // factor-1, then factor-2 partials
+  T *red = El + (size_t)kLM * npair;      // 16 wave slots
+  for (int g = tid; g < nc; g += nt) {
+    rc[g] = Cg[(size_t)i * nc + g];
+    gc[g] = 0;
+  }
   const T *Brow = B + (size_t)brow * t.nd;
   const T *Grow = G + (size_t)brow * g_ld;
   T ninv = T(1) / (T)nrm, gb = 0;
   if (project) {
     T part = 0;
+#pragma unroll 16
     for (int kk = tid; kk < t.nd; kk += nt) part += Grow[kk] * Brow[kk];
     gb = block_sum(part, red) * ninv;
   }
+  // FP32: the next l-set's slices of dE/dB and B are loaded into registers
+  // while the current l-set is processed (each thread holds at most kPre).
+  // FP64 reads them when staging (its registers go to the coupling sweep).
+  constexpr int NQ = sizeof(T) == 4 ? kPre : 1;
+  T gq[NQ], bq[NQ];
+  auto fetch = [&](int ls) {
+    if constexpr (sizeof(T) == 4) {
+      const T *gs = Grow + (size_t)ls * n_tri, *bs = Brow + (size_t)ls * n_tri;
+#pragma unroll
+      for (int j = 0; j < NQ; j++) {
+        int kk = tid + j * nt;
+        gq[j] = kk < n_tri ? gs[kk] : T(0);
+        bq[j] = (project && kk < n_tri) ? bs[kk] : T(0);
+      }
+    }
+  };
+  fetch(0);
   bool act = tid < npair;
   int n1 = 0, n2 = 0, tb = 0;
   if (act) {
@@ -609,147 +636,112 @@ __global__ __launch_bounds__(512) void k_dEdc_b3p(int n, B3Dev t, const T *__res
   }
   for (int ls = 0; ls < t.n_ls; ls++) {
     int l1 = t.ls_l[3 * ls], l2 = t.ls_l[3 * ls + 1], l3 = t.ls_l[3 * ls + 2];
-    int S = l1 + l2 + l3, n1m = 2 * l1 + 1, n2m = 2 * l2 + 1, n3m = 2 * l3 + 1;
-    const T *gs = Grow + (size_t)ls * n_tri, *bs = Brow + (size_t)ls * n_tri;
-    for (int kk = tid; kk < n_tri; kk += nt)
-      wl[kk] = project ? (gs[kk] - gb * bs[kk] * ninv) * ninv : gs[kk];
+    int n1m = 2 * l1 + 1, n2m = 2 * l2 + 1, n3m = 2 * l3 + 1;
+    if constexpr (sizeof(T) == 4) {
+#pragma unroll
+      for (int j = 0; j < NQ; j++) {
+        int kk = tid + j * nt;
+        if (kk < n_tri) wl[kk] = project ? (gq[j] - gb * bq[j] * ninv) * ninv : gq[j];
+      }
+    } else {
+      const T *gs = Grow + (size_t)ls * n_tri, *bs = Brow + (size_t)ls * n_tri;
+#pragma unroll 4
+      for (int kk = tid; kk < n_tri; kk += nt)
+        wl[kk] = project ? (gs[kk] - gb * bs[kk] * ninv) * ninv : gs[kk];
+    }
     __syncthreads();
-    const T *wa = wA + (size_t)ls * kLM * kLM, *wb = wB + (size_t)ls * kLM * kLM;
-    const T *x = cc + 2 * (n1 * H + l1 * l1), *y = cc + 2 * (n2 * H + l2 * l2);
-    T er2[kLM], ei2[kLM];
+    if (ls + 1 < t.n_ls) fetch(ls + 1);
+    T e2[kLM];
     if (act) {
-      T pr[kLM], pi[kLM];
-      pair_product<T>(wa, S, n1m, x, y, pr, pi);
+      T x[kLM], y[kLM], d[kLM], p[kLM], e1[kLM];
+      load_block<T>(rc + n1 * H + l1 * l1, n1m, x);
+      load_block<T>(rc + n2 * H + l2 * l2, n2m, y);
 #pragma unroll
-      for (int m3 = 0; m3 < kLM; m3++)
-        if (m3 < n3m) {
-          Pl[2 * (m3 * npair + tid)] = pr[m3];
-          Pl[2 * (m3 * npair + tid) + 1] = pi[m3];
-        }
-      T dr[kLM], di[kLM];
-#pragma unroll
-      for (int m3 = 0; m3 < kLM; m3++) dr[m3] = di[m3] = 0;
+      for (int m = 0; m < kLM; m++) d[m] = p[m] = e1[m] = e2[m] = 0;
       for (int n3 = n2; n3 < nr; n3++) {
         T wv = wl[tb + n3 - n2];
-        const T *z = cc + 2 * (n3 * H + l3 * l3);
+        const T *z = rc + n3 * H + l3 * l3;
 #pragma unroll
-        for (int m3 = 0; m3 < kLM; m3++)
-          if (m3 < n3m) {
-            dr[m3] += wv * z[2 * m3];
-            di[m3] += wv * z[2 * m3 + 1];
-          }
+        for (int m = 0; m < kLM; m++)
+          if (m < n3m) d[m] += wv * z[m];
       }
-      // Factor 1 (target c[n1, l1 m1]): conj(c[n2, l2 m2] D[m3]).
-#pragma unroll
-      for (int m1 = 0; m1 < kLM; m1++) {
-        if (m1 < n1m) {
-          T er = 0, ei = 0;
-#pragma unroll
-          for (int m3 = 0; m3 < kLM; m3++) {
-            T w = wa[m1 * kLM + m3];
-            if (w != T(0)) {
-              int m2 = S - m1 - m3;
-              T yr = y[2 * m2], yi = y[2 * m2 + 1];
-              er += w * (yr * dr[m3] - yi * di[m3]);
-              ei -= w * (yr * di[m3] + yi * dr[m3]);
-            }
-          }
-          El[2 * (m1 * npair + tid)] = er;
-          El[2 * (m1 * npair + tid) + 1] = ei;
-        }
+      int q1 = rt_off[ls + 1];
+      for (int q = rt_off[ls]; q < q1; q++) {
+        int c = rt_pk[q];
+        T w = rt_w[q];
+        int a = c & 15, b = (c >> 4) & 15, e = c >> 8;
+        T wx = w * x[a], wy = w * y[b];
+        p[e] += wx * y[b];
+        e1[a] += wy * d[e];
+        e2[b] += wx * d[e];
       }
-      // Factor 2 (target c[n2, l2 m2]): conj(c[n1, l1 m1] D[m3]), kept in
-      // registers until the factor-1 sums have read El.
 #pragma unroll
-      for (int m2 = 0; m2 < kLM; m2++) {
-        er2[m2] = ei2[m2] = 0;
-        if (m2 < n2m) {
-#pragma unroll
-          for (int m3 = 0; m3 < kLM; m3++) {
-            T w = wb[m2 * kLM + m3];
-            if (w != T(0)) {
-              int m1 = S - m2 - m3;
-              T xr = x[2 * m1], xi = x[2 * m1 + 1];
-              er2[m2] += w * (xr * dr[m3] - xi * di[m3]);
-              ei2[m2] -= w * (xr * di[m3] + xi * dr[m3]);
-            }
-          }
-        }
+      for (int m = 0; m < kLM; m++) {
+        if (m < n3m) Pl[m * npair + tid] = p[m];
+        if (m < n1m) El[m * npair + tid] = e1[m];
       }
     }
     __syncthreads();
-    // Factor 1 targets (n, l1 m1): pairs (n, n2 >= n).
+    // Factor 1 targets (n, l1 mu1): pairs (n, n2 >= n).
     for (int tg = tid; tg < nr * n1m; tg += nt) {
       int nn = tg / n1m, m1 = tg - nn * n1m;
-      T ar = 0, ai = 0;
-      for (int b = nn; b < nr; b++) {
-        const T *e = El + 2 * (m1 * npair + b * (b + 1) / 2 + nn);
-        ar += e[0];
-        ai += e[1];
-      }
-      gc[2 * (nn * H + l1 * l1 + m1)] += ar;
-      gc[2 * (nn * H + l1 * l1 + m1) + 1] += ai;
+      T acc = 0;
+      for (int b = nn; b < nr; b++) acc += El[m1 * npair + b * (b + 1) / 2 + nn];
+      gc[nn * H + l1 * l1 + m1] += acc;
     }
-    __syncthreads();
-    // Factor 3 targets (n, l3 m3): the pairs with n2 <= n, a prefix.
-    for (int tg = tid; tg < nr * n3m; tg += nt) {
-      int nn = tg / n3m, m3 = tg - nn * n3m;
-      int np = (nn + 1) * (nn + 2) / 2;
-      T ar = 0, ai = 0;
-      for (int q = 0; q < np; q++) {
-        int b = t.pair_pk[q] >> 8;
-        T wv = wl[t.pair_tb[q] + nn - b];
-        const T *p = Pl + 2 * (m3 * npair + q);
-        ar += wv * p[0];
-        ai -= wv * p[1];
-      }
-      gc[2 * (nn * H + l3 * l3 + m3)] += ar;
-      gc[2 * (nn * H + l3 * l3 + m3) + 1] += ai;
-    }
+    // Factor 3, first stage: the thread of pair (a, b) forms
+    // Q[mu3] = sum_{n1 <= a} w(n1, a, b) P(n1, a)[mu3]; the triple
+    // (n1, a, b) and the pair (n1, a) are stepped along n1.
+    T qv[kLM];
     if (act) {
 #pragma unroll
-      for (int m2 = 0; m2 < kLM; m2++)
-        if (m2 < n2m) {
-          El[2 * (m2 * npair + tid)] = er2[m2];
-          El[2 * (m2 * npair + tid) + 1] = ei2[m2];
-        }
-    }
-    __syncthreads();
-    // Factor 2 targets (n, l2 m2): pairs (n1 <= n, n), contiguous.
-    for (int tg = tid; tg < nr * n2m; tg += nt) {
-      int nn = tg / n2m, m2 = tg - nn * n2m;
-      T ar = 0, ai = 0;
-      const T *e = El + 2 * (m2 * npair + nn * (nn + 1) / 2);
-      for (int a = 0; a <= nn; a++) {
-        ar += e[2 * a];
-        ai += e[2 * a + 1];
+      for (int m = 0; m < kLM; m++) qv[m] = 0;
+      int a = n1, b = n2;
+      int idx = a * nr - a * (a - 1) / 2 + b - a;
+      const T *pp = Pl + a * (a + 1) / 2;
+      for (int c = 0; c <= a; c++) {
+        T wv = wl[idx];
+#pragma unroll
+        for (int m = 0; m < kLM; m++)
+          if (m < n3m) qv[m] += wv * pp[(size_t)m * npair];
+        idx += (nr - c) * (nr - c - 1) / 2;
+        pp += 1;
       }
-      gc[2 * (nn * H + l2 * l2 + m2)] += ar;
-      gc[2 * (nn * H + l2 * l2 + m2) + 1] += ai;
     }
     __syncthreads();
-  }
-  // u[n, l, m] = d E / d c^r: transpose of the complex map.
-  const T s2 = T(0.70710678118654752440);
-  for (int g = tid; g < nc; g += nt) {
-    int nn = g / H, lm = g - nn * H;
-    int l = 0;
-    while ((l + 1) * (l + 1) <= lm) l++;
-    int m = lm - l * l - l;
-    const T *gr = gc + 2 * (nn * H + l * l + l);
-    T u;
-    if (m == 0) {
-      u = gr[0];
-    } else {
-      int mm = m > 0 ? m : -m;
-      T sg = (mm & 1) ? T(-1) : T(1);
-      if (m > 0)
-        u = s2 * (sg * gr[2 * mm] + gr[-2 * mm]);
-      else
-        u = s2 * (sg * gr[2 * mm + 1] - gr[-2 * mm + 1]);
+    if (act) {
+#pragma unroll
+      for (int m = 0; m < kLM; m++) {
+        if (m < n3m) Pl[m * npair + tid] = qv[m];
+        if (m < n2m) El[m * npair + tid] = e2[m];
+      }
     }
-    urow[g] = u;
+    __syncthreads();
+    // Factor 3 targets (n, l3 mu3): Q over the pairs (n2 <= n, n), and factor
+    // 2 targets (n, l2 mu2): E2 over the pairs (n1 <= n, n); both contiguous.
+    // With l2 == l3 the targets coincide and one thread adds both.
+    for (int tg = tid; tg < nr * n3m; tg += nt) {
+      int nn = tg / n3m, m3 = tg - nn * n3m;
+      const T *e = Pl + m3 * npair + nn * (nn + 1) / 2;
+      T acc = 0;
+      for (int a = 0; a <= nn; a++) acc += e[a];
+      if (l2 == l3) {
+        const T *f = El + m3 * npair + nn * (nn + 1) / 2;
+        for (int a = 0; a <= nn; a++) acc += f[a];
+      }
+      gc[nn * H + l3 * l3 + m3] += acc;
+    }
+    if (l2 != l3)
+      for (int tg = tid; tg < nr * n2m; tg += nt) {
+        int nn = tg / n2m, m2 = tg - nn * n2m;
+        const T *e = El + m2 * npair + nn * (nn + 1) / 2;
+        T acc = 0;
+        for (int a = 0; a <= nn; a++) acc += e[a];
+        gc[nn * H + l2 * l2 + m2] += acc;
+      }
+    __syncthreads();
   }
+  for (int g = tid; g < nc; g += nt) urow[g] = gc[g];
 }
 
 // Column / row permutations between the reference's B3 order and the
@@ -790,7 +782,7 @@ __global__ void k_b3_permute_rows(int nd, long long ld, int n_tri, int n_ls, int
 struct B3State {
   B3Tables t;
   DevBuf tri, tri_base, ls_l, term_off, term_lm, term_w, gls_off, gls_term, tri_pk;
-  DevBuf pair_pk, pair_tb, wA64, wB64, wA32, wB32;
+  DevBuf pair_pk, pair_tb, rt_off, rt_pk, rt_w64, rt_w32;
   B3Dev dev() const {
     return B3Dev{tri.as<int>(),      ls_l.as<int>(),     term_off.as<int>(),
                  term_lm.as<int>(),  gls_off.as<int>(),  gls_term.as<int>(),
@@ -800,16 +792,10 @@ struct B3State {
                  pair_tb.as<int>(),  t.npair};
   }
   template <typename T>
-  const T *wA() const {
-    return std::is_same<T, double>::value ? (const T *)wA64.as<double>()
-                                          : (const T *)wA32.as<float>();
-  }
-  template <typename T>
-  const T *wB() const {
-    return std::is_same<T, double>::value ? (const T *)wB64.as<double>()
-                                          : (const T *)wB32.as<float>();
-  }
-};
+  const T *rt_w() const {
+    return std::is_same<T, double>::value ? (const T *)rt_w64.as<double>()
+                                          : (const T *)rt_w32.as<float>();
+  }};
 
 static B3State *b3_state(fm_structure s) {
   if (!s->b3) {
@@ -833,11 +819,11 @@ static B3State *b3_state(fm_structure s) {
     up(st->tri_pk, t.tri_pk.data(), t.tri_pk.size() * 4);
     up(st->pair_pk, t.pair_pk.data(), t.pair_pk.size() * 4);
     up(st->pair_tb, t.pair_tb.data(), t.pair_tb.size() * 4);
-    up(st->wA64, t.wA.data(), t.wA.size() * 8);
-    up(st->wB64, t.wB.data(), t.wB.size() * 8);
-    std::vector<float> wa(t.wA.begin(), t.wA.end()), wb(t.wB.begin(), t.wB.end());
-    up(st->wA32, wa.data(), wa.size() * 4);
-    up(st->wB32, wb.data(), wb.size() * 4);
+    up(st->rt_off, t.rt_off.data(), t.rt_off.size() * 4);
+    up(st->rt_pk, t.rt_pk.data(), t.rt_pk.size() * 4);
+    up(st->rt_w64, t.rt_w.data(), t.rt_w.size() * 8);
+    std::vector<float> w32(t.rt_w.begin(), t.rt_w.end());
+    up(st->rt_w32, w32.data(), w32.size() * 4);
     FM_HIP(hipStreamSynchronize(stream));
     s->b3 = st;
   }
@@ -935,9 +921,10 @@ static int waves_per_block(size_t lds_per_atom) {
 static size_t b3_pair_lds(const B3Dev &d, size_t esz, bool adjoint, int &nt) {
   if (d.npair > 512) return 0;
   nt = std::max(64, (d.npair + 63) / 64 * 64);
+  if (d.n_tri > kPre * nt) return 0;
   size_t nc = (size_t)d.nr * d.H;
-  size_t el = 2 * nc + d.n_tri + 16;
-  if (adjoint) el += 2 * nc + 4 * (size_t)kLM * d.npair;
+  size_t el = nc + d.n_tri + 16;
+  if (adjoint) el += nc + 2 * (size_t)kLM * d.npair;
   size_t bytes = el * esz;
   return bytes <= 160 * 1024 ? bytes : 0;
 }
@@ -958,13 +945,13 @@ void launch_b3_rows(fm_structure s, int r0, int n, bool tile, void *Bout, void *
   size_t lds = b3_pair_lds(d, esz, false, nt);
   if (lds) {
     if (f64)
-      hipLaunchKernelGGL(k_b3p<double>, dim3(n), dim3(nt), lds, st, n, d, b->wA<double>(),
-                         s->C.as<double>(), s->row_of_atom.as<int>(), rr, (double *)Bout,
+      hipLaunchKernelGGL(k_b3p<double>, dim3(n), dim3(nt), lds, st, n, d, b->rt_off.as<int>(),
+                         b->rt_pk.as<int>(), b->rt_w<double>(), s->C.as<double>(), s->row_of_atom.as<int>(), rr, (double *)Bout,
                          s->norm.as<double>(), s->nn.as<double>(), (__bf16 *)nullptr,
                          (__bf16 *)nullptr, 0);
     else
-      hipLaunchKernelGGL(k_b3p<float>, dim3(n), dim3(nt), lds, st, n, d, b->wA<float>(),
-                         s->C.as<float>(), s->row_of_atom.as<int>(), rr, (float *)Bout,
+      hipLaunchKernelGGL(k_b3p<float>, dim3(n), dim3(nt), lds, st, n, d, b->rt_off.as<int>(),
+                         b->rt_pk.as<int>(), b->rt_w<float>(), s->C.as<float>(), s->row_of_atom.as<int>(), rr, (float *)Bout,
                          s->norm.as<double>(), s->nn.as<double>(), (__bf16 *)B0, (__bf16 *)B1,
                          (int)split_ld(s->nd));
     FM_HIP(hipGetLastError());
@@ -1007,13 +994,15 @@ void launch_dEdc_b3_rows(fm_structure s, int r0, int n, bool tile, const void *B
   if (plds) {
     if (f64)
       hipLaunchKernelGGL(k_dEdc_b3p<double>, dim3(n), dim3(nt), plds, st, n, d,
-                         b->wA<double>(), b->wB<double>(), s->C.as<double>(),
+                         b->rt_off.as<int>(), b->rt_pk.as<int>(), b->rt_w<double>(),
+                         s->C.as<double>(),
                          s->row_of_atom.as<int>(), rr, (const double *)B, s->norm.as<double>(),
                          s->nn.as<double>(), (const double *)G, (long long)g_ld, project,
                          empty_norm, empty_nn, (double *)U);
     else
-      hipLaunchKernelGGL(k_dEdc_b3p<float>, dim3(n), dim3(nt), plds, st, n, d, b->wA<float>(),
-                         b->wB<float>(), s->C.as<float>(), s->row_of_atom.as<int>(), rr,
+      hipLaunchKernelGGL(k_dEdc_b3p<float>, dim3(n), dim3(nt), plds, st, n, d,
+                         b->rt_off.as<int>(), b->rt_pk.as<int>(), b->rt_w<float>(),
+                         s->C.as<float>(), s->row_of_atom.as<int>(), rr,
                          (const float *)B, s->norm.as<double>(), s->nn.as<double>(),
                          (const float *)G, (long long)g_ld, project, empty_norm, empty_nn,
                          (float *)U);

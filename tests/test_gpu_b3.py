@@ -128,13 +128,12 @@ def test_b3_row_tiles(C, oracle, system, form):
             s.descriptors[0].descriptors
 
 
-@pytest.mark.parametrize("nos,nmax,n_atoms", [(3, 6, 14), (4, 7, 9)])
+@pytest.mark.parametrize("nos,nmax,n_atoms", [(3, 6, 14), (4, 8, 9)])
 def test_b3_lmax4_settings(C, oracle, nos, nmax, n_atoms):
     """l_max = 4 B3 against the oracle: [3, 6, 4] is the setting of the
-    million-atom alloy (18 radial channels); [4, 7, 4] (28 channels) is wide
-    enough that the FP64 adjoint's workspace exceeds LDS, so that form runs
-    the wave-per-atom adjoint while FP32 keeps the pair-factorized one (its
-    second species has no atoms)."""
+    million-atom alloy (18 radial channels, pair-factorized kernels); [4, 8, 4]
+    has 32 channels, 528 radial pairs, more than one workgroup takes, so both
+    forms run the wave-per-atom kernels (one species has no atoms)."""
     say("B3 l_max 4 settings", nos, nmax, n_atoms)
     rc = 4.4
     cut = np.full((nos, nos), rc)
