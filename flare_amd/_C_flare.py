@@ -114,17 +114,20 @@ class DescriptorValues:
     from the device. Fields are per species lists, in species-sorted order
     with atom order kept within a species (b2.cpp:104-176)."""
 
-    def __init__(self, structure=None, calc=None):
+    def __init__(self, structure=None, calc=None, handle=None):
         self._structure = structure
         self._calc = calc
         self._cache = {}
+        # The device structure holding this calculator's descriptors.
+        self._handle = handle if handle is not None else (
+            structure._h if structure is not None else None)
         if structure is not None:
             L = _lib.load()
             nos = calc.descriptor_settings[0]
             nd = _lib.C.c_int()
             ncl = np.zeros(nos, np.int32)
             nnb = np.zeros(nos, np.int32)
-            _lib.check(L.fm_structure_b2_sizes(structure._h, _lib.C.byref(nd), _lib.ptr(ncl),
+            _lib.check(L.fm_structure_b2_sizes(self._handle, _lib.C.byref(nd), _lib.ptr(ncl),
                                                _lib.ptr(nnb)))
             self.n_descriptors = int(nd.value)
             self.n_types = nos
@@ -154,7 +157,7 @@ class DescriptorValues:
             ai = np.zeros(ns, np.int32)
             ni = np.zeros(nb, np.int32)
             co = np.zeros((nb, 3))
-            _lib.check(L.fm_structure_get_b2(self._structure._h, s, _lib.ptr(d), _lib.ptr(nrm),
+            _lib.check(L.fm_structure_get_b2(self._handle, s, _lib.ptr(d), _lib.ptr(nrm),
                                              _lib.ptr(nc), _lib.ptr(cnc), _lib.ptr(ai),
                                              _lib.ptr(ni), _lib.ptr(co)))
             for k, v in zip(out, (d, nrm, nc, cnc, ai, ni, co)):
@@ -188,7 +191,7 @@ class DescriptorValues:
             nb = self.n_neighbors_by_type[s]
             d = np.zeros((3 * nb, self.n_descriptors))
             f = np.zeros(3 * nb)
-            _lib.check(L.fm_structure_get_b2_dervs(self._structure._h, s, _lib.ptr(d),
+            _lib.check(L.fm_structure_get_b2_dervs(self._handle, s, _lib.ptr(d),
                                                    _lib.ptr(f)))
             dervs.append(d)
             dots.append(f)
@@ -287,7 +290,7 @@ class NormalizedDotProduct(Kernel):
         st = struc._structure
         m = _DeviceModel.from_sparse_set(st._ctx, sig, self.power, envs)
         K = np.zeros((envs.n_clusters, 1 + 3 * st.noa + 6))
-        _lib.check(_lib.load().fm_envs_struc(m.h, st._h, _lib.ptr(K)))
+        _lib.check(_lib.load().fm_envs_struc(m.h, struc._handle, _lib.ptr(K)))
         return K
 
     def self_kernel_struc(self, struc, hyps=None):
@@ -297,7 +300,7 @@ class NormalizedDotProduct(Kernel):
         empty = ClusterDescriptor(struc.n_types, struc.n_descriptors)
         m = _DeviceModel.from_sparse_set(st._ctx, sig, self.power, empty)
         out = np.zeros(1 + 3 * st.noa + 6)
-        _lib.check(_lib.load().fm_self_kernel_struc(m.h, st._h, _lib.ptr(out)))
+        _lib.check(_lib.load().fm_self_kernel_struc(m.h, struc._handle, _lib.ptr(out)))
         return out
 
 
@@ -331,6 +334,8 @@ class Structure:
         self.descriptors = []
         self.descriptor_calculators = list(descriptor_calculators or [])
         self._h = None
+        self._hs = []  # device structure of each descriptor calculator (_hs[0] is _h)
+        self._slot = 0
         self._ctx = context or _lib.default_context()
         self._nbr_cache = None
         self.cutoff = cutoff
@@ -341,7 +346,7 @@ class Structure:
         if cutoff is not None:
             self.compute_descriptors()
 
-    def _build(self, cutoff):
+    def _create_device(self, cutoff):
         L = _lib.load()
         h = _lib.C.c_void_p()
         cell = _lib.f64(self.cell.reshape(-1))
@@ -349,7 +354,14 @@ class Structure:
         pos = _lib.f64(self.positions.reshape(-1))
         _lib.check(L.fm_structure_create(self._ctx.h, _lib.ptr(cell), self.noa, _lib.ptr(sp),
                                          _lib.ptr(pos), float(cutoff), _lib.C.byref(h)))
+        return h
+
+    def _build(self, cutoff):
+        L = _lib.load()
+        self._build_cutoff = cutoff
+        h = self._create_device(cutoff)
         self._h = h
+        self._hs = [h]
         sweep = _lib.C.c_int()
         nnb = _lib.C.c_int64()
         vol = _lib.C.c_double()
@@ -363,8 +375,8 @@ class Structure:
 
     def __del__(self):
         try:
-            if self._h is not None:
-                _lib.load().fm_structure_destroy(self._h)
+            for h in self._hs:
+                _lib.load().fm_structure_destroy(h)
         except Exception:
             pass
 
@@ -398,20 +410,26 @@ class Structure:
         return self.wrapped_positions.copy()
 
     def compute_descriptors(self):
-        """structure.cpp:45-50."""
-        self.descriptors = [calc.compute_struc(self) for calc in self.descriptor_calculators]
+        """structure.cpp:45-50. Each calculator's descriptors live in a device
+        structure of their own (the first is the one built at construction;
+        the others repeat its neighbour list)."""
+        out = []
+        for k, calc in enumerate(self.descriptor_calculators):
+            self._slot = k
+            out.append(calc.compute_struc(self))
+        self._slot = 0
+        self.descriptors = out
 
     def _compute_b2(self, calc):
-        if len(self.descriptor_calculators) > 1:
-            # Each calculator needs its own device descriptor buffers.
-            raise ValueError("one descriptor calculator per Structure is supported on the GPU "
-                             "path")
+        while len(self._hs) <= self._slot:
+            self._hs.append(self._create_device(self._build_cutoff))
+        h = self._hs[self._slot]
         settings, keep = calc._settings()
         fn = (_lib.load().fm_structure_compute_b3 if isinstance(calc, B3)
               else _lib.load().fm_structure_compute_b2)
-        _lib.check(fn(self._h, _lib.C.byref(settings), int(self.form)))
+        _lib.check(fn(h, _lib.C.byref(settings), int(self.form)))
         del keep
-        return DescriptorValues(self, calc)
+        return DescriptorValues(self, calc, h)
 
 
 class SparseGP:
@@ -552,7 +570,12 @@ class SparseGP:
                                "update_matrices_QR and try again.")
 
     def _run(self, structure, flags):
+        """Kernel k acts on descriptor k of the structure (sparse_gp.cpp:
+        643-665, 731-757); means add up, local variances are per kernel."""
         self._check_state()
+        if len(structure.descriptors) < self.n_kernels:
+            raise ValueError("the structure has fewer descriptor calculators than the model "
+                             "has kernels")
         n = structure.noa
         mean = np.zeros(1 + 3 * n + 6)
         local = []
@@ -560,8 +583,9 @@ class SparseGP:
             m = self._model(k, structure._ctx)
             out = np.zeros(1 + 3 * n + 6)
             var = np.zeros(n) if flags & _lib.PREDICT_LOCAL else None
-            _lib.check(_lib.load().fm_predict(m.h, structure._h, flags, int(structure.form),
-                                              _lib.ptr(out), _lib.ptr(var), None))
+            _lib.check(_lib.load().fm_predict(m.h, structure.descriptors[k]._handle, flags,
+                                              int(structure.form), _lib.ptr(out),
+                                              _lib.ptr(var), None))
             mean += out
             if var is not None:
                 local.append(var)

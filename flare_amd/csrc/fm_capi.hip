@@ -794,6 +794,9 @@ int fm_predict(fm_model m, fm_structure s, int flags, int form, double *mean_efs
     FM_CHECK(m && s, FM_ERR_INVALID, "null argument");
     FM_CHECK(m->ctx == s->ctx, FM_ERR_INVALID, "model and structure belong to different contexts");
     FM_HIP(hipSetDevice(s->ctx->device));
+    // Stages this prediction does not run (the row-tiled path has no
+    // separate gradient or variance stage) report zero, not an older call.
+    for (int st = fm::ST_SGEMM; st < fm::ST_COUNT; st++) s->ctx->recorded[st] = false;
     fm::predict(m, s, flags, form, mean_efs, local_var, local_energy);
   });
 }

@@ -163,3 +163,54 @@ def test_b3_lmax4_settings(C, oracle, nos, nmax, n_atoms):
     dF = np.abs(s32.mean_efs[1:1 + 3 * n] - s.mean_efs[1:1 + 3 * n]).max()
     say(f"  chosen form dF {dF * 1e3:.3e} meV/A")
     assert dF < 1e-3
+
+
+@pytest.mark.parametrize("form", [0, 1])
+def test_b2_and_b3_kernels_in_one_model(C, oracle, system, form):
+    """Two descriptor calculators on one structure and a two-kernel model
+    (kernel i acts on descriptor i, sparse_gp.cpp:643-757): the means add and
+    the local variances are per kernel, equal bit for bit to the two
+    single-kernel models, and (FP64) to the oracle's sum at the drop-in bar."""
+    say("B2 + B3 model, form", form)
+    from scipy.linalg import block_diag
+    cell, species, pos = system["cell"], system["species"], system["pos"]
+    b2 = C.B2("chebyshev", "quadratic", [0.0, 4.6], [], [NOS, 4, 3], CUT)
+    b3 = system["calc"]
+    ref3 = system["ref"]
+    ref2 = oracle.B2Values(oracle.Neighbors(cell, species, pos, 4.6, "literal"), NOS, 4, 3, CUT)
+    picks = [list(range(min(4, ref3.n_clusters_by_type[t]))) for t in range(NOS)]
+    X2, N2, a2, L2 = model_state(ref2, picks, 1.7, 2.0, seed=21)
+    X3, N3, a3, L3 = model_state(ref3, picks, 1.3, 2.0, seed=11)
+    s0 = C.Structure(cell, species, pos, 4.6, [b2, b3])
+    atoms = [int(ref3.atom_indices[t][k]) for t in range(NOS) for k in picks[t]]
+    kern = [C.NormalizedDotProduct(1.7, 2.0), C.NormalizedDotProduct(1.3, 2.0)]
+    both = C.SparseGP(kern, 0.01, 0.05, 0.005)
+    both.add_specific_environments(s0, atoms)
+    both.set_state(np.concatenate([a2, a3]), L_inv=block_diag(L2, L3))
+    singles = []
+    for k, (calc, a, L) in enumerate(((b2, a2, L2), (b3, a3, L3))):
+        g = C.SparseGP([kern[k]], 0.01, 0.05, 0.005)
+        g.add_specific_environments(C.Structure(cell, species, pos, 4.6, [calc]), atoms)
+        g.set_state(a, L_inv=L)
+        singles.append((calc, g))
+    s = C.Structure(cell, species, pos, 4.6, [b2, b3], form=form)
+    both.predict_local_uncertainties(s)
+    parts = []
+    for calc, g in singles:
+        t = C.Structure(cell, species, pos, 4.6, [calc], form=form)
+        g.predict_local_uncertainties(t)
+        parts.append(t)
+    assert np.array_equal(s.mean_efs, parts[0].mean_efs + parts[1].mean_efs)
+    assert len(s.local_uncertainties) == 2
+    for k in range(2):
+        assert np.array_equal(s.local_uncertainties[k], parts[k].local_uncertainties[0])
+    if form == 0:
+        n = s.noa
+        m2, v2 = oracle.Model(1.7, 2.0, X2, N2, a2, L_inv=L2).predict(ref2, "local")
+        m3, v3 = oracle.Model(1.3, 2.0, X3, N3, a3, L_inv=L3).predict(ref3, "local")
+        mean = m2 + m3
+        assert abs(s.mean_efs[0] - mean[0]) / abs(mean[0]) < TOL
+        assert rel_err(s.mean_efs[1:1 + 3 * n], mean[1:1 + 3 * n]) < TOL
+        assert rel_err(s.mean_efs[-6:], mean[-6:]) < TOL
+        assert var_err(s.local_uncertainties[0], v2, 1.7 ** 2) < TOL
+        assert var_err(s.local_uncertainties[1], v3, 1.3 ** 2) < TOL
