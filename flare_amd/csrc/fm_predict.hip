@@ -176,6 +176,24 @@ static void read_outputs(fm_structure s, const double *o, size_t n_out, const do
 }
 
 
+// Rows per tile of one species in the chosen form: at most T and, when T
+// allows, a whole number of rounds of the kernel contraction's 128 x 128
+// split tiles over the GPU's workgroup slots (three such workgroups fit a
+// CU), so the last round of each launch is not left nearly empty (74 row
+// tiles x 11 column tiles on 768 slots would run a second round of 46).
+// Rows of a tile do not change any result.
+static int species_tile_rows(fm_context ctx, int T, int Ms, int form) {
+  if (form != FM_FORM_CHOSEN || Ms == 0) return T;
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) !=
+          hipSuccess ||
+      cus <= 0)
+    return T;
+  int per_round = 3 * cus / ((Ms + 127) / 128);
+  int rounds = per_round > 0 ? (T / 128) / per_round : 0;
+  return rounds > 0 ? rounds * per_round * 128 : T;
+}
+
 // B3 structures too large to keep their descriptors (s->row_tile > 0): each
 // species' rows are processed in tiles of row_tile rows. Per tile: B3 rows
 // (from the coefficients c kept for every atom), the kernel contraction with
@@ -235,8 +253,9 @@ static void predict_tiled(fm_model m, fm_structure s, int flags, int form, doubl
     int ns = s->n_by_type[t], Ms = m->M_by_type[t], ro = s->type_off[t];
     int n_et = Ms == 0 ? 1 : gemm_col_tiles(form, Ms);
     int n_qt = (want_var && Ms > 0) ? gemm_row_tiles(FM_FORM_FP64, m->L_rows[t]) : 0;
-    for (int c0 = 0; c0 < ns; c0 += T) {
-      int cnt = std::min(T, ns - c0), r0 = ro + c0;
+    int Ts = species_tile_rows(ctx, T, Ms, form);
+    for (int c0 = 0; c0 < ns; c0 += Ts) {
+      int cnt = std::min(Ts, ns - c0), r0 = ro + c0;
       launch_b3_rows(s, r0, cnt, true, Bt.as<void>(), f64 ? nullptr : Bt0.as<void>(),
                      f64 ? nullptr : Bt1.as<void>());
       double *ep = Epart.as<double>() + eo[t] + c0;
