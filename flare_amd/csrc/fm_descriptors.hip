@@ -25,6 +25,8 @@ namespace fm {
 
 constexpr int kAtomsPerBlock = 4;
 
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
 struct B2Args {
   int n, nos, N, lmax, H, nrad, nd, cutkind;
   double r1;
@@ -285,12 +287,15 @@ __global__ __launch_bounds__(64 * kAtomsPerBlock) void k_b2(B2Args a, T *B, doub
   for (int k = lane; k < nc; k += 64) Cg[(size_t)i * nc + k] = c[k];
   if (!want_b2) return;
   // B_l = C_l C_l^T on the matrix cores (C_l[n][m] = c[n, l^2 + m]); the
-  // upper triangle n2 >= n1 goes straight to the row at p(n1, n2) (L + 1) + l
-  // (each element is written once; the L2 merges a row's scattered stores).
+  // upper triangle n2 >= n1 belongs at p(n1, n2) (L + 1) + l of the row. FP64
+  // stores each element there directly (the L2 merges a row's scattered
+  // stores); FP32 stages the row in LDS (over the bond tables, no longer
+  // needed) and writes it and its two BF16 pieces with coalesced stores.
   int L1 = a.lmax + 1;
   int r16 = lane & 15, k4 = lane >> 4;
   int row = a.row_of_atom[i];
   T *Brow = B + (size_t)row * a.nd;
+  T *bst = tab;
   T s2 = 0;
   for (int l = 0; l <= a.lmax; l++) {
     int nm = 2 * l + 1, lm0 = l * l;
@@ -319,26 +324,37 @@ __global__ __launch_bounds__(64 * kAtomsPerBlock) void k_b2(B2Args a, T *B, doub
           if (n1 < nr && n2 < nr && n2 >= n1) {
             int k = (n1 * nr - (n1 * (n1 - 1)) / 2 + (n2 - n1)) * L1 + l;
             T v = acc[q];
-            Brow[k] = v;
-            if constexpr (sizeof(T) == 4) {
-              float r = v;
-#pragma unroll
-              for (int pc = 0; pc < 2; pc++) {
-                __bf16 h = (__bf16)r;
-                Bs.p[pc][(size_t)row * ldp + k] = h;
-                r -= (float)h;
-              }
-            }
+            if constexpr (sizeof(T) == 4)
+              bst[k] = v;
+            else
+              Brow[k] = v;
             s2 += v * v;
           }
         }
       }
     }
   }
-  if constexpr (sizeof(T) == 4)
-    for (int k = a.nd + lane; k < ldp; k += 64)
+  if constexpr (sizeof(T) == 4) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int k = lane; k < a.nd; k += 64) Brow[k] = bst[k];
+    // Pieces: 8 consecutive values per lane, one 16-byte store per piece
+    // (ldp is a multiple of 32; the padding past nd is zero).
+    for (int k8 = lane; k8 < ldp / 8; k8 += 64) {
+      bf16x8 h0, h1;
 #pragma unroll
-      for (int pc = 0; pc < 2; pc++) Bs.p[pc][(size_t)row * ldp + k] = (__bf16)0.0f;
+      for (int e = 0; e < 8; e++) {
+        int k = 8 * k8 + e;
+        float r = k < a.nd ? (float)bst[k] : 0.0f;
+        __bf16 h = (__bf16)r;
+        h0[e] = h;
+        h1[e] = (__bf16)(r - (float)h);
+      }
+      *reinterpret_cast<bf16x8 *>(Bs.p[0] + (size_t)row * ldp + 8 * k8) = h0;
+      *reinterpret_cast<bf16x8 *>(Bs.p[1] + (size_t)row * ldp + 8 * k8) = h1;
+    }
+  }
   s2 = wave_sum(s2);
   if (lane == 0) {
     norm[row] = (double)sqrt(s2);
@@ -723,6 +739,9 @@ void launch_b2(fm_structure s, bool want_b2) {
   B2Args a = make_args(s);
   size_t esz = s->form == FM_FORM_FP64 ? sizeof(double) : sizeof(float);
   size_t work = b2_work(a.nrad * a.H, a.H, a.N);
+  // FP32 stages the B row after c (over the bond tables).
+  if (want_b2 && s->form != FM_FORM_FP64)
+    work = std::max(work, (size_t)a.nrad * a.H + (size_t)a.nd);
   size_t per = work * esz;
   int apb = atoms_per_block(per);
   size_t lds = apb * per;
@@ -732,7 +751,8 @@ void launch_b2(fm_structure s, bool want_b2) {
   SplitOut none{{nullptr, nullptr}};
   SplitOut pieces{{s->Bs[0].as<__bf16>(), s->Bs[1].as<__bf16>()}};
   // The unrolled sweep (compile-time l_max) is faster for the FP64 kernel;
-  // for FP32 its extra registers cost more occupancy than it saves.
+  // for FP32 its extra registers cost more occupancy than it saves at
+  // l_max 6.
   if (s->form == FM_FORM_FP64)
     with_lmax(a.lmax, [&](auto lc) {
       constexpr int LC = decltype(lc)::value;
@@ -741,9 +761,12 @@ void launch_b2(fm_structure s, bool want_b2) {
                          s->C.as<double>(), (int)want_b2, (int)work);
     });
   else
-    hipLaunchKernelGGL((k_b2<float, -1>), dim3(nblk), dim3(64 * apb), lds, st, a,
-                       s->B.as<float>(), s->norm.as<double>(), s->nn.as<double>(), pieces,
-                       (int)split_ld(s->nd), s->C.as<float>(), (int)want_b2, (int)work);
+    with_lmax(a.lmax <= 4 ? a.lmax : -1, [&](auto lc) {
+      constexpr int LC = decltype(lc)::value;
+      hipLaunchKernelGGL((k_b2<float, LC>), dim3(nblk), dim3(64 * apb), lds, st, a,
+                         s->B.as<float>(), s->norm.as<double>(), s->nn.as<double>(), pieces,
+                         (int)split_ld(s->nd), s->C.as<float>(), (int)want_b2, (int)work);
+    });
   FM_HIP(hipGetLastError());
 }
 
