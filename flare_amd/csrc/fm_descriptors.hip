@@ -404,6 +404,31 @@ __device__ __forceinline__ void stage_row(const T *src, T *dst, int n, int lane)
   for (int k = done + lane; k < n; k += 64) dst[k] = src[k];
 }
 
+// G . B over one row, summed over the wave: 16-byte loads of both rows when
+// both are 16-byte aligned (all of a lane's vectors in flight together), a
+// fixed order per lane and a fixed wave reduction.
+template <typename T>
+__device__ __forceinline__ T row_dot(const T *G, const T *B, int n, int lane) {
+  constexpr int E = 16 / sizeof(T);
+  T acc = 0;
+  int done = 0;
+  if (((reinterpret_cast<uintptr_t>(G) | reinterpret_cast<uintptr_t>(B)) & 15) == 0) {
+    using V = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
+    const V *gv = reinterpret_cast<const V *>(G), *bv = reinterpret_cast<const V *>(B);
+    int nv = n / E;
+#pragma unroll 8
+    for (int v = lane; v < nv; v += 64) {
+      V x = gv[v], y = bv[v];
+      const T *xs = reinterpret_cast<const T *>(&x), *ys = reinterpret_cast<const T *>(&y);
+#pragma unroll
+      for (int e = 0; e < E; e++) acc += xs[e] * ys[e];
+    }
+    done = nv * E;
+  }
+  for (int k = done + lane; k < n; k += 64) acc += G[k] * B[k];
+  return wave_sum(acc);
+}
+
 // w = sg G - sb B (the atom's dE/dB row) into LDS, n elements: 16-byte
 // loads of G and B when both rows are 16-byte aligned, four vectors of each
 // in flight per lane; B is not read when sb is zero.
@@ -470,10 +495,7 @@ __global__ __launch_bounds__(64 * kAtomsPerBlock) void k_dEdc(B2Args a, const T 
     if (aa.energy) {
       sb = (T)(aa.power * aa.energy[row] / (nrm * nrm));
     } else {
-      T gb = 0;
-#pragma unroll 8
-      for (int k = lane; k < a.nd; k += 64) gb += Grow[k] * Brow[k];
-      sb = wave_sum(gb) * ninv * ninv * ninv;
+      sb = row_dot(Grow, Brow, a.nd, lane) * ninv * ninv * ninv;
     }
   }
   // One pass over HBM with 16-byte loads: c and w to LDS.
