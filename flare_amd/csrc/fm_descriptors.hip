@@ -289,8 +289,9 @@ __global__ __launch_bounds__(64 * kAtomsPerBlock) void k_b2(B2Args a, T *B, doub
   // B_l = C_l C_l^T on the matrix cores (C_l[n][m] = c[n, l^2 + m]); the
   // upper triangle n2 >= n1 belongs at p(n1, n2) (L + 1) + l of the row. FP64
   // stores each element there directly (the L2 merges a row's scattered
-  // stores); FP32 stages the row in LDS (over the bond tables, no longer
-  // needed) and writes it and its two BF16 pieces with coalesced stores.
+  // stores; staging would cost it occupancy); FP32 stages the row in LDS
+  // (over the bond tables, no longer needed) and writes it and its two BF16
+  // pieces with coalesced stores.
   int L1 = a.lmax + 1;
   int r16 = lane & 15, k4 = lane >> 4;
   int row = a.row_of_atom[i];
