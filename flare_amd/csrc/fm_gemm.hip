@@ -397,13 +397,17 @@ struct SplitLoader {
   }
 };
 
-template <int EP, int NP, int TM, int TN>
+// With DB the LDS tiles are double-buffered: the next k-step's tile is
+// stored while the current one feeds the MFMAs, one barrier per k-step (the
+// products and their order are the same, so results are too). It pays on
+// long k (one workgroup of 82 KB per CU, against three single-buffered).
+template <int EP, int NP, int TM, int TN, bool DB>
 __global__ __launch_bounds__(256) void k_gemm_split(int M, int N, int Kpad, SplitPtrs Ag,
                                                     long long lda, SplitPtrs Bg, long long ldb,
                                                     EpArgs e) {
-  constexpr int BM = 32 * TM, BN = 32 * TN;
-  __shared__ __align__(16) __bf16 As[NP][BM * LDK];
-  __shared__ __align__(16) __bf16 Bs[NP][BN * LDK];
+  constexpr int BM = 32 * TM, BN = 32 * TN, NB = DB ? 2 : 1;
+  __shared__ __align__(16) __bf16 As[NB][NP][BM * LDK];
+  __shared__ __align__(16) __bf16 Bs[NB][NP][BN * LDK];
   __shared__ double red[2 * (BM > BN ? BM : BN)];
   int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   int wr = wid >> 1, wc = wid & 1;
@@ -419,12 +423,21 @@ __global__ __launch_bounds__(256) void k_gemm_split(int M, int N, int Kpad, Spli
   SplitLoader<NP, BN> lb(tid, col0, N, ldb);
   la.load(Ag, 0);
   lb.load(Bg, 0);
+  if (DB) {
+    la.store(As[0]);
+    lb.store(Bs[0]);
+    __syncthreads();
+  }
+  int cur = 0;
   for (int k0 = 0; k0 < Kpad; k0 += BK16) {
-    __syncthreads();
-    la.store(As);
-    lb.store(Bs);
-    __syncthreads();
-    if (k0 + BK16 < Kpad) {
+    if (!DB) {
+      __syncthreads();
+      la.store(As[0]);
+      lb.store(Bs[0]);
+      __syncthreads();
+    }
+    bool more = k0 + BK16 < Kpad;
+    if (more) {
       la.load(Ag, k0 + BK16);
       lb.load(Bg, k0 + BK16);
     }
@@ -435,12 +448,12 @@ __global__ __launch_bounds__(256) void k_gemm_split(int M, int N, int Kpad, Spli
 #pragma unroll
       for (int t = 0; t < TM; t++) {
         int ar = wr * 16 * TM + t * 16 + (lane & 15);
-        af[pc][t] = *reinterpret_cast<const v8bf *>(&As[pc][ar * LDK + kk]);
+        af[pc][t] = *reinterpret_cast<const v8bf *>(&As[cur][pc][ar * LDK + kk]);
       }
 #pragma unroll
       for (int t = 0; t < TN; t++) {
         int br = wc * 16 * TN + t * 16 + (lane & 15);
-        bf[pc][t] = *reinterpret_cast<const v8bf *>(&Bs[pc][br * LDK + kk]);
+        bf[pc][t] = *reinterpret_cast<const v8bf *>(&Bs[cur][pc][br * LDK + kk]);
       }
     }
     // Products of piece orders i + j <= NP - 1, smallest terms first.
@@ -455,6 +468,14 @@ __global__ __launch_bounds__(256) void k_gemm_split(int M, int N, int Kpad, Spli
     prod(af[1], bf[0]);
     prod(af[0], bf[1]);
     prod(af[0], bf[0]);
+    if (DB) {
+      if (more) {
+        la.store(As[cur ^ 1]);
+        lb.store(Bs[cur ^ 1]);
+      }
+      __syncthreads();
+      cur ^= 1;
+    }
   }
   TileOut<float, TM, TN> t;
 #pragma unroll
@@ -478,9 +499,9 @@ int gemm_row_tiles(int form, int M) {
 
 // ---------------------------------------------------------------------------
 // Tile autotuner. The first contraction of a given shape (epilogue, M rounded
-// up to a power of two, N, K, device) times every admissible tile shape once
-// on the caller's stream (after a warm-up launch) and keeps the fastest for
-// later calls. Admissible shapes never change results: the energy partials
+// up to a power of two, N, K, device) times every admissible tile shape
+// (the faster of two launches after a warm-up one) on the caller's stream
+// and keeps the fastest for later calls. Admissible shapes never change results: the energy partials
 // of EP_KERNEL are per column tile and the |Q|^2 partials of EP_SUMSQ per
 // row tile, so EP_KERNEL only varies TM and EP_SUMSQ only TN; every output
 // element sums its k terms in the same order whatever the tile.
@@ -488,10 +509,10 @@ int gemm_row_tiles(int form, int M) {
 namespace {
 
 struct TileShape {
-  int tm, tn;
+  int tm, tn, db;
 };
-constexpr TileShape kF64Shapes[] = {{2, 2}, {4, 2}, {2, 4}, {4, 4}};
-constexpr TileShape kSplitShapes[] = {{4, 4}, {2, 4}, {4, 2}};
+constexpr TileShape kF64Shapes[] = {{2, 2, 0}, {4, 2, 0}, {2, 4, 0}, {4, 4, 0}};
+constexpr TileShape kSplitShapes[] = {{4, 4, 0}, {2, 4, 0}, {4, 2, 0}, {4, 4, 1}, {4, 2, 1}};
 
 bool admissible(int ep, TileShape t, TileShape base) {
   if (ep == EP_KERNEL) return t.tn == base.tn;
@@ -529,12 +550,16 @@ int tuned(int family, int ep, int M, int N, int K, hipStream_t st, const TileSha
   for (int i = 0; i < n_shapes; i++) {
     if (!admissible(ep, shapes[i], shapes[0])) continue;
     launch(i);  // warm-up (code object load, caches)
-    FM_HIP(hipEventRecord(e0, st));
-    launch(i);
-    FM_HIP(hipEventRecord(e1, st));
-    FM_HIP(hipEventSynchronize(e1));
     float ms = 0;
-    FM_HIP(hipEventElapsedTime(&ms, e0, e1));
+    for (int r = 0; r < 2; r++) {  // the faster of two timed launches
+      FM_HIP(hipEventRecord(e0, st));
+      launch(i);
+      FM_HIP(hipEventRecord(e1, st));
+      FM_HIP(hipEventSynchronize(e1));
+      float t = 0;
+      FM_HIP(hipEventElapsedTime(&t, e0, e1));
+      ms = r == 0 ? t : std::min(ms, t);
+    }
     if (best < 0 || ms < best_ms) {
       best = i;
       best_ms = ms;
@@ -571,21 +596,23 @@ void f64_shape(int i, hipStream_t st, int M, int N, int K, const double *A, long
   }
 }
 
-template <int EP, int TM, int TN>
+template <int EP, int TM, int TN, bool DB>
 void split_launch(hipStream_t st, int M, int N, int Kpad, const SplitPtrs &a, long long lda,
                   const SplitPtrs &b, long long ldb, const EpArgs &e) {
   dim3 grid((N + 32 * TN - 1) / (32 * TN), (M + 32 * TM - 1) / (32 * TM));
-  hipLaunchKernelGGL((k_gemm_split<EP, 2, TM, TN>), grid, dim3(256), 0, st, M, N, Kpad, a, lda,
-                     b, ldb, e);
+  hipLaunchKernelGGL((k_gemm_split<EP, 2, TM, TN, DB>), grid, dim3(256), 0, st, M, N, Kpad, a,
+                     lda, b, ldb, e);
 }
 
 template <int EP>
 void split_shape(int i, hipStream_t st, int M, int N, int Kpad, const SplitPtrs &a,
                  long long lda, const SplitPtrs &b, long long ldb, const EpArgs &e) {
   switch (i) {
-    case 0: split_launch<EP, 4, 4>(st, M, N, Kpad, a, lda, b, ldb, e); break;
-    case 1: split_launch<EP, 2, 4>(st, M, N, Kpad, a, lda, b, ldb, e); break;
-    default: split_launch<EP, 4, 2>(st, M, N, Kpad, a, lda, b, ldb, e); break;
+    case 0: split_launch<EP, 4, 4, false>(st, M, N, Kpad, a, lda, b, ldb, e); break;
+    case 1: split_launch<EP, 2, 4, false>(st, M, N, Kpad, a, lda, b, ldb, e); break;
+    case 2: split_launch<EP, 4, 2, false>(st, M, N, Kpad, a, lda, b, ldb, e); break;
+    case 3: split_launch<EP, 4, 4, true>(st, M, N, Kpad, a, lda, b, ldb, e); break;
+    default: split_launch<EP, 4, 2, true>(st, M, N, Kpad, a, lda, b, ldb, e); break;
   }
 }
 
@@ -635,7 +662,7 @@ void gemm_split(hipStream_t st, int ep, int np, int M, int N, int Kpad, const Sp
     else split_shape<EP_SUMSQ>(i, st, M, N, Kpad, a, A.ld, b, B.ld, e);
     FM_HIP(hipGetLastError());
   };
-  launch(tuned(1, ep, M, N, Kpad, st, kSplitShapes, 3, launch));
+  launch(tuned(1, ep, M, N, Kpad, st, kSplitShapes, 5, launch));
 }
 
 static uint16_t bf16_rne(float x) {
