@@ -350,11 +350,12 @@ def serving_leg(sgp, calc, batched, n_requests=2000, n_fp64=200):
     srv = Server(sgp, calc, 5.5, form=_lib.FORM_CHOSEN, hbm_budget=24 << 30, n_workers=3,
                  max_batch_atoms=32768)
     try:
-        # Untimed warm-up stream (other structures, same distribution): the
-        # contraction tiles of each new batch shape are autotuned on first
-        # use, which would otherwise stall the first batches of the timed run.
-        poisson_run(srv, batch_structures(300, seed=43, n_min=32, n_max=8192), rate=rate,
-                    seed=4)
+        # Untimed warm-up stream of the same length and distribution (other
+        # structures): the contraction tiles of each batch shape are
+        # autotuned on first use, and the batch shapes that only form under
+        # load would otherwise stall the timed run.
+        poisson_run(srv, batch_structures(n_requests, seed=43, n_min=32, n_max=8192),
+                    rate=rate, seed=4)
         srv.batches.clear()
         reqs, wall = poisson_run(srv, reqs_s, rate=rate, seed=3)
     finally:
