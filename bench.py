@@ -342,11 +342,11 @@ def serving_leg(sgp, calc, batched, n_requests=2000, n_fp64=200):
     reqs_s = batch_structures(n_requests, seed=41, n_min=32, n_max=8192)
     mean_n = float(np.mean([len(s[1]) for s in reqs_s]))
     rate = 0.7 * batched["chosen"]["atom_evals_per_s"] / mean_n
-    # Three workers (contexts) and batches of at most 32,768 atoms: three
-    # batches overlap on the GPU, and the cap keeps batches short while
-    # leaving enough headroom to absorb the fixed cost of a batch at this
-    # rate (16,384 had lower latency when it kept up, but at 0.7x the
-    # large-batch rate it sits at the edge of saturation).
+    # Three workers (contexts) and batches of at most 32,768 atoms (the best
+    # p50/p99 of the worker counts and caps tried). At 0.7x the large-batch
+    # rate the server runs near saturation: batches fill to the cap and
+    # their fixed cost (launches, synchronisations, small-kernel tails) and
+    # the contention between overlapping batches set the latency.
     srv = Server(sgp, calc, 5.5, form=_lib.FORM_CHOSEN, hbm_budget=24 << 30, n_workers=3,
                  max_batch_atoms=32768)
     try:
@@ -449,10 +449,11 @@ def main():
     ap.add_argument("--sparse", type=int, default=2000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workloads", default="all",
-                    choices=["single", "batched", "large", "large_b3", "all"],
+                    choices=["single", "batched", "serving", "large", "large_b3", "all"],
                     help="single: the 32k-atom structure only; all: also the batched, "
                          "served and 1M-atom B2 workloads; batched / large / large_b3: that "
-                         "workload alone (its own JSON line); large_b3 (the 1M-atom alloy "
+                         "workload alone (its own JSON line), serving: the batched workload "
+                         "and the served stream at its rate; large_b3 (the 1M-atom alloy "
                          "with a B2 and a B3 kernel) runs only when named")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a 512 MiB buffer three times (the known-size kernel "
@@ -477,10 +478,14 @@ def main():
     from flare_amd.workloads import fcc_alloy_32k
     lib = _lib.load()
     ctx = _lib.default_context()
-    if args.workloads == "batched":
+    if args.workloads in ("batched", "serving"):
         log("batched workload ...")
-        out, _ = batched_leg(lib, ctx, Hip(), max(3, args.steps // 2), 1)
-        print(json.dumps({"batched": out}), flush=True)
+        out, (bsgp, bcalc) = batched_leg(lib, ctx, Hip(), max(3, args.steps // 2), 1)
+        res = {"batched": out}
+        if args.workloads == "serving":
+            log("serving workload ...")
+            res["serving"] = serving_leg(bsgp, bcalc, out)
+        print(json.dumps(res), flush=True)
         return
     if args.workloads == "large":
         log("1M-atom workload ...")
