@@ -525,12 +525,20 @@ __global__ __launch_bounds__(512) void k_b3p(int n, B3Dev t, const int *__restri
       load_block<T>(rc + n2 * H + l2 * l2, 2 * l2 + 1, y);
 #pragma unroll
       for (int m = 0; m < kLM; m++) p[m] = 0;
-      int q1 = rt_off[ls + 1];
-      for (int q = rt_off[ls]; q < q1; q++) {
-        int c = rt_pk[q];
-        T w = rt_w[q];
-        p[c >> 8] += w * x[c & 15] * y[(c >> 4) & 15];
+      // Couplings in order, their entries loaded four at a time (one wait
+      // for the scalar loads per four terms).
+#define FM_B3_FWD_TERM(c, w) p[(c) >> 8] += (w) * x[(c) & 15] * y[((c) >> 4) & 15]
+      int q = rt_off[ls], q1 = rt_off[ls + 1];
+      for (; q + 4 <= q1; q += 4) {
+        int c0 = rt_pk[q], c1 = rt_pk[q + 1], c2 = rt_pk[q + 2], c3 = rt_pk[q + 3];
+        T w0 = rt_w[q], w1 = rt_w[q + 1], w2 = rt_w[q + 2], w3 = rt_w[q + 3];
+        FM_B3_FWD_TERM(c0, w0);
+        FM_B3_FWD_TERM(c1, w1);
+        FM_B3_FWD_TERM(c2, w2);
+        FM_B3_FWD_TERM(c3, w3);
       }
+      for (; q < q1; q++) FM_B3_FWD_TERM(rt_pk[q], rt_w[q]);
+#undef FM_B3_FWD_TERM
       int n3m = 2 * l3 + 1;
       for (int n3 = n2; n3 < nr; n3++) {
         const T *z = rc + n3 * H + l3 * l3;
@@ -665,16 +673,27 @@ __global__ __launch_bounds__(512) void k_dEdc_b3p(int n, B3Dev t, const int *__r
         for (int m = 0; m < kLM; m++)
           if (m < n3m) d[m] += wv * z[m];
       }
-      int q1 = rt_off[ls + 1];
-      for (int q = rt_off[ls]; q < q1; q++) {
-        int c = rt_pk[q];
-        T w = rt_w[q];
-        int a = c & 15, b = (c >> 4) & 15, e = c >> 8;
-        T wx = w * x[a], wy = w * y[b];
-        p[e] += wx * y[b];
-        e1[a] += wy * d[e];
-        e2[b] += wx * d[e];
+      // Couplings in order, their entries loaded four at a time (one wait
+      // for the scalar loads per four terms).
+#define FM_B3_ADJ_TERM(c, w)                              \
+  {                                                       \
+    int a_ = (c) & 15, b_ = ((c) >> 4) & 15, e_ = (c) >> 8; \
+    T wx = (w) * x[a_], wy = (w) * y[b_];                 \
+    p[e_] += wx * y[b_];                                  \
+    e1[a_] += wy * d[e_];                                 \
+    e2[b_] += wx * d[e_];                                 \
+  }
+      int q = rt_off[ls], q1 = rt_off[ls + 1];
+      for (; q + 4 <= q1; q += 4) {
+        int c0 = rt_pk[q], c1 = rt_pk[q + 1], c2 = rt_pk[q + 2], c3 = rt_pk[q + 3];
+        T w0 = rt_w[q], w1 = rt_w[q + 1], w2 = rt_w[q + 2], w3 = rt_w[q + 3];
+        FM_B3_ADJ_TERM(c0, w0);
+        FM_B3_ADJ_TERM(c1, w1);
+        FM_B3_ADJ_TERM(c2, w2);
+        FM_B3_ADJ_TERM(c3, w3);
       }
+      for (; q < q1; q++) FM_B3_ADJ_TERM(rt_pk[q], rt_w[q]);
+#undef FM_B3_ADJ_TERM
 #pragma unroll
       for (int m = 0; m < kLM; m++) {
         if (m < n3m) Pl[m * npair + tid] = p[m];
