@@ -195,16 +195,20 @@ __device__ __forceinline__ void nbr_unkey(long long key, int sw, int &j, int &s1
 // bins (up to kStencilCap per pass) are laid out in LDS with a prefix sum of
 // their sizes, so the wave's 64 lanes take consecutive candidates across
 // bins instead of one bin at a time.
-constexpr int kNbrCap = 2048;  // keys per atom held in LDS
+// Keys per atom held in LDS: lists of up to kNbrCapSmall (every common
+// setting) use the small-LDS instance (seven workgroups per CU instead of
+// two), up to kNbrCap the large one, longer lists the fallback below.
+constexpr int kNbrCapSmall = 512;
+constexpr int kNbrCap = 2048;
 constexpr int kNbrWaves = 4;
 constexpr int kStencilCap = 128;
 
-template <bool kFill>
+template <bool kFill, int CAP = kNbrCap>
 __global__ __launch_bounds__(64 * kNbrWaves) void k_neighbors(
     int n, const double *wrapped, const int *bin_of, const int *bstart, const int *order,
     const int *species, const CellArgs *cas, const int *atom_struct, const int *bin_off,
     int *count, const int *cum, int *sidx, int *nspec, double *rel, int *max_count) {
-  __shared__ long long keys[kNbrWaves][kFill ? kNbrCap : 1];
+  __shared__ long long keys[kNbrWaves][kFill ? CAP : 1];
   __shared__ int s_off[kNbrWaves][kStencilCap + 1];
   __shared__ int s_p0[kNbrWaves][kStencilCap];
   __shared__ int s_img[kNbrWaves][kStencilCap];
@@ -293,7 +297,7 @@ __global__ __launch_bounds__(64 * kNbrWaves) void k_neighbors(
       if (kFill && ok) {
         int pre = __popcll(mask & ((1ull << lane) - 1ull));
         int slot = found + pre;
-        if (slot < kNbrCap) keys[w][slot] = nbr_key(j, s1, s2, s3, sw);
+        if (slot < CAP) keys[w][slot] = nbr_key(j, s1, s2, s3, sw);
       }
       found += __popcll(mask);
     }
@@ -301,8 +305,8 @@ __global__ __launch_bounds__(64 * kNbrWaves) void k_neighbors(
   if (!kFill) {
     if (lane == 0) {
       count[i] = found;
-      // Only lists longer than the LDS capacity need the fallback.
-      if (found > kNbrCap) atomicMax(max_count, found);
+      // Only lists longer than the small LDS capacity need another instance.
+      if (found > kNbrCapSmall) atomicMax(max_count, found);
     }
     return;
   }
@@ -471,8 +475,14 @@ void build_neighbors(fm_structure s) {
   s->sidx.alloc(pool, std::max<size_t>(1, (size_t)total * sizeof(int)));
   s->nspecies.alloc(pool, std::max<size_t>(1, (size_t)total * sizeof(int)));
   s->rel.alloc(pool, std::max<size_t>(1, 4 * (size_t)total * sizeof(double)));
-  if (max_count <= kNbrCap) {
-    hipLaunchKernelGGL(k_neighbors<true>, dim3(wblk), dim3(64 * kNbrWaves), 0, st, n,
+  if (max_count <= kNbrCapSmall) {
+    hipLaunchKernelGGL((k_neighbors<true, kNbrCapSmall>), dim3(wblk), dim3(64 * kNbrWaves), 0, st,
+                       n, s->wrapped.as<double>(), bin.as<int>(), bstart.as<int>(),
+                       order.as<int>(), s->species.as<int>(), cas, ast, boffp, nullptr,
+                       s->cumulative.as<int>(), s->sidx.as<int>(), s->nspecies.as<int>(),
+                       s->rel.as<double>(), nullptr);
+  } else if (max_count <= kNbrCap) {
+    hipLaunchKernelGGL((k_neighbors<true, kNbrCap>), dim3(wblk), dim3(64 * kNbrWaves), 0, st, n,
                        s->wrapped.as<double>(), bin.as<int>(), bstart.as<int>(), order.as<int>(),
                        s->species.as<int>(), cas, ast, boffp, nullptr, s->cumulative.as<int>(),
                        s->sidx.as<int>(), s->nspecies.as<int>(), s->rel.as<double>(), nullptr);

@@ -39,6 +39,8 @@ NBR_CASES = [
     ("triclinic3", lambda: random_cell(40, 4, nos=3, scale=8.0, skew=1.5), 4.5),
     ("small_cell_sweep2", lambda: random_cell(3, 7, nos=2, scale=3.0, skew=0.2), 5.0),
     ("big_list_fallback", lambda: random_cell(4, 8, nos=1, scale=3.0, skew=0.1), 18.0),
+    # ~800 neighbours per atom: longer than the small-LDS list, within the large one.
+    ("mid_list", lambda: random_cell(4, 9, nos=1, scale=3.0, skew=0.1), 11.0),
     ("slab", lambda: (np.diag([9.0, 9.0, 25.0]), np.zeros(20, np.int32),
                       np.random.default_rng(2).uniform(0, 1, (20, 3)) * [9, 9, 6]), 5.0),
     ("negative_positions", lambda: (np.diag([7.0, 7.5, 8.0]), np.zeros(30, np.int32),
