@@ -203,11 +203,15 @@ constexpr int kNbrCap = 2048;
 constexpr int kNbrWaves = 4;
 constexpr int kStencilCap = 128;
 
+// With kFill and `slots` set (one pass instead of count + fill): each atom's
+// count goes to count[i] and, when it fits CAP, its sorted keys to
+// slots[i CAP, ...), for k_neighbors_emit to write at the scanned offsets.
 template <bool kFill, int CAP = kNbrCap>
 __global__ __launch_bounds__(64 * kNbrWaves) void k_neighbors(
     int n, const double *wrapped, const int *bin_of, const int *bstart, const int *order,
     const int *species, const CellArgs *cas, const int *atom_struct, const int *bin_off,
-    int *count, const int *cum, int *sidx, int *nspec, double *rel, int *max_count) {
+    int *count, const int *cum, int *sidx, int *nspec, double *rel, int *max_count,
+    long long *slots = nullptr) {
   __shared__ long long keys[kNbrWaves][kFill ? CAP : 1];
   __shared__ int s_off[kNbrWaves][kStencilCap + 1];
   __shared__ int s_p0[kNbrWaves][kStencilCap];
@@ -302,13 +306,13 @@ __global__ __launch_bounds__(64 * kNbrWaves) void k_neighbors(
       found += __popcll(mask);
     }
   }
-  if (!kFill) {
+  if (!kFill || slots) {
     if (lane == 0) {
       count[i] = found;
       // Only lists longer than the small LDS capacity need another instance.
       if (found > kNbrCapSmall) atomicMax(max_count, found);
     }
-    return;
+    if (!kFill || found > CAP) return;
   }
   // Bitonic sort of the keys (padded to a power of two with the largest key).
   int np2 = 1;
@@ -335,10 +339,42 @@ __global__ __launch_bounds__(64 * kNbrWaves) void k_neighbors(
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
   }
+  if (slots) {
+    for (int t = lane; t < found; t += 64) slots[(size_t)i * CAP + t] = keys[w][t];
+    return;
+  }
   int base = cum[i];
   for (int t = lane; t < found; t += 64) {
     int j, s1, s2, s3;
     nbr_unkey(keys[w][t], sw, j, s1, s2, s3);
+    double wj[3] = {wrapped[3 * j], wrapped[3 * j + 1], wrapped[3 * j + 2]};
+    double im[3];
+    double dist = image_dist(wi, wj, ca, s1, s2, s3, im);
+    int o = base + t;
+    sidx[o] = j;
+    nspec[o] = species[j];
+    rel[4 * (size_t)o] = dist;
+    rel[4 * (size_t)o + 1] = im[0];
+    rel[4 * (size_t)o + 2] = im[1];
+    rel[4 * (size_t)o + 3] = im[2];
+  }
+}
+
+// The lists parked by the one-pass k_neighbors: one wavefront per atom writes
+// its sorted keys at cum[i] with the same image expression as the fill.
+__global__ __launch_bounds__(64 * kNbrWaves) void k_neighbors_emit(
+    int n, const double *wrapped, const int *species, const CellArgs *cas,
+    const int *atom_struct, const int *count, const int *cum, const long long *slots, int cap,
+    int *sidx, int *nspec, double *rel) {
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int i = blockIdx.x * kNbrWaves + w;
+  if (i >= n) return;
+  const CellArgs &ca = cas[atom_struct[i]];
+  int sw = ca.sweep, cnt = count[i], base = cum[i];
+  double wi[3] = {wrapped[3 * i], wrapped[3 * i + 1], wrapped[3 * i + 2]};
+  for (int t = lane; t < cnt; t += 64) {
+    int j, s1, s2, s3;
+    nbr_unkey(slots[(size_t)i * cap + t], sw, j, s1, s2, s3);
     double wj[3] = {wrapped[3 * j], wrapped[3 * j + 1], wrapped[3 * j + 2]};
     double im[3];
     double dist = image_dist(wi, wj, ca, s1, s2, s3, im);
@@ -457,10 +493,24 @@ void build_neighbors(fm_structure s) {
   hipLaunchKernelGGL(k_bin_fill, dim3(nblk), dim3(tpb), 0, st, n, bin.as<int>(), bstart.as<int>(),
                      cursor.as<int>(), order.as<int>());
   int wblk = (n + kNbrWaves - 1) / kNbrWaves;
-  hipLaunchKernelGGL(k_neighbors<false>, dim3(wblk), dim3(64 * kNbrWaves), 0, st, n,
-                     s->wrapped.as<double>(), bin.as<int>(), bstart.as<int>(), order.as<int>(),
-                     s->species.as<int>(), cas, ast, boffp, s->count.as<int>(), nullptr, nullptr,
-                     nullptr, nullptr, maxc.as<int>());
+  // One pass over the candidates that also parks each atom's sorted keys
+  // (kNbrCapSmall per atom, while that area stays within 16 GB); else a
+  // counting pass first.
+  bool one_pass = (size_t)n * kNbrCapSmall * sizeof(long long) <= (16ull << 30);
+  DevBuf slots;
+  if (one_pass) {
+    slots.alloc(pool, std::max<size_t>(1, (size_t)n * kNbrCapSmall * sizeof(long long)));
+    hipLaunchKernelGGL((k_neighbors<true, kNbrCapSmall>), dim3(wblk), dim3(64 * kNbrWaves), 0, st,
+                       n, s->wrapped.as<double>(), bin.as<int>(), bstart.as<int>(),
+                       order.as<int>(), s->species.as<int>(), cas, ast, boffp,
+                       s->count.as<int>(), nullptr, nullptr, nullptr, nullptr, maxc.as<int>(),
+                       slots.as<long long>());
+  } else {
+    hipLaunchKernelGGL(k_neighbors<false>, dim3(wblk), dim3(64 * kNbrWaves), 0, st, n,
+                       s->wrapped.as<double>(), bin.as<int>(), bstart.as<int>(), order.as<int>(),
+                       s->species.as<int>(), cas, ast, boffp, s->count.as<int>(), nullptr,
+                       nullptr, nullptr, nullptr, maxc.as<int>());
+  }
   FM_HIP(hipGetLastError());
   scan_exclusive(ctx, s->count.as<int>(), s->cumulative.as<int>(), n, false);
   // Total and largest count in pinned words [2, 4), one synchronisation
@@ -475,7 +525,13 @@ void build_neighbors(fm_structure s) {
   s->sidx.alloc(pool, std::max<size_t>(1, (size_t)total * sizeof(int)));
   s->nspecies.alloc(pool, std::max<size_t>(1, (size_t)total * sizeof(int)));
   s->rel.alloc(pool, std::max<size_t>(1, 4 * (size_t)total * sizeof(double)));
-  if (max_count <= kNbrCapSmall) {
+  if (max_count <= kNbrCapSmall && one_pass) {
+    hipLaunchKernelGGL(k_neighbors_emit, dim3(wblk), dim3(64 * kNbrWaves), 0, st, n,
+                       s->wrapped.as<double>(), s->species.as<int>(), cas, ast,
+                       s->count.as<int>(), s->cumulative.as<int>(), slots.as<long long>(),
+                       kNbrCapSmall, s->sidx.as<int>(), s->nspecies.as<int>(),
+                       s->rel.as<double>());
+  } else if (max_count <= kNbrCapSmall) {
     hipLaunchKernelGGL((k_neighbors<true, kNbrCapSmall>), dim3(wblk), dim3(64 * kNbrWaves), 0, st,
                        n, s->wrapped.as<double>(), bin.as<int>(), bstart.as<int>(),
                        order.as<int>(), s->species.as<int>(), cas, ast, boffp, nullptr,
