@@ -471,7 +471,9 @@ __device__ __forceinline__ void stage_w(const T *G, const T *B, T sg, T sb, T *w
   for (int k = done + lane; k < n; k += 64) w[k] = withb ? sg * G[k] - sb * B[k] : sg * G[k];
 }
 
-template <typename T>
+// LC >= 0: l_max known at compile time (one accumulator per l up to it, in
+// place of kMaxL + 1); LC = -1 for any l_max.
+template <typename T, int LC>
 __global__ __launch_bounds__(64 * kAtomsPerBlock) void k_dEdc(B2Args a, const T *B,
                                                                 const double *norm,
                                                                 const double *nnv, const T *G,
@@ -516,9 +518,10 @@ __global__ __launch_bounds__(64 * kAtomsPerBlock) void k_dEdc(B2Args a, const T 
   int r16 = lane & 15, k4 = lane >> 4;
   for (int ct = 0; ct < 2 * a.lmax + 1; ct += 16) {
     for (int rt = 0; rt < nr; rt += 16) {
-      typename MfmaAcc<T>::type acc[kMaxL + 1];
+      constexpr int KL = LC >= 0 ? LC : kMaxL;
+      typename MfmaAcc<T>::type acc[KL + 1];
 #pragma unroll
-      for (int l = 0; l <= kMaxL; l++) acc[l] = {};
+      for (int l = 0; l <= KL; l++) acc[l] = {};
       int n1 = rt + r16, m = ct + r16;
       for (int k0 = 0; k0 < nr; k0 += 4) {
         int n2 = k0 + k4;
@@ -529,7 +532,7 @@ __global__ __launch_bounds__(64 * kAtomsPerBlock) void k_dEdc(B2Args a, const T 
         const T *cr = c + (n2 < nr ? n2 : 0) * a.H;
         // Loads clamped in bounds and zeroed by selects (no divergent branches).
 #pragma unroll
-        for (int l = 0; l <= kMaxL; l++) {
+        for (int l = 0; l <= KL; l++) {
           int nm = 2 * l + 1;
           if (l <= a.lmax && ct < nm) {
             int mc = m < nm ? m : nm - 1;
@@ -541,7 +544,7 @@ __global__ __launch_bounds__(64 * kAtomsPerBlock) void k_dEdc(B2Args a, const T 
         }
       }
 #pragma unroll
-      for (int l = 0; l <= kMaxL; l++) {
+      for (int l = 0; l <= KL; l++) {
         int nm = 2 * l + 1;
         if (l <= a.lmax && ct < nm) {
 #pragma unroll
@@ -814,19 +817,20 @@ void launch_adjoint(fm_structure s, const void *G, int64_t g_ld, void *fbond,
   if (s->n == 0) return;
   if (s->desc_kind == 3)
     launch_dEdc_b3(s, G, g_ld, project, empty_norm, empty_nn, U.as<void>());
-  if (f64) {
-    if (s->desc_kind == 2)
-      hipLaunchKernelGGL(k_dEdc<double>, dim3(nblk1), dim3(64 * apb1), apb1 * per1, st, a,
-                         s->B.as<double>(), s->norm.as<double>(), s->nn.as<double>(),
-                         (const double *)G, (long long)g_ld, aa, s->C.as<double>(),
-                         U.as<double>());
-  } else {
-    if (s->desc_kind == 2)
-      hipLaunchKernelGGL(k_dEdc<float>, dim3(nblk1), dim3(64 * apb1), apb1 * per1, st, a,
-                         s->B.as<float>(), s->norm.as<double>(), s->nn.as<double>(),
-                         (const float *)G, (long long)g_ld, aa, s->C.as<float>(),
-                         U.as<float>());
-  }
+  if (s->desc_kind == 2)
+    with_lmax(a.lmax, [&](auto lc) {
+      constexpr int LC = decltype(lc)::value;
+      if (f64)
+        hipLaunchKernelGGL((k_dEdc<double, LC>), dim3(nblk1), dim3(64 * apb1), apb1 * per1, st,
+                           a, s->B.as<double>(), s->norm.as<double>(), s->nn.as<double>(),
+                           (const double *)G, (long long)g_ld, aa, s->C.as<double>(),
+                           U.as<double>());
+      else
+        hipLaunchKernelGGL((k_dEdc<float, LC>), dim3(nblk1), dim3(64 * apb1), apb1 * per1, st,
+                           a, s->B.as<float>(), s->norm.as<double>(), s->nn.as<double>(),
+                           (const float *)G, (long long)g_ld, aa, s->C.as<float>(),
+                           U.as<float>());
+    });
   FM_HIP(hipGetLastError());
   launch_bond_forces(s, U.as<void>(), fbond, virial_atom, nullptr, s->n, empty_norm, empty_nn);
 }
