@@ -390,7 +390,7 @@ __host__ __device__ inline int pad4(int n) { return (n + 3) & ~3; }
 // n elements from global memory to LDS: 16-byte vectors when the source is
 // 16-byte aligned (the LDS destination always is), four in flight per lane.
 template <typename T>
-__device__ __forceinline__ void stage_row(const T *src, T *dst, int n, int lane) {
+__device__ __forceinline__ void stage_row(const T *src, T *dst, int n, int lane, int nt = 64) {
   constexpr int E = 16 / sizeof(T);
   int done = 0;
   if ((reinterpret_cast<uintptr_t>(src) & 15) == 0) {
@@ -399,17 +399,18 @@ __device__ __forceinline__ void stage_row(const T *src, T *dst, int n, int lane)
     V *dv = reinterpret_cast<V *>(dst);
     int nv = n / E;
 #pragma unroll 4
-    for (int v = lane; v < nv; v += 64) dv[v] = sv[v];
+    for (int v = lane; v < nv; v += nt) dv[v] = sv[v];
     done = nv * E;
   }
-  for (int k = done + lane; k < n; k += 64) dst[k] = src[k];
+  for (int k = done + lane; k < n; k += nt) dst[k] = src[k];
 }
 
-// G . B over one row, summed over the wave: 16-byte loads of both rows when
-// both are 16-byte aligned (all of a lane's vectors in flight together), a
-// fixed order per lane and a fixed wave reduction.
+// G . B over the elements of one row that this lane's wave takes (nt lanes
+// stride the row; nt = 64: the whole row), summed over the wave: 16-byte
+// loads of both rows when both are 16-byte aligned (all of a lane's vectors
+// in flight together), a fixed order per lane and a fixed wave reduction.
 template <typename T>
-__device__ __forceinline__ T row_dot(const T *G, const T *B, int n, int lane) {
+__device__ __forceinline__ T row_dot(const T *G, const T *B, int n, int lane, int nt = 64) {
   constexpr int E = 16 / sizeof(T);
   T acc = 0;
   int done = 0;
@@ -418,7 +419,7 @@ __device__ __forceinline__ T row_dot(const T *G, const T *B, int n, int lane) {
     const V *gv = reinterpret_cast<const V *>(G), *bv = reinterpret_cast<const V *>(B);
     int nv = n / E;
 #pragma unroll 8
-    for (int v = lane; v < nv; v += 64) {
+    for (int v = lane; v < nv; v += nt) {
       V x = gv[v], y = bv[v];
       const T *xs = reinterpret_cast<const T *>(&x), *ys = reinterpret_cast<const T *>(&y);
 #pragma unroll
@@ -426,7 +427,7 @@ __device__ __forceinline__ T row_dot(const T *G, const T *B, int n, int lane) {
     }
     done = nv * E;
   }
-  for (int k = done + lane; k < n; k += 64) acc += G[k] * B[k];
+  for (int k = done + lane; k < n; k += nt) acc += G[k] * B[k];
   return wave_sum(acc);
 }
 
@@ -435,7 +436,7 @@ __device__ __forceinline__ T row_dot(const T *G, const T *B, int n, int lane) {
 // in flight per lane; B is not read when sb is zero.
 template <typename T>
 __device__ __forceinline__ void stage_w(const T *G, const T *B, T sg, T sb, T *w, int n,
-                                        int lane) {
+                                        int lane, int nt = 64) {
   constexpr int E = 16 / sizeof(T);
   int done = 0;
   bool withb = sb != T(0);
@@ -447,7 +448,7 @@ __device__ __forceinline__ void stage_w(const T *G, const T *B, T sg, T sb, T *w
     int nv = n / E;
     if (withb) {
 #pragma unroll 4
-      for (int v = lane; v < nv; v += 64) {
+      for (int v = lane; v < nv; v += nt) {
         V x = gv[v], y = bv[v], r;
         const T *xs = reinterpret_cast<const T *>(&x), *ys = reinterpret_cast<const T *>(&y);
         T *rs = reinterpret_cast<T *>(&r);
@@ -457,7 +458,7 @@ __device__ __forceinline__ void stage_w(const T *G, const T *B, T sg, T sb, T *w
       }
     } else {
 #pragma unroll 4
-      for (int v = lane; v < nv; v += 64) {
+      for (int v = lane; v < nv; v += nt) {
         V x = gv[v], r;
         const T *xs = reinterpret_cast<const T *>(&x);
         T *rs = reinterpret_cast<T *>(&r);
@@ -468,27 +469,32 @@ __device__ __forceinline__ void stage_w(const T *G, const T *B, T sg, T sb, T *w
     }
     done = nv * E;
   }
-  for (int k = done + lane; k < n; k += 64) w[k] = withb ? sg * G[k] - sb * B[k] : sg * G[k];
+  for (int k = done + lane; k < n; k += nt) w[k] = withb ? sg * G[k] - sb * B[k] : sg * G[k];
 }
 
 // LC >= 0: l_max known at compile time (one accumulator per l up to it, in
-// place of kMaxL + 1); LC = -1 for any l_max.
+// place of kMaxL + 1); LC = -1 for any l_max. Two wavefronts per atom share
+// its LDS rows: both stage (twice the loads in flight per atom for the same
+// LDS) and each takes half of the 16-row output tiles.
 template <typename T, int LC>
-__global__ __launch_bounds__(64 * kAtomsPerBlock) void k_dEdc(B2Args a, const T *B,
-                                                                const double *norm,
-                                                                const double *nnv, const T *G,
-                                                                long long g_ld, AdjArgs aa,
-                                                                const T *Cg, T *Ug) {
+__global__ __launch_bounds__(128 * kAtomsPerBlock) void k_dEdc(B2Args a, const T *B,
+                                                                 const double *norm,
+                                                                 const double *nnv, const T *G,
+                                                                 long long g_ld, AdjArgs aa,
+                                                                 const T *Cg, T *Ug) {
   extern __shared__ __align__(16) unsigned char smem[];
-  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  int i = blockIdx.x * (blockDim.x >> 6) + w;
-  if (i >= a.n) return;
+  __shared__ T red[2 * kAtomsPerBlock];
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6, slot = w >> 1, sub = w & 1;
+  int t2 = threadIdx.x & 127;  // thread of the atom's two wavefronts
+  int i = blockIdx.x * (blockDim.x >> 7) + slot;
   int nc = a.nrad * a.H, ncp = pad4(nc), ndp = pad4(a.nd);
-  T *c = reinterpret_cast<T *>(smem) + (size_t)w * ((size_t)ncp + (size_t)ndp);
+  T *c = reinterpret_cast<T *>(smem) + (size_t)slot * ((size_t)ncp + (size_t)ndp);
   T *wl = c + ncp;  // the w = dE/dB row
-  int row = a.row_of_atom[i];
-  double nrm = norm[row];
-  if (nrm < aa.empty_norm || nnv[row] < aa.empty_nn) return;
+  // Atoms past the end or empty skip the work but stay for the barriers.
+  bool live = i < a.n;
+  int row = live ? a.row_of_atom[i] : 0;
+  double nrm = live ? norm[row] : 1.0;
+  live = live && !(nrm < aa.empty_norm || nnv[row] < aa.empty_nn);
   const T *Brow = B + (size_t)row * a.nd;
   const T *Grow = G + (size_t)row * g_ld;
   // w = dE/dB = (G - (G . B_hat) B_hat) / |B| = sg G - sb B.
@@ -496,17 +502,21 @@ __global__ __launch_bounds__(64 * kAtomsPerBlock) void k_dEdc(B2Args a, const T 
   T sg = aa.project ? ninv : T(1), sb = 0;
   if (aa.project) {
     if (aa.energy) {
-      sb = (T)(aa.power * aa.energy[row] / (nrm * nrm));
+      if (live) sb = (T)(aa.power * aa.energy[row] / (nrm * nrm));
     } else {
-      sb = row_dot(Grow, Brow, a.nd, lane) * ninv * ninv * ninv;
+      T part = live ? row_dot(Grow, Brow, a.nd, t2, 128) : T(0);
+      if (lane == 0) red[w] = part;
+      __syncthreads();
+      sb = (red[2 * slot] + red[2 * slot + 1]) * ninv * ninv * ninv;
     }
   }
   // One pass over HBM with 16-byte loads: c and w to LDS.
-  stage_row(Cg + (size_t)i * nc, c, nc, lane);
-  stage_w(Grow, Brow, sg, sb, wl, a.nd, lane);
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  if (live) {
+    stage_row(Cg + (size_t)i * nc, c, nc, t2, 128);
+    stage_w(Grow, Brow, sg, sb, wl, a.nd, t2, 128);
+  }
+  __syncthreads();
+  if (!live) return;
   // u_l = W_l C_l per l on the matrix cores: W_l[n1][n2] = (1 + [n1 = n2])
   // w[p(min, max), l] (nr x nr, symmetric), C_l[n2][m] = c[n2, l^2 + m]
   // (nr x (2l + 1)); 16 x 16 output tiles, 4 n2 per MFMA. Lane L supplies
@@ -517,7 +527,7 @@ __global__ __launch_bounds__(64 * kAtomsPerBlock) void k_dEdc(B2Args a, const T 
   T *urow = Ug + (size_t)i * nc;
   int r16 = lane & 15, k4 = lane >> 4;
   for (int ct = 0; ct < 2 * a.lmax + 1; ct += 16) {
-    for (int rt = 0; rt < nr; rt += 16) {
+    for (int rt = 16 * sub; rt < nr; rt += 32) {
       constexpr int KL = LC >= 0 ? LC : kMaxL;
       typename MfmaAcc<T>::type acc[KL + 1];
 #pragma unroll
@@ -821,12 +831,12 @@ void launch_adjoint(fm_structure s, const void *G, int64_t g_ld, void *fbond,
     with_lmax(a.lmax, [&](auto lc) {
       constexpr int LC = decltype(lc)::value;
       if (f64)
-        hipLaunchKernelGGL((k_dEdc<double, LC>), dim3(nblk1), dim3(64 * apb1), apb1 * per1, st,
+        hipLaunchKernelGGL((k_dEdc<double, LC>), dim3(nblk1), dim3(128 * apb1), apb1 * per1, st,
                            a, s->B.as<double>(), s->norm.as<double>(), s->nn.as<double>(),
                            (const double *)G, (long long)g_ld, aa, s->C.as<double>(),
                            U.as<double>());
       else
-        hipLaunchKernelGGL((k_dEdc<float, LC>), dim3(nblk1), dim3(64 * apb1), apb1 * per1, st,
+        hipLaunchKernelGGL((k_dEdc<float, LC>), dim3(nblk1), dim3(128 * apb1), apb1 * per1, st,
                            a, s->B.as<float>(), s->norm.as<double>(), s->nn.as<double>(),
                            (const float *)G, (long long)g_ld, aa, s->C.as<float>(),
                            U.as<float>());
