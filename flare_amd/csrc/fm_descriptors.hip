@@ -160,34 +160,35 @@ __device__ __forceinline__ int mfma_row(int lane, int q) {
   return sizeof(T) == 8 ? (lane >> 4) + 4 * q : 4 * (lane >> 4) + q;
 }
 
-// Bonds per table chunk of the descriptor kernel (lanes [0, kBondChunk) write
-// the tables; a smaller chunk means a smaller LDS workspace per atom and so
-// more atoms resident per CU).
-constexpr int kBondChunk = 32;
+// Bonds per table chunk of the descriptor kernel, CH (a template parameter):
+// lanes [0, CH) write the tables, so a 32-bond chunk leaves half the wave
+// idle in the harmonic sweep and a 64-bond chunk doubles the tables' LDS
+// (fewer atoms resident per CU). launch_b2 picks 64 where that costs at
+// most a third more LDS per atom, else 32.
 
 // Workspace (elements of T) of one atom of the descriptor kernel: c, then the
-// bond tables of one chunk (Y kBondChunk x H, g kBondChunk x N).
-__host__ __device__ inline size_t b2_work(int nc, int H, int N) {
-  return (size_t)nc + (size_t)kBondChunk * ((size_t)H + (size_t)N);
+// bond tables of one chunk (Y ch x H, g ch x N).
+__host__ __device__ inline size_t b2_work(int nc, int H, int N, int ch) {
+  return (size_t)nc + (size_t)ch * ((size_t)H + (size_t)N);
 }
 
 // c[(t N + n) H + lm] of atom i for all neighbour species t into LDS
-// (b2.cpp:367-395). For each chunk of up to kBondChunk bonds, lane k takes
+// (b2.cpp:367-395). For each chunk of up to CH bonds, lane k takes
 // bond k and writes its radial basis g_n and every Y_lm to LDS tables, the
 // rows grouped by neighbour species (list order kept within a species); then,
 // per species t and 16 harmonics, c_t += G_t^T Y_t on the matrix cores over
 // that species' rows only (A = g_n of the bonds, B = their Y_lm).
-template <typename T, int LC>
+template <typename T, int LC, int CH>
 __device__ void accumulate_c(const B2Args &a, int i, int lane, T *c, T *tab) {
   int si = a.species[i];
   int b0 = a.bond_off[i], b1 = a.bond_off[i + 1];
   int nc = a.nrad * a.H, H = a.H, N = a.N;
-  T *Yt = tab, *gt = tab + kBondChunk * (size_t)H;
+  T *Yt = tab, *gt = tab + CH * (size_t)H;
   for (int k = lane; k < nc; k += 64) c[k] = 0;
   int r16 = lane & 15, k4 = lane >> 4;
   unsigned long long below = (1ull << lane) - 1ull;
-  for (int bb = b0; bb < b1; bb += kBondChunk) {
-    int nk = min(kBondChunk, b1 - bb);
+  for (int bb = b0; bb < b1; bb += CH) {
+    int nk = min(CH, b1 - bb);
     int q = 0, sj = -1;
     if (lane < nk) {
       q = a.bond_q[bb + lane];
@@ -239,7 +240,7 @@ __device__ void accumulate_c(const B2Args &a, int i, int lane, T *c, T *tab) {
 #pragma unroll
             for (int u = 0; u < 4; u++) {
               int b = k0 + 4 * u + k4;
-              int bc = b < kBondChunk ? b : kBondChunk - 1;
+              int bc = b < CH ? b : CH - 1;
               T x = gt[bc * N + rc], y = Yt[bc * H + lc];
               av[u] = (b < r1 && r16 < N) ? x : T(0);
               bv[u] = (b < r1 && lm < H) ? y : T(0);
@@ -272,7 +273,7 @@ struct SplitOut {
   __bf16 *p[2];
 };
 
-template <typename T, int LC>
+template <typename T, int LC, int CH>
 __global__ __launch_bounds__(64 * kAtomsPerBlock) void k_b2(B2Args a, T *B, double *norm,
                                                               double *nn, SplitOut Bs, int ldp,
                                                               T *Cg, int want_b2, int work) {
@@ -283,7 +284,7 @@ __global__ __launch_bounds__(64 * kAtomsPerBlock) void k_b2(B2Args a, T *B, doub
   int nc = a.nrad * a.H, H = a.H, nr = a.nrad;
   T *c = reinterpret_cast<T *>(smem) + (size_t)w * work;  // work: elements per atom
   T *tab = c + nc;
-  accumulate_c<T, LC>(a, i, lane, c, tab);
+  accumulate_c<T, LC, CH>(a, i, lane, c, tab);
   for (int k = lane; k < nc; k += 64) Cg[(size_t)i * nc + k] = c[k];
   if (!want_b2) return;
   // B_l = C_l C_l^T on the matrix cores (C_l[n][m] = c[n, l^2 + m]); the
@@ -774,10 +775,15 @@ void launch_b2(fm_structure s, bool want_b2) {
   upload_rcp(st);
   B2Args a = make_args(s);
   size_t esz = s->form == FM_FORM_FP64 ? sizeof(double) : sizeof(float);
-  size_t work = b2_work(a.nrad * a.H, a.H, a.N);
   // FP32 stages the B row after c (over the bond tables).
-  if (want_b2 && s->form != FM_FORM_FP64)
-    work = std::max(work, (size_t)a.nrad * a.H + (size_t)a.nd);
+  auto work_of = [&](int ch) {
+    size_t w = b2_work(a.nrad * a.H, a.H, a.N, ch);
+    if (want_b2 && s->form != FM_FORM_FP64)
+      w = std::max(w, (size_t)a.nrad * a.H + (size_t)a.nd);
+    return w;
+  };
+  bool wide = 3 * work_of(64) <= 4 * work_of(32);
+  size_t work = work_of(wide ? 64 : 32);
   size_t per = work * esz;
   int apb = atoms_per_block(per);
   size_t lds = apb * per;
@@ -789,20 +795,27 @@ void launch_b2(fm_structure s, bool want_b2) {
   // The unrolled sweep (compile-time l_max) is faster for the FP64 kernel;
   // for FP32 its extra registers cost more occupancy than it saves at
   // l_max 6.
-  if (s->form == FM_FORM_FP64)
-    with_lmax(a.lmax, [&](auto lc) {
-      constexpr int LC = decltype(lc)::value;
-      hipLaunchKernelGGL((k_b2<double, LC>), dim3(nblk), dim3(64 * apb), lds, st, a,
-                         s->B.as<double>(), s->norm.as<double>(), s->nn.as<double>(), none, 0,
-                         s->C.as<double>(), (int)want_b2, (int)work);
-    });
+  auto go = [&](auto ch) {
+    constexpr int CH = decltype(ch)::value;
+    if (s->form == FM_FORM_FP64)
+      with_lmax(a.lmax, [&](auto lc) {
+        constexpr int LC = decltype(lc)::value;
+        hipLaunchKernelGGL((k_b2<double, LC, CH>), dim3(nblk), dim3(64 * apb), lds, st, a,
+                           s->B.as<double>(), s->norm.as<double>(), s->nn.as<double>(), none, 0,
+                           s->C.as<double>(), (int)want_b2, (int)work);
+      });
+    else
+      with_lmax(a.lmax <= 4 ? a.lmax : -1, [&](auto lc) {
+        constexpr int LC = decltype(lc)::value;
+        hipLaunchKernelGGL((k_b2<float, LC, CH>), dim3(nblk), dim3(64 * apb), lds, st, a,
+                           s->B.as<float>(), s->norm.as<double>(), s->nn.as<double>(), pieces,
+                           (int)split_ld(s->nd), s->C.as<float>(), (int)want_b2, (int)work);
+      });
+  };
+  if (wide)
+    go(std::integral_constant<int, 64>{});
   else
-    with_lmax(a.lmax <= 4 ? a.lmax : -1, [&](auto lc) {
-      constexpr int LC = decltype(lc)::value;
-      hipLaunchKernelGGL((k_b2<float, LC>), dim3(nblk), dim3(64 * apb), lds, st, a,
-                         s->B.as<float>(), s->norm.as<double>(), s->nn.as<double>(), pieces,
-                         (int)split_ld(s->nd), s->C.as<float>(), (int)want_b2, (int)work);
-    });
+    go(std::integral_constant<int, 32>{});
   FM_HIP(hipGetLastError());
 }
 
