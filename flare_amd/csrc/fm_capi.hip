@@ -509,27 +509,8 @@ static void compute_descriptor(fm_structure s, const fm_b2_settings *st_in, int 
       if (kind == 3 && s->row_tile == 0) fm::launch_b3(s);
     }
     ctx->mark_end(fm::ST_DESC);
-    // Per-structure species-sorted position of every global row (the
-    // identity for a single structure).
-    s->var_pos.resize(s->n);
-    for (int r = 0; r < s->n; r++) s->var_pos[r] = r;
-    if (s->n > 0 && s->n_struct > 1) {
-      std::vector<int> sp(s->n);
-      FM_HIP(hipMemcpyAsync(sp.data(), s->species.as<int>(), s->n * sizeof(int),
-                            hipMemcpyDeviceToHost, stream));
-      FM_HIP(hipStreamSynchronize(stream));
-      std::vector<int> grank(s->nos, 0);
-      for (int k = 0; k < s->n_struct; k++) {
-        std::vector<int> cnt(s->nos, 0), off(s->nos, 0), rk(s->nos, 0);
-        for (int i = s->atom_off[k]; i < s->atom_off[k + 1]; i++) cnt[sp[i]]++;
-        for (int t = 1; t < s->nos; t++) off[t] = off[t - 1] + cnt[t - 1];
-        for (int i = s->atom_off[k]; i < s->atom_off[k + 1]; i++) {
-          int t = sp[i];
-          int row = s->type_off[t] + grank[t]++;
-          s->var_pos[row] = s->atom_off[k] + off[t] + rk[t]++;
-        }
-      }
-    }
+    // Per-structure species-sorted position of every row (batches only).
+    fm::build_var_pos(s);
     s->nb_by_type.clear();  // filled on first export (nb_by_type_of)
     s->has_dervs = false;    // dense derivatives are rebuilt on demand
     s->DB.release();

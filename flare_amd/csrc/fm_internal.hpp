@@ -152,8 +152,9 @@ struct fm_structure_s {
   int n_chunks = 0;
   fm::DevBuf d_chunk_begin, d_chunk_end, d_struct_chunk_off;
   // Position of each species-sorted row in the per-structure species-sorted
-  // order of the local variances (identity for a single structure).
-  std::vector<int> var_pos;
+  // order of the local variances (device ints; empty for a single
+  // structure, whose order is the identity).
+  fm::DevBuf var_pos;
   double cell[9], cell_t_inv[9], cell_dot[9], cell_dot_inv[9];
   double volume = 0, single_sweep_cutoff = 0, cutoff = 0;
   int sweep = 0;
@@ -233,6 +234,8 @@ void build_neighbors(fm_structure s);
 void build_bonds(fm_structure s);
 // fm_descriptors.hip (want_b2 = false computes and keeps only c)
 void launch_b2(fm_structure s, bool want_b2 = true);
+// Device var_pos of a batch (after build_bonds); releases it for one structure.
+void build_var_pos(fm_structure s);
 // fm_b3.hip
 int b3_size(int nos, int N, int lmax);
 void launch_b3(fm_structure s);

@@ -145,34 +145,90 @@ static void gather_forces(fm_structure s, bool f64, DevBuf &fbond, DevBuf &vir, 
                      s->d_mean_off.as<long long>(), s->d_atom_off.as<int>(), o);
 }
 
-// Results to the caller's host buffers (any may be null).
+// out[pos[r]] = in[r]: species-sorted rows to another order on the device.
+__global__ void k_scatter_rows(int n, const int *pos, const double *in, double *out) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n) out[pos[r]] = in[r];
+}
+
+// Atoms of each (species, structure) of a batch, species-major.
+__global__ void k_struct_species_count(int n, const int *species, const int *atom_struct,
+                                       int n_struct, int *cnt) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) atomicAdd(&cnt[(size_t)species[i] * n_struct + atom_struct[i]], 1);
+}
+
+// Position of species-sorted row r in its structure's species-sorted order.
+// Rows of one species keep atom order, so structure k's atoms of species t
+// are the rows [ex[t][k], ex[t][k] + cnt[t][k]), ex the exclusive scan of
+// the species-major counts.
+__global__ void k_var_pos(int n, const int *atom_of_row, const int *species,
+                          const int *atom_struct, const int *atom_off, const int *cnt,
+                          const int *ex, int n_struct, int *var_pos) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  int i = atom_of_row[r], t = species[i], k = atom_struct[i];
+  int before = 0;
+  for (int u = 0; u < t; u++) before += cnt[(size_t)u * n_struct + k];
+  var_pos[r] = atom_off[k] + before + (r - ex[(size_t)t * n_struct + k]);
+}
+
+void build_var_pos(fm_structure s) {
+  fm_context ctx = s->ctx;
+  hipStream_t st = ctx->stream;
+  int n = s->n, ns = s->n_struct;
+  if (ns <= 1 || n == 0) {
+    s->var_pos.release();
+    return;
+  }
+  int m = s->nos * ns;
+  DevBuf cnt(&ctx->pool, (size_t)m * sizeof(int)), ex(&ctx->pool, (size_t)(m + 1) * sizeof(int));
+  FM_HIP(hipMemsetAsync(cnt.as<int>(), 0, (size_t)m * sizeof(int), st));
+  hipLaunchKernelGGL(k_struct_species_count, dim3((n + 255) / 256), dim3(256), 0, st, n,
+                     s->species.as<int>(), s->atom_struct.as<int>(), ns, cnt.as<int>());
+  scan_exclusive(ctx, cnt.as<int>(), ex.as<int>(), m, false);
+  s->var_pos.alloc(&ctx->pool, (size_t)n * sizeof(int));
+  hipLaunchKernelGGL(k_var_pos, dim3((n + 255) / 256), dim3(256), 0, st, n,
+                     s->atom_of_row.as<int>(), s->species.as<int>(), s->atom_struct.as<int>(),
+                     s->d_atom_off.as<int>(), cnt.as<int>(), ex.as<int>(), ns,
+                     s->var_pos.as<int>());
+  FM_HIP(hipGetLastError());
+}
+
+// Results to the caller's host buffers (any may be null); returns after the
+// copies have finished.
 static void read_outputs(fm_structure s, const double *o, size_t n_out, const double *var_row,
                          const double *E_row, double *mean_efs, double *local_var,
                          double *local_energy) {
   hipStream_t st = s->ctx->stream;
+  Pool *pool = &s->ctx->pool;
   int n = s->n;
+  DevBuf vperm, eperm;
   if (mean_efs)
     FM_HIP(hipMemcpyAsync(mean_efs, o, (size_t)n_out * sizeof(double), hipMemcpyDeviceToHost, st));
   if (local_var && var_row && n > 0) {
     // Species-sorted rows of the batch to each structure's species-sorted
     // order (compute_cluster_uncertainties order).
-    std::vector<double> vr(n);
-    FM_HIP(hipMemcpyAsync(vr.data(), var_row, (size_t)n * sizeof(double),
-                          hipMemcpyDeviceToHost, st));
-    FM_HIP(hipStreamSynchronize(st));
-    for (int r = 0; r < n; r++) local_var[s->var_pos[r]] = vr[r];
+    const double *src = var_row;
+    if (s->n_struct > 1) {
+      vperm.alloc(pool, (size_t)n * sizeof(double));
+      hipLaunchKernelGGL(k_scatter_rows, dim3((n + 255) / 256), dim3(256), 0, st, n,
+                         s->var_pos.as<int>(), var_row, vperm.as<double>());
+      src = vperm.as<double>();
+    }
+    FM_HIP(hipMemcpyAsync(local_var, src, (size_t)n * sizeof(double), hipMemcpyDeviceToHost,
+                          st));
   }
   if (local_energy && n > 0) {
-    // E_row is in species-sorted order; map to atom order on the host.
-    std::vector<double> er(n);
-    std::vector<int> aor(n);
-    FM_HIP(hipMemcpyAsync(er.data(), E_row, (size_t)n * sizeof(double),
+    // E_row is in species-sorted order; local energies are in atom order.
+    eperm.alloc(pool, (size_t)n * sizeof(double));
+    hipLaunchKernelGGL(k_scatter_rows, dim3((n + 255) / 256), dim3(256), 0, st, n,
+                       s->atom_of_row.as<int>(), E_row, eperm.as<double>());
+    FM_HIP(hipMemcpyAsync(local_energy, eperm.as<double>(), (size_t)n * sizeof(double),
                           hipMemcpyDeviceToHost, st));
-    FM_HIP(hipMemcpyAsync(aor.data(), s->atom_of_row.as<int>(), (size_t)n * sizeof(int),
-                          hipMemcpyDeviceToHost, st));
-    FM_HIP(hipStreamSynchronize(st));
-    for (int r = 0; r < n; r++) local_energy[aor[r]] = er[r];
   }
+  FM_HIP(hipGetLastError());
+  FM_HIP(hipStreamSynchronize(st));
 }
 
 
