@@ -473,11 +473,69 @@ __device__ __forceinline__ void stage_w(const T *G, const T *B, T sg, T sb, T *w
   for (int k = done + lane; k < n; k += nt) w[k] = withb ? sg * G[k] - sb * B[k] : sg * G[k];
 }
 
+// FP32 rows: G . B and w = sg G - sb B (sb = (G . B) s3) in one pass over
+// both rows. Each of the atom's 128 threads stores its V-vectors of G to LDS
+// as they arrive and keeps those of B (at most R, rows of at most 128 R E
+// elements) in registers until sb is known; the partition and summation
+// order of the dot are those of row_dot. red holds the two wavefronts'
+// partial dots; the barrier inside is reached by every thread of the block
+// (dead atoms load nothing).
+template <typename V, int R>
+__device__ __forceinline__ void stage_w_onepass(const float *G, const float *B, float sg,
+                                                float s3, float *wl, int n, bool live, int t2,
+                                                int lane, int w, int slot, float *red) {
+  constexpr int E = sizeof(V) / sizeof(float);
+  const V *gv = reinterpret_cast<const V *>(G), *bv = reinterpret_cast<const V *>(B);
+  V *wv = reinterpret_cast<V *>(wl);
+  int nv = live ? n / E : 0;
+  V b[R];
+  float acc = 0;
+#pragma unroll
+  for (int j = 0; j < R; j++) {
+    int v = t2 + 128 * j;
+    if (v < nv) {
+      V g = gv[v];
+      b[j] = bv[v];
+      const float *xs = reinterpret_cast<const float *>(&g);
+      const float *ys = reinterpret_cast<const float *>(&b[j]);
+#pragma unroll
+      for (int e = 0; e < E; e++) acc += xs[e] * ys[e];
+      wv[v] = g;
+    }
+  }
+  int kt = nv * E + t2;
+  bool tail = live && kt < n;
+  float gt = tail ? G[kt] : 0.f, bt = tail ? B[kt] : 0.f;
+  acc += gt * bt;
+  acc = wave_sum(acc);
+  if (lane == 0) red[w] = acc;
+  __syncthreads();
+  float sb = (red[2 * slot] + red[2 * slot + 1]) * s3;
+#pragma unroll
+  for (int j = 0; j < R; j++) {
+    int v = t2 + 128 * j;
+    if (v < nv) {
+      V r = wv[v];
+      float *rs = reinterpret_cast<float *>(&r);
+      const float *ys = reinterpret_cast<const float *>(&b[j]);
+#pragma unroll
+      for (int e = 0; e < E; e++) rs[e] = sg * rs[e] - sb * ys[e];
+      wv[v] = r;
+    }
+  }
+  if (tail) wl[kt] = sg * gt - sb * bt;
+}
+
+// Longest row for the one-pass staging of k_dEdc<float, LC, true>.
+constexpr int kOnePassRow = 128 * 32;
+
 // LC >= 0: l_max known at compile time (one accumulator per l up to it, in
-// place of kMaxL + 1); LC = -1 for any l_max. Two wavefronts per atom share
+// place of kMaxL + 1); LC = -1 for any l_max. OP (FP32 only):
+// projection from the rows with one pass over G and B (more registers;
+// chosen where LDS, not registers, bounds the atoms per CU). Two wavefronts per atom share
 // its LDS rows: both stage (twice the loads in flight per atom for the same
 // LDS) and each takes half of the 16-row output tiles.
-template <typename T, int LC>
+template <typename T, int LC, bool OP = false>
 __global__ __launch_bounds__(128 * kAtomsPerBlock) void k_dEdc(B2Args a, const T *B,
                                                                  const double *norm,
                                                                  const double *nnv, const T *G,
@@ -501,20 +559,35 @@ __global__ __launch_bounds__(128 * kAtomsPerBlock) void k_dEdc(B2Args a, const T
   // w = dE/dB = (G - (G . B_hat) B_hat) / |B| = sg G - sb B.
   T ninv = T(1) / (T)nrm;
   T sg = aa.project ? ninv : T(1), sb = 0;
-  if (aa.project) {
-    if (aa.energy) {
-      if (live) sb = (T)(aa.power * aa.energy[row] / (nrm * nrm));
-    } else {
-      T part = live ? row_dot(Grow, Brow, a.nd, t2, 128) : T(0);
-      if (lane == 0) red[w] = part;
-      __syncthreads();
-      sb = (red[2 * slot] + red[2 * slot + 1]) * ninv * ninv * ninv;
+  if constexpr (OP && sizeof(T) == 4) {
+    // The launch guarantees project, no energies and nd <= kOnePassRow.
+    if (live) stage_row(Cg + (size_t)i * nc, c, nc, t2, 128);
+    // Every row 16-byte aligned: the same for all atoms of the grid, so the
+    // whole block takes one branch and one barrier.
+    bool al = ((reinterpret_cast<uintptr_t>(G) | reinterpret_cast<uintptr_t>(B)) & 15) == 0 &&
+              (g_ld & 3) == 0 && (a.nd & 3) == 0;
+    if (al)
+      stage_w_onepass<float4, kOnePassRow / 512>(Grow, Brow, sg, ninv * ninv * ninv, wl, a.nd,
+                                                 live, t2, lane, w, slot, red);
+    else
+      stage_w_onepass<float, kOnePassRow / 128>(Grow, Brow, sg, ninv * ninv * ninv, wl, a.nd,
+                                                live, t2, lane, w, slot, red);
+  } else {
+    if (aa.project) {
+      if (aa.energy) {
+        if (live) sb = (T)(aa.power * aa.energy[row] / (nrm * nrm));
+      } else {
+        T part = live ? row_dot(Grow, Brow, a.nd, t2, 128) : T(0);
+        if (lane == 0) red[w] = part;
+        __syncthreads();
+        sb = (red[2 * slot] + red[2 * slot + 1]) * ninv * ninv * ninv;
+      }
     }
-  }
-  // One pass over HBM with 16-byte loads: c and w to LDS.
-  if (live) {
-    stage_row(Cg + (size_t)i * nc, c, nc, t2, 128);
-    stage_w(Grow, Brow, sg, sb, wl, a.nd, t2, 128);
+    // One pass over HBM with 16-byte loads: c and w to LDS.
+    if (live) {
+      stage_row(Cg + (size_t)i * nc, c, nc, t2, 128);
+      stage_w(Grow, Brow, sg, sb, wl, a.nd, t2, 128);
+    }
   }
   __syncthreads();
   if (!live) return;
@@ -840,6 +913,11 @@ void launch_adjoint(fm_structure s, const void *G, int64_t g_ld, void *fbond,
   if (s->n == 0) return;
   if (s->desc_kind == 3)
     launch_dEdc_b3(s, G, g_ld, project, empty_norm, empty_nn, U.as<void>());
+  // FP32 projection from the rows in one pass over G and B where LDS
+  // already holds the CU to two blocks (the one-pass registers cost no
+  // occupancy there; with three or more blocks they would).
+  bool one_pass = !f64 && project && !energy && a.nd <= kOnePassRow &&
+                  3 * (size_t)apb1 * per1 > (size_t)160 * 1024;
   if (s->desc_kind == 2)
     with_lmax(a.lmax, [&](auto lc) {
       constexpr int LC = decltype(lc)::value;
@@ -848,6 +926,11 @@ void launch_adjoint(fm_structure s, const void *G, int64_t g_ld, void *fbond,
                            a, s->B.as<double>(), s->norm.as<double>(), s->nn.as<double>(),
                            (const double *)G, (long long)g_ld, aa, s->C.as<double>(),
                            U.as<double>());
+      else if (one_pass)
+        hipLaunchKernelGGL((k_dEdc<float, LC, true>), dim3(nblk1), dim3(128 * apb1),
+                           apb1 * per1, st, a, s->B.as<float>(), s->norm.as<double>(),
+                           s->nn.as<double>(), (const float *)G, (long long)g_ld, aa,
+                           s->C.as<float>(), U.as<float>());
       else
         hipLaunchKernelGGL((k_dEdc<float, LC>), dim3(nblk1), dim3(128 * apb1), apb1 * per1, st,
                            a, s->B.as<float>(), s->norm.as<double>(), s->nn.as<double>(),

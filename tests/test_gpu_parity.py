@@ -213,16 +213,24 @@ def test_predict_scalable_medium(C, oracle):
     assert np.abs(F.sum(0)).max() < 1e-9 * np.abs(F).max() * n
 
 
-def test_chosen_form_error_budget(C, oracle):
+# B2 settings of the chosen-form budget test: d = 2100 (adjoint rows staged
+# in two passes), d = 3255 (one pass, rows not 16-byte aligned) and d = 3696
+# (one pass, aligned rows).
+BUDGET_CASES = [[2, 12, 6], [3, 10, 6], [2, 16, 6]]
+
+
+@pytest.mark.parametrize("settings", BUDGET_CASES)
+def test_chosen_form_error_budget(C, oracle, settings):
     """FP32 descriptors + split-BF16x3 contraction against the FP64 path on
     the same model: energy per atom, force and variance errors."""
     from flare_amd.workloads import FCC_BASIS, lattice
     cell, pos = lattice(FCC_BASIS, 4.0, (6, 6, 6))
     rng = np.random.default_rng(8)
-    species = rng.integers(0, 2, len(pos)).astype(np.int32)
+    nos = settings[0]
+    species = rng.integers(0, nos, len(pos)).astype(np.int32)
     pos = pos + rng.normal(scale=0.08, size=pos.shape)
-    cut = np.full((2, 2), 6.0)
-    calc = C.B2("chebyshev", "quadratic", [0.0, 6.0], [], [2, 12, 6], cut)
+    cut = np.full((nos, nos), 6.0)
+    calc = C.B2("chebyshev", "quadratic", [0.0, 6.0], [], settings, cut)
     s64 = gpu_structure(C, cell, species, pos, 6.0, calc, form=0)
     d = s64.descriptors[0]
     picks = [list(rng.choice(n, 150, replace=False)) for n in d.n_clusters_by_type]
