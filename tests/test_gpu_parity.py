@@ -252,6 +252,31 @@ def test_chosen_form_error_budget(C, oracle, settings):
     assert dE < 1e-3
 
 
+@pytest.mark.parametrize("shape", range(7))
+def test_split_tile_shapes_bitwise(C, monkeypatch, shape):
+    """Each tile shape of the split contraction, pinned for every contraction
+    that admits it, gives the autotuned evaluation's results bit for bit
+    (every output element sums its k terms in the same order)."""
+    from flare_amd.workloads import FCC_BASIS, lattice
+    cell, pos = lattice(FCC_BASIS, 4.0, (7, 7, 7))
+    rng = np.random.default_rng(12)
+    species = rng.integers(0, 2, len(pos)).astype(np.int32)
+    pos = pos + rng.normal(scale=0.08, size=pos.shape)
+    cut = np.full((2, 2), 5.0)
+    calc = C.B2("chebyshev", "quadratic", [0.0, 5.0], [], [2, 8, 4], cut)
+    s64 = gpu_structure(C, cell, species, pos, 5.0, calc, form=0)
+    d = s64.descriptors[0]
+    picks = [list(rng.choice(n, 300, replace=False)) for n in d.n_clusters_by_type]
+    sgp = make_sgp(C, s64, picks, 2.0, 2.0, seed=4, calc=calc)
+    s0 = gpu_structure(C, cell, species, pos, 5.0, calc, form=1)
+    sgp.predict_local_uncertainties(s0)
+    monkeypatch.setenv("FLARE_MI_TILE_SHAPE", str(shape))
+    s1 = gpu_structure(C, cell, species, pos, 5.0, calc, form=1)
+    sgp.predict_local_uncertainties(s1)
+    assert np.array_equal(s1.mean_efs, s0.mean_efs)
+    assert np.array_equal(s1.local_uncertainties[0], s0.local_uncertainties[0])
+
+
 def test_determinism(C):
     cell, species, pos = random_cell(60, 21, nos=2, scale=9.0, skew=0.4)
     cut = np.full((2, 2), 5.0)
