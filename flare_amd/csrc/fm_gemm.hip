@@ -499,8 +499,8 @@ int gemm_row_tiles(int form, int M) {
 }
 
 // ---------------------------------------------------------------------------
-// Tile autotuner. The first contraction of a given shape (epilogue, M rounded
-// up to a power of two, N, K, device) times every admissible tile shape
+// Tile autotuner. The first contraction of a given shape (epilogue, M and N
+// rounded up to powers of two, K, device) times every admissible tile shape
 // (the faster of two launches after a warm-up one) on the caller's stream
 // and keeps the fastest for later calls. Admissible shapes never change results: the energy partials
 // of EP_KERNEL are per column tile and the |Q|^2 partials of EP_SUMSQ per
@@ -546,7 +546,10 @@ int tuned(int family, int ep, int M, int N, int K, hipStream_t st, const TileSha
   }
   int dev = 0;
   FM_HIP(hipGetDevice(&dev));
-  auto key = std::make_tuple(family, ep, next_pow2(M), N, K, dev, 0);
+  // M and N up to powers of two: the atom count of a new batch (rows of the
+  // kernel and gradient contractions, columns of the variance one) reuses the
+  // choice for its size class instead of timing every shape again.
+  auto key = std::make_tuple(family, ep, next_pow2(M), next_pow2(N), K, dev, 0);
   {
     std::lock_guard<std::mutex> g(g_tune_mu);
     auto it = g_tune.find(key);

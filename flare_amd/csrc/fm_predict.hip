@@ -151,11 +151,24 @@ __global__ void k_scatter_rows(int n, const int *pos, const double *in, double *
   if (r < n) out[pos[r]] = in[r];
 }
 
-// Atoms of each (species, structure) of a batch, species-major.
+// Atoms of each (species, structure) of a batch, species-major. A wavefront
+// whose atoms all lie in one structure (atoms of a structure are
+// consecutive) adds one count per species from lane 0.
 __global__ void k_struct_species_count(int n, const int *species, const int *atom_struct,
-                                       int n_struct, int *cnt) {
+                                       int n_struct, int nos, int *cnt) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) atomicAdd(&cnt[(size_t)species[i] * n_struct + atom_struct[i]], 1);
+  bool ok = i < n;
+  int k = ok ? atom_struct[i] : -1, t = ok ? species[i] : -1;
+  int k0 = __shfl(k, 0, 64);
+  if (nos <= 8 && __all(!ok || k == k0)) {
+    for (int u = 0; u < nos; u++) {
+      unsigned long long m = __ballot(ok && t == u);
+      if (m != 0 && (threadIdx.x & 63) == 0)
+        atomicAdd(&cnt[(size_t)u * n_struct + k0], (int)__popcll(m));
+    }
+  } else if (ok) {
+    atomicAdd(&cnt[(size_t)t * n_struct + k], 1);
+  }
 }
 
 // Position of species-sorted row r in its structure's species-sorted order.
@@ -185,7 +198,7 @@ void build_var_pos(fm_structure s) {
   DevBuf cnt(&ctx->pool, (size_t)m * sizeof(int)), ex(&ctx->pool, (size_t)(m + 1) * sizeof(int));
   FM_HIP(hipMemsetAsync(cnt.as<int>(), 0, (size_t)m * sizeof(int), st));
   hipLaunchKernelGGL(k_struct_species_count, dim3((n + 255) / 256), dim3(256), 0, st, n,
-                     s->species.as<int>(), s->atom_struct.as<int>(), ns, cnt.as<int>());
+                     s->species.as<int>(), s->atom_struct.as<int>(), ns, s->nos, cnt.as<int>());
   scan_exclusive(ctx, cnt.as<int>(), ex.as<int>(), m, false);
   s->var_pos.alloc(&ctx->pool, (size_t)n * sizeof(int));
   hipLaunchKernelGGL(k_var_pos, dim3((n + 255) / 256), dim3(256), 0, st, n,
