@@ -208,3 +208,72 @@ def test_coefficient_file_grammar(oracle, system):
             pass
     first = body[0]
     assert first.startswith("  ") or first.startswith(" -")
+
+
+def _pair_kernel_sum(d1, d2, power):
+    """sum over same-species atom pairs i of structure 1, j of structure 2 of
+    k(B_i, B_j) = sigma^2 (B_i . B_j / |B_i||B_j|)^p, empty environments
+    skipped (normalized_dot_product.cpp:28-89 summed over both structures)."""
+    tot = 0.0
+    for s in range(NOS):
+        A, An = d1.descriptors[s], d1.descriptor_norms[s]
+        B, Bn = d2.descriptors[s], d2.descriptor_norms[s]
+        a_ok, b_ok = An >= 1e-8, Bn >= 1e-8
+        if not a_ok.any() or not b_ok.any():
+            continue
+        Ah = A[a_ok] / An[a_ok, None]
+        Bh = B[b_ok] / Bn[b_ok, None]
+        tot += np.sum(SIG ** 2 * (Ah @ Bh.T) ** power)
+    return tot
+
+
+def _mixed_second_difference(oracle, displaced, power, h):
+    """d^2 K(t1, t2) / dt1 dt2 at 0 for K(t1, t2) = sum_ij k(B_i(t1), B_j(t2)),
+    where displaced(t) returns (cell, species, positions) moved by t along
+    one coordinate: central differences at steps h and 2h combined by
+    Richardson extrapolation (error O(h^4))."""
+    def D(step):
+        ds = {t: build(oracle, *displaced(t), dervs=False)[1] for t in (step, -step)}
+        k = lambda a, b: _pair_kernel_sum(ds[a], ds[b], power)
+        return (k(step, step) - k(step, -step) - k(-step, step) + k(-step, -step)) / (4 * step ** 2)
+    return (4 * D(h) - D(2 * h)) / 3
+
+
+@pytest.mark.parametrize("power", [1.0, 2.0])
+def test_self_kernel_forces_mixed_fd(oracle, system, power):
+    """The force entries of self_kernel_struc (normalized_dot_product.cpp:
+    450-586) are the variances of the forces under the structure-structure
+    kernel: entry (a, m) = d^2 K / dx_am dx'_am with both copies displaced
+    independently (the ctests/test_kernels.cpp:140-406 recipe applied to
+    the second derivative)."""
+    cell, species, pos = system["cell"], system["species"], system["pos"]
+    m = oracle.Model(SIG, power, system["X"], system["N"], system["alpha"])
+    kself = m.self_kernel(system["d"])
+    for a, c in ((0, 0), (7, 1), (15, 2), (21, 0)):
+        def moved(t, a=a, c=c):
+            p = pos.copy()
+            p[a, c] += t
+            return cell, species, p
+        fd = _mixed_second_difference(oracle, moved, power, 1e-3)
+        assert kself[1 + 3 * a + c] == pytest.approx(fd, rel=1e-7, abs=1e-9)
+
+
+@pytest.mark.parametrize("power", [1.0, 2.0])
+def test_self_kernel_stress_mixed_fd(oracle, system, power):
+    """Stress entries: entry (m, n) = d^2 K / de de' / V^2 for the strain
+    cell(:, m) += cell(:, n) e, positions likewise (the strain of the
+    stress FD test above), applied to each copy independently."""
+    cell, species, pos = system["cell"], system["species"], system["pos"]
+    n_at = len(pos)
+    V = abs(np.linalg.det(cell))
+    m = oracle.Model(SIG, power, system["X"], system["N"], system["alpha"])
+    kself = m.self_kernel(system["d"])
+    comps = [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]
+    for q, (a, b) in enumerate(comps):
+        def strained(t, a=a, b=b):
+            c, p = cell.copy(), pos.copy()
+            c[:, a] += c[:, b] * t
+            p[:, a] += p[:, b] * t
+            return c, species, p
+        fd = _mixed_second_difference(oracle, strained, power, 1e-4) / V ** 2
+        assert kself[1 + 3 * n_at + q] == pytest.approx(fd, rel=1e-6, abs=1e-10)
