@@ -263,6 +263,10 @@ class NormalizedDotProduct(Kernel):
     def kernel_hyperparameters(self):
         return np.array([self._sigma])
 
+    def _set_hyperparameters(self, hyps):
+        """normalized_dot_product.cpp:625-629."""
+        self._sigma = float(np.asarray(hyps).reshape(-1)[0])
+
     def envs_envs(self, envs1, envs2, hyps=None):
         """normalized_dot_product.cpp:28-89 on host FP64 (a sparse-set
         sized product, used to build Kuu)."""
@@ -433,35 +437,59 @@ class Structure:
 
 
 class SparseGP:
-    """Sparse GP prediction state and predictions (bffs/sparse_gp.h).
+    """Sparse GP (bffs/sparse_gp.h, sparse_gp.cpp): the sparse set, the
+    training labels and the kernel matrices that train it, the trained state
+    (alpha, L_inv, Kuu_inverse, R_inv, Sigma) and the predictions.
 
-    The state (sparse set, alpha, L_inv) is set by ``add_*_environments`` +
-    ``update_L_inverse`` (Kuu + jitter Cholesky in FP64 on the host, as
-    sparse_gp.cpp:607-618) and ``set_alpha``, or all at once by
-    ``set_state``. Predictions run on the GPU in the Structure's form.
+    Kernel matrices between sparse environments and training structures
+    (Kuf columns, normalized_dot_product.cpp:117-214) are built on the GPU;
+    the sparse-set-sized algebra (Kuu, the Cholesky and Householder QR of
+    update_matrices_QR, likelihoods) is FP64 on the host, as the reference
+    keeps it FP64. Predictions run on the GPU in the Structure's form.
     """
 
     def __init__(self, kernels=None, energy_noise=1.0, force_noise=1.0, stress_noise=1.0):
+        # sparse_gp.cpp:14-56.
         self.kernels = list(kernels or [])
         self.n_kernels = len(self.kernels)
         self.energy_noise = float(energy_noise)
         self.force_noise = float(force_noise)
         self.stress_noise = float(stress_noise)
         self.Kuu_jitter = 1e-8  # sparse_gp.cpp:19
+        h = [np.asarray(k.kernel_hyperparameters, dtype=np.float64) for k in self.kernels]
+        self.hyperparameters = np.concatenate(
+            h + [np.array([self.energy_noise, self.force_noise, self.stress_noise])])
         self.sparse_descriptors = [ClusterDescriptor() for _ in self.kernels]
+        self.sparse_indices = [[] for _ in self.kernels]
+        self.Kuu_kernels = [np.zeros((0, 0)) for _ in self.kernels]
+        self.Kuf_kernels = [np.zeros((0, 0)) for _ in self.kernels]
+        self.Kuu = np.zeros((0, 0))
+        self.Kuf = np.zeros((0, 0))
+        self.training_structures = []
+        self.training_atom_indices = []
+        self.n_energy_labels = self.n_force_labels = self.n_stress_labels = 0
+        self.n_labels = self.n_strucs = 0
+        self.label_count = np.zeros(1, np.int64)
+        self.y = np.zeros(0)
+        self.noise_vector = np.zeros(0)
+        self.e_noise_one = np.zeros(0)
+        self.f_noise_one = np.zeros(0)
+        self.s_noise_one = np.zeros(0)
+        self.inv_e_noise_one = np.zeros(0)
+        self.inv_f_noise_one = np.zeros(0)
+        self.inv_s_noise_one = np.zeros(0)
         self.alpha = np.zeros(0)
         self.L_inv = np.zeros((0, 0))
-        self.Kuu = np.zeros((0, 0))
+        self.L_diag = np.zeros(0)
         self.Kuu_inverse = np.zeros((0, 0))
         self.Sigma = np.zeros((0, 0))
         self.R_inv = np.zeros((0, 0))
+        self.R_inv_diag = np.zeros(0)
+        self.log_marginal_likelihood = self.data_fit = 0.0
+        self.complexity_penalty = self.constant_term = 0.0
+        self.likelihood_gradient = np.zeros(0)
+        self.Kuf_e_noise_Kfu, self.Kuf_f_noise_Kfu, self.Kuf_s_noise_Kfu = [], [], []
         self._models = {}
-
-    @property
-    def hyperparameters(self):
-        h = [k.kernel_hyperparameters for k in self.kernels]
-        return np.concatenate(h + [np.array([self.energy_noise, self.force_noise,
-                                             self.stress_noise])])
 
     @property
     def n_sparse(self):
@@ -470,63 +498,277 @@ class SparseGP:
     def _invalidate(self):
         self._models = {}
 
-    # Sparse set (sparse_gp.cpp:141-341) ---------------------------------
-    def add_specific_environments(self, structure, atoms):
-        """Adds the given atoms of each descriptor (atom indices in structure
-        order) to the sparse set."""
-        atoms = [int(a) for a in atoms]
-        for k, vals in enumerate(structure.descriptors):
-            by_type = [[] for _ in range(vals.n_types)]
-            pos = {}
-            for s in range(vals.n_types):
-                for r, a in enumerate(vals.atom_indices[s]):
-                    pos[int(a)] = (s, r)
-            for a in atoms:
-                s, r = pos[a]
-                by_type[s].append(r)
-            self.sparse_descriptors[k].add_clusters_by_type(vals, by_type)
+    # Sparse set (sparse_gp.cpp:58-341) ---------------------------------
+    def _initialize_sparse_descriptors(self, structure):
+        """sparse_gp.cpp:58-71."""
+        for k, cd in enumerate(self.sparse_descriptors):
+            if cd.n_types == 0 and k < len(structure.descriptors):
+                v = structure.descriptors[k]
+                cd.__init__(v.n_types, v.n_descriptors)
+
+    def _add_clusters(self, structure, by_type):
+        """Kuu and Kuf rows of the new clusters (update_Kuu / update_Kuf /
+        stack, sparse_gp.cpp:343-472), then the clusters themselves. by_type[k]
+        lists, per species, cluster rows of structure.descriptors[k]."""
+        new = []
+        for k in range(self.n_kernels):
+            cl = ClusterDescriptor()
+            cl.add_clusters_by_type(structure.descriptors[k], by_type[k])
+            new.append(cl)
+        self._update_Kuu(new)
+        self._update_Kuf(new)
+        self._stack()
+        for k in range(self.n_kernels):
+            self.sparse_descriptors[k].add_clusters_by_type(structure.descriptors[k],
+                                                            by_type[k])
         self._invalidate()
+
+    @staticmethod
+    def _type_major(old, new):
+        """Row order of (old rows; new rows) once the new clusters join the
+        sparse set: per species, the old rows then the new ones
+        (sparse_gp.cpp:343-402)."""
+        order = []
+        M0 = old.n_clusters
+        co, cn = old.cumulative_type_count, new.cumulative_type_count
+        for s in range(new.n_types):
+            no = old.n_clusters_by_type[s] if old.n_types else 0
+            if no:
+                order.extend(range(co[s], co[s] + no))
+            order.extend(range(M0 + cn[s], M0 + cn[s] + new.n_clusters_by_type[s]))
+        return np.asarray(order, dtype=np.int64)
+
+    def _update_Kuu(self, new):
+        """sparse_gp.cpp:343-402: the previous block, its kernel with the new
+        clusters and the new clusters' own block, laid out species-major."""
+        for k, kern in enumerate(self.kernels):
+            old = self.sparse_descriptors[k]
+            self_block = kern.envs_envs(new[k], new[k])
+            if old.n_clusters:
+                prev = kern.envs_envs(old, new[k])
+                full = np.block([[self.Kuu_kernels[k], prev], [prev.T, self_block]])
+            else:
+                full = self_block
+            order = self._type_major(old, new[k])
+            self.Kuu_kernels[k] = full[np.ix_(order, order)]
+
+    def _label_columns(self, j, K):
+        """Columns of structure j's labels in an envs_struc matrix K
+        (energy, forces of the training atoms, stress; sparse_gp.cpp:423-456)."""
+        st = self.training_structures[j]
+        cols = []
+        if np.size(st.energy):
+            cols.append(K[:, :1])
+        if np.size(st.forces):
+            for a in self.training_atom_indices[j]:
+                cols.append(K[:, 1 + 3 * a:4 + 3 * a])
+        if np.size(st.stresses):
+            cols.append(K[:, 1 + 3 * st.noa:])
+        return np.concatenate(cols, axis=1) if cols else K[:, :0]
+
+    def _update_Kuf(self, new):
+        """sparse_gp.cpp:404-472: kernels between the new clusters and every
+        training structure, on the GPU."""
+        for k, kern in enumerate(self.kernels):
+            old = self.sparse_descriptors[k]
+            rows = np.zeros((new[k].n_clusters, self.n_labels))
+            if new[k].n_clusters:
+                for j, st in enumerate(self.training_structures):
+                    K = kern.envs_struc(new[k], st.descriptors[k])
+                    a, b = self.label_count[j], self.label_count[j + 1]
+                    rows[:, a:b] = self._label_columns(j, K)
+            prev = self.Kuf_kernels[k] if old.n_clusters else np.zeros((0, self.n_labels))
+            full = np.concatenate([prev, rows], axis=0)
+            self.Kuf_kernels[k] = full[self._type_major(old, new[k])]
+
+    def _stack(self):
+        """stack_Kuu / stack_Kuf (sparse_gp.cpp:585-605)."""
+        M = sum(b.shape[0] for b in self.Kuu_kernels)
+        self.Kuu = np.zeros((M, M))
+        self.Kuf = np.zeros((M, self.n_labels))
+        o = 0
+        for ku, kf in zip(self.Kuu_kernels, self.Kuf_kernels):
+            m = ku.shape[0]
+            self.Kuu[o:o + m, o:o + m] = ku
+            if m:
+                self.Kuf[o:o + m, :] = kf
+            o += m
+
+    def _atom_indices_of_clusters(self, vals, clusters):
+        ai = np.concatenate([np.asarray(a) for a in vals.atom_indices]) if vals.n_clusters else []
+        return [int(ai[c]) for c in clusters]
+
+    @staticmethod
+    def _clusters_by_type(vals, clusters):
+        """ClusterDescriptor::add_clusters (descriptor.cpp:77-100): cluster
+        indices (species-sorted order) to per-species rows, order kept."""
+        cum = vals.cumulative_type_count
+        out = [[] for _ in range(vals.n_types)]
+        for c in clusters:
+            for s in range(vals.n_types):
+                if cum[s] <= c < cum[s + 1]:
+                    out[s].append(int(c) - cum[s])
+                    break
+        return out
+
+    def add_specific_environments(self, structure, atoms):
+        """sparse_gp.cpp:140-186: per species, the clusters whose atom is in
+        ``atoms``, in the structure's species-sorted order (an atom listed
+        twice is added twice)."""
+        self._initialize_sparse_descriptors(structure)
+        atoms = [int(a) for a in atoms]
+        by_kernel = []
+        for k in range(self.n_kernels):
+            self.sparse_indices[k].append(list(atoms))
+            vals = structure.descriptors[k]
+            by_type = []
+            for s in range(vals.n_types):
+                rows = []
+                for r, a in enumerate(np.asarray(vals.atom_indices[s])):
+                    rows.extend([r] * atoms.count(int(a)))
+                by_type.append(rows)
+            by_kernel.append(by_type)
+        self._add_clusters(structure, by_kernel)
 
     def add_all_environments(self, structure):
-        for k, vals in enumerate(structure.descriptors):
-            self.sparse_descriptors[k].add_all_clusters(vals)
-        self._invalidate()
+        """sparse_gp.cpp:309-341."""
+        self._initialize_sparse_descriptors(structure)
+        by_kernel = [[list(range(n)) for n in structure.descriptors[k].n_clusters_by_type]
+                     for k in range(self.n_kernels)]
+        self._add_clusters(structure, by_kernel)
+        for k in range(self.n_kernels):
+            self.sparse_indices[k].append(list(range(structure.noa)))
 
     def add_random_environments(self, structure, n_added, seed=None):
+        """sparse_gp.cpp:259-307. The reference draws with std::random_shuffle
+        (unseeded); here numpy's generator with an optional seed draws the
+        same kind of sample: n_added[k] clusters without replacement."""
+        self._initialize_sparse_descriptors(structure)
         rng = np.random.default_rng(seed)
-        for k, vals in enumerate(structure.descriptors):
-            by_type = [[] for _ in range(vals.n_types)]
-            n = structure.noa
-            pick = rng.choice(n, size=min(int(n_added[k]), n), replace=False)
-            idx = {}
-            for s in range(vals.n_types):
-                for r, a in enumerate(vals.atom_indices[s]):
-                    idx[int(a)] = (s, r)
-            for a in sorted(int(x) for x in pick):
-                s, r = idx[a]
-                by_type[s].append(r)
-            self.sparse_descriptors[k].add_clusters_by_type(vals, by_type)
+        by_kernel, picks = [], []
+        for k in range(self.n_kernels):
+            vals = structure.descriptors[k]
+            n = min(int(n_added[k]), vals.n_clusters)
+            pick = [int(c) for c in rng.permutation(vals.n_clusters)[:n]]
+            picks.append(pick)
+            by_kernel.append(self._clusters_by_type(vals, pick))
+        self._add_clusters(structure, by_kernel)
+        for k in range(self.n_kernels):
+            self.sparse_indices[k].append(
+                self._atom_indices_of_clusters(structure.descriptors[k], picks[k]))
+
+    def add_uncertain_environments(self, structure, n_added):
+        """sparse_gp.cpp:188-257: the n_added[k] clusters of largest local
+        variance (stable sort, decreasing)."""
+        if len(n_added) != self.n_kernels:
+            raise RuntimeError("n_added must have the same size as the number of kernels")
+        self._initialize_sparse_descriptors(structure)
+        var = self.compute_cluster_uncertainties(structure)
+        by_kernel, picks = [], []
+        for k in range(self.n_kernels):
+            order = sorted(range(len(var[k])), key=lambda i: -var[k][i])
+            pick = order[:min(int(n_added[k]), len(order))]
+            picks.append(pick)
+            by_kernel.append(self._clusters_by_type(structure.descriptors[k], pick))
+        self._add_clusters(structure, by_kernel)
+        for k in range(self.n_kernels):
+            self.sparse_indices[k].append(
+                self._atom_indices_of_clusters(structure.descriptors[k], picks[k]))
+
+    # Training labels (sparse_gp.cpp:474-583) ------------------------------
+    def add_training_structure(self, structure, atom_indices=(-1,), rel_e_noise=1.0,
+                               rel_f_noise=1.0, rel_s_noise=1.0):
+        """Adds the structure's energy, forces (of ``atom_indices``, all atoms
+        for [-1]) and stress labels; their Kuf columns come from envs_struc
+        on the GPU (FP64 descriptors)."""
+        self._initialize_sparse_descriptors(structure)
+        if structure.form != FORM_FP64:
+            raise ValueError("training structures need FP64 descriptors")
+        n_atoms = structure.noa
+        energy = np.asarray(structure.energy, dtype=np.float64).reshape(-1)
+        forces = np.asarray(structure.forces, dtype=np.float64).reshape(-1)
+        stresses = np.asarray(structure.stresses, dtype=np.float64).reshape(-1)
+        atom_indices = [int(a) for a in atom_indices]
+        if forces.size == 0:
+            atoms = []
+        elif atom_indices[0] == -1:
+            atoms = list(range(n_atoms))
+        else:
+            atoms = atom_indices
+        if forces.size and forces.size != 3 * n_atoms:
+            raise ValueError("forces must hold 3 values per atom")
+        self.training_atom_indices.append(atoms)
+        n_e, n_f, n_s = energy.size, 3 * len(atoms), stresses.size
+        n_new = n_e + n_f + n_s
+        y = np.concatenate([energy] + [forces[3 * a:3 * a + 3] for a in atoms] + [stresses])
+        e1 = np.r_[np.full(n_e, 1 / rel_e_noise ** 2), np.zeros(n_f + n_s)]
+        f1 = np.r_[np.zeros(n_e), np.full(n_f, 1 / rel_f_noise ** 2), np.zeros(n_s)]
+        s1 = np.r_[np.zeros(n_e + n_f), np.full(n_s, 1 / rel_s_noise ** 2)]
+        ie = np.r_[np.full(n_e, rel_e_noise ** 2), np.zeros(n_f + n_s)]
+        iff = np.r_[np.zeros(n_e), np.full(n_f, rel_f_noise ** 2), np.zeros(n_s)]
+        is_ = np.r_[np.zeros(n_e + n_f), np.full(n_s, rel_s_noise ** 2)]
+        noise = (e1 / self.energy_noise ** 2 + f1 / self.force_noise ** 2
+                 + s1 / self.stress_noise ** 2)
+        self.y = np.r_[self.y, y]
+        self.noise_vector = np.r_[self.noise_vector, noise]
+        self.e_noise_one = np.r_[self.e_noise_one, e1]
+        self.f_noise_one = np.r_[self.f_noise_one, f1]
+        self.s_noise_one = np.r_[self.s_noise_one, s1]
+        self.inv_e_noise_one = np.r_[self.inv_e_noise_one, ie]
+        self.inv_f_noise_one = np.r_[self.inv_f_noise_one, iff]
+        self.inv_s_noise_one = np.r_[self.inv_s_noise_one, is_]
+        self.label_count = np.r_[self.label_count, self.n_labels + n_new]
+        self.training_structures.append(structure)
+        self.n_strucs += 1
+        j = self.n_strucs - 1
+        for k, kern in enumerate(self.kernels):
+            cd = self.sparse_descriptors[k]
+            cols = np.zeros((cd.n_clusters, n_new))
+            if cd.n_clusters:
+                cols = self._label_columns(j, kern.envs_struc(cd, structure.descriptors[k]))
+            prev = self.Kuf_kernels[k] if self.Kuf_kernels[k].size else np.zeros(
+                (cd.n_clusters, self.n_labels))
+            self.Kuf_kernels[k] = np.concatenate([prev, cols], axis=1)
+        self.n_energy_labels += n_e
+        self.n_force_labels += n_f
+        self.n_stress_labels += n_s
+        self.n_labels += n_new
+        self._stack()
         self._invalidate()
 
-    # State -----------------------------------------------------------------
-    def update_Kuu(self):
-        blocks = [k.envs_envs(c, c) for k, c in zip(self.kernels, self.sparse_descriptors)]
-        M = sum(b.shape[0] for b in blocks)
-        self.Kuu = np.zeros((M, M))
-        o = 0
-        for b in blocks:
-            self.Kuu[o:o + b.shape[0], o:o + b.shape[0]] = b
-            o += b.shape[0]
-        return self.Kuu
+    # Trained state (sparse_gp.cpp:607-641) --------------------------------
+    def update_matrices_QR(self):
+        """L = chol(Kuu + jitter I), L_inv, Kuu^-1 = L_inv^T L_inv; the
+        Householder QR of A = [Lambda^1/2 Kuf^T; L^T]; R_inv = R^-1,
+        alpha = R_inv (Q^T b)[:M] with b = [Lambda^1/2 y; 0], Sigma =
+        R_inv R_inv^T. FP64 on the host (LAPACK geqrf: the Householder
+        convention of Eigen's HouseholderQR)."""
+        from scipy.linalg import solve_triangular
+        M = self.Kuu.shape[0]
+        eye = np.eye(M)
+        L = np.linalg.cholesky(self.Kuu + self.Kuu_jitter * eye)
+        self.L_inv = solve_triangular(L, eye, lower=True)
+        self.L_diag = np.diag(self.L_inv).copy()
+        self.Kuu_inverse = self.L_inv.T @ self.L_inv
+        ns = np.sqrt(self.noise_vector)
+        A = np.vstack([ns[:, None] * self.Kuf.T, L.T])
+        b = np.r_[ns * self.y, np.zeros(M)]
+        Q, R = np.linalg.qr(A, mode="reduced")
+        self.R_inv = solve_triangular(R, eye, lower=False)
+        self.R_inv_diag = np.diag(self.R_inv).copy()
+        self.alpha = self.R_inv @ (Q.T @ b)
+        self.Sigma = self.R_inv @ self.R_inv.T
+        self._invalidate()
 
     def update_L_inverse(self):
         """L_inv = chol(Kuu + jitter I)^-1 and Kuu^-1 = L_inv^T L_inv
-        (sparse_gp.cpp:610-618), FP64 on the host."""
-        self.update_Kuu()
+        (sparse_gp.cpp:610-618) alone, for a model whose alpha is set
+        directly (no training labels)."""
+        from scipy.linalg import solve_triangular
         M = self.Kuu.shape[0]
         L = np.linalg.cholesky(self.Kuu + self.Kuu_jitter * np.eye(M))
-        from scipy.linalg import solve_triangular
         self.L_inv = solve_triangular(L, np.eye(M), lower=True)
+        self.L_diag = np.diag(self.L_inv).copy()
         self.Kuu_inverse = self.L_inv.T @ self.L_inv
         self._invalidate()
 
@@ -538,12 +780,13 @@ class SparseGP:
         self._invalidate()
 
     def set_state(self, alpha, L_inv=None, Sigma=None, Kuu_inverse=None, R_inv=None):
-        """Sets the trained state. R_inv (the inverse of the QR factor R,
-        sparse_gp.cpp:635-640) defines Sigma = R_inv R_inv^T; a Sigma given
-        alone is factored as Sigma = R_inv R_inv^T by Cholesky."""
+        """Sets a trained state directly (alpha, L_inv, and R_inv or Sigma;
+        Sigma = R_inv R_inv^T, a Sigma given alone is factored by Cholesky).
+        update_matrices_QR produces the same state from training labels."""
         self.set_alpha(alpha)
         if L_inv is not None:
             self.L_inv = np.array(L_inv, dtype=np.float64)
+            self.L_diag = np.diag(self.L_inv).copy()
             self.Kuu_inverse = (self.L_inv.T @ self.L_inv if Kuu_inverse is None
                                 else np.array(Kuu_inverse, dtype=np.float64))
         if R_inv is not None:
@@ -555,10 +798,203 @@ class SparseGP:
             self.R_inv = np.linalg.cholesky(self.Sigma)
         self._invalidate()
 
+    def set_hyperparameters(self, hyps):
+        """sparse_gp.cpp:1166-1198: kernel hyperparameters rescale Kuu and Kuf
+        (both are sigma^2 times a sigma-free matrix for these kernels), the
+        noises reset the noise vector, then update_matrices_QR."""
+        hyps = np.asarray(hyps, dtype=np.float64).reshape(-1)
+        h = 0
+        for k, kern in enumerate(self.kernels):
+            nh = len(kern.kernel_hyperparameters)
+            new = hyps[h:h + nh]
+            scale = new[0] ** 2 / kern.sigma ** 2
+            self.Kuu_kernels[k] = self.Kuu_kernels[k] * scale
+            self.Kuf_kernels[k] = self.Kuf_kernels[k] * scale
+            kern._set_hyperparameters(new)
+            h += nh
+        self._stack()
+        self.hyperparameters = hyps.copy()
+        self.energy_noise, self.force_noise, self.stress_noise = (float(x) for x in hyps[h:h + 3])
+        self.noise_vector = (self.e_noise_one / self.energy_noise ** 2
+                             + self.f_noise_one / self.force_noise ** 2
+                             + self.s_noise_one / self.stress_noise ** 2)
+        self.update_matrices_QR()
+
+    # Likelihoods (sparse_gp.cpp:759-1164) ----------------------------------
+    def compute_likelihood_stable(self):
+        """sparse_gp.cpp:759-787 (the reference also prints two lines of
+        intermediate terms; this does not)."""
+        r = self.y - self.Kuf.T @ self.alpha
+        self.data_fit = -0.5 * float(self.y @ (self.noise_vector * r))
+        self.constant_term = -0.5 * self.n_labels * np.log(2 * np.pi)
+        noise_det = float(np.sum(np.log(self.noise_vector)))
+        kuu_inv_det = -2 * float(np.sum(np.log(np.abs(self.L_diag))))
+        sigma_inv_det = 2 * float(np.sum(np.log(np.abs(self.R_inv_diag))))
+        self.complexity_penalty = 0.5 * (noise_det + kuu_inv_det + sigma_inv_det)
+        self.log_marginal_likelihood = self.complexity_penalty + self.data_fit + self.constant_term
+
+    def precompute_KnK(self):
+        """sparse_gp.cpp:874-897: Kuf_i N_x Kuf_j^T / (sigma_i sigma_j)^2 per
+        noise kind x."""
+        self.Kuf_e_noise_Kfu, self.Kuf_f_noise_Kfu, self.Kuf_s_noise_Kfu = [], [], []
+        for ki, kf_i in zip(self.kernels, self.Kuf_kernels):
+            for kj, kf_j in zip(self.kernels, self.Kuf_kernels):
+                s4 = ki.sigma ** 2 * kj.sigma ** 2
+                self.Kuf_e_noise_Kfu.append((kf_i * self.e_noise_one) @ kf_j.T / s4)
+                self.Kuf_f_noise_Kfu.append((kf_i * self.f_noise_one) @ kf_j.T / s4)
+                self.Kuf_s_noise_Kfu.append((kf_i * self.s_noise_one) @ kf_j.T / s4)
+
+    def _sizes(self):
+        return [b.shape[0] for b in self.Kuu_kernels]
+
+    def _KnK(self, precomputed):
+        """compute_KnK (sparse_gp.cpp:899-935)."""
+        if not precomputed:
+            return tuple((self.Kuf * one) @ self.Kuf.T
+                         for one in (self.e_noise_one, self.f_noise_one, self.s_noise_one))
+        M = self.n_sparse
+        out = [np.zeros((M, M)) for _ in range(3)]
+        sz = self._sizes()
+        off = np.r_[0, np.cumsum(sz)]
+        c = 0
+        for i, ki in enumerate(self.kernels):
+            for j, kj in enumerate(self.kernels):
+                s4 = ki.sigma ** 2 * kj.sigma ** 2
+                for o, pre in zip(out, (self.Kuf_e_noise_Kfu, self.Kuf_f_noise_Kfu,
+                                        self.Kuf_s_noise_Kfu)):
+                    o[off[i]:off[i + 1], off[j]:off[j + 1]] += pre[c] * s4
+                c += 1
+        return tuple(out)
+
+    def _dKnK(self, i):
+        """compute_dKnK (sparse_gp.cpp:937-1012)."""
+        M = self.n_sparse
+        out = np.zeros((M, M))
+        off = np.r_[0, np.cumsum(self._sizes())]
+        si = self.kernels[i].sigma
+        for j, kj in enumerate(self.kernels):
+            c = i * self.n_kernels + j
+            s3 = 2 * si * kj.sigma ** 2
+            blk = (self.Kuf_e_noise_Kfu[c] * s3 / self.energy_noise ** 2
+                   + self.Kuf_f_noise_Kfu[c] * s3 / self.force_noise ** 2
+                   + self.Kuf_s_noise_Kfu[c] * s3 / self.stress_noise ** 2)
+            out[off[i]:off[i + 1], off[j]:off[j + 1]] += blk
+        return out
+
+    def compute_likelihood_gradient_stable(self, precomputed_KnK=False):
+        """sparse_gp.cpp:789-872: the likelihood of the current state and
+        its gradient with respect to every hyperparameter."""
+        r = self.y - self.Kuf.T @ self.alpha
+        self.compute_likelihood_stable()
+        M = self.n_sparse
+        nh = len(self.hyperparameters)
+        grad = np.zeros(nh)
+        off = np.r_[0, np.cumsum(self._sizes())]
+        h = 0
+        for i, kern in enumerate(self.kernels):
+            n_hyps = len(kern.kernel_hyperparameters)
+            cur = self.hyperparameters[h:h + n_hyps]
+            ku = self.Kuu_kernels[i]
+            ku0 = ku * (cur[0] ** 2 / kern.sigma ** 2)
+            ku1 = ku * (2 * cur[0] / kern.sigma ** 2)
+            dKuu = np.zeros((M, M))
+            dKuu[off[i]:off[i + 1], off[i]:off[i + 1]] = ku1
+            if precomputed_KnK:
+                dKnK = self._dKnK(i)
+            else:
+                dKuf = np.zeros((M, self.n_labels))
+                dKuf[off[i]:off[i + 1]] = self.Kuf_kernels[i] * (2 * cur[0] / kern.sigma ** 2)
+                dKnK = (dKuf * self.noise_vector) @ self.Kuf.T
+            Pi = dKnK + dKnK.T + dKuu
+            comp = 0.5 * np.trace(np.linalg.inv(ku0) @ ku1) - 0.5 * np.trace(Pi @ self.Sigma)
+            if precomputed_KnK:
+                dKuf = np.zeros((M, self.n_labels))
+                dKuf[off[i]:off[i + 1]] = self.Kuf[off[i]:off[i + 1]]
+                dK_alpha = (2.0 / cur[0]) * dKuf.T @ self.alpha
+            else:
+                dK_alpha = dKuf.T @ self.alpha
+            fit = float(dK_alpha @ (self.noise_vector * r)) - 0.5 * float(
+                self.alpha @ dKuu @ self.alpha)
+            grad[h] = comp + fit
+            h += n_hyps
+        KnK_e, KnK_f, KnK_s = self._KnK(precomputed_KnK)
+        for q, (n_lab, sig, KnK, one) in enumerate((
+                (self.n_energy_labels, self.energy_noise, KnK_e, self.e_noise_one),
+                (self.n_force_labels, self.force_noise, KnK_f, self.f_noise_one),
+                (self.n_stress_labels, self.stress_noise, KnK_s, self.s_noise_one))):
+            comp = -n_lab / sig + np.trace(KnK @ self.Sigma) / sig ** 3
+            fit = float(r @ (one * r)) / sig ** 3
+            grad[h + q] = comp + fit
+        self.likelihood_gradient = grad
+        return self.log_marginal_likelihood
+
+    def compute_likelihood(self):
+        """sparse_gp.cpp:1014-1045: the direct O(n_labels^3) likelihood
+        through a column-pivoted QR of Qff + Lambda^-1."""
+        if self.n_labels == 0:
+            return
+        from scipy.linalg import qr, solve_triangular
+        Q = self.Kuf.T @ self.Kuu_inverse @ self.Kuf + np.diag(1 / self.noise_vector)
+        q, R, piv = qr(Q, pivoting=True)
+        z = solve_triangular(R, q.T @ self.y)
+        x = np.empty_like(z)
+        x[piv] = z
+        self.complexity_penalty = -0.5 * float(np.sum(np.log(np.abs(np.diag(R)))))
+        self.data_fit = -0.5 * float(self.y @ x)
+        self.constant_term = -0.5 * self.n_labels * np.log(2 * np.pi)
+        self.log_marginal_likelihood = self.complexity_penalty + self.data_fit + self.constant_term
+
+    def compute_likelihood_gradient(self, hyperparameters):
+        """sparse_gp.cpp:1047-1164: likelihood and gradient at the given
+        hyperparameters through the dense n_labels x n_labels Qff."""
+        from scipy.linalg import lu_factor, lu_solve
+        hyps = np.asarray(hyperparameters, dtype=np.float64)
+        M, n = self.n_sparse, self.n_labels
+        Kuu = np.zeros((M, M))
+        Kuf = np.zeros((M, n))
+        dKuu, dKuf = [], []
+        off = np.r_[0, np.cumsum(self._sizes())]
+        h = 0
+        for i, kern in enumerate(self.kernels):
+            nh = len(kern.kernel_hyperparameters)
+            c = hyps[h:h + nh]
+            sl = slice(off[i], off[i + 1])
+            Kuu[sl, sl] = self.Kuu_kernels[i] * (c[0] ** 2 / kern.sigma ** 2)
+            Kuf[sl] = self.Kuf_kernels[i] * (c[0] ** 2 / kern.sigma ** 2)
+            du, df = np.zeros((M, M)), np.zeros((M, n))
+            du[sl, sl] = self.Kuu_kernels[i] * (2 * c[0] / kern.sigma ** 2)
+            df[sl] = self.Kuf_kernels[i] * (2 * c[0] / kern.sigma ** 2)
+            dKuu.append(du)
+            dKuf.append(df)
+            h += nh
+        Kinv = np.linalg.inv(Kuu + self.Kuu_jitter * np.eye(M))
+        se, sf, ss = hyps[h:h + 3]
+        noise = se ** 2 * self.inv_e_noise_one + sf ** 2 * self.inv_f_noise_one \
+            + ss ** 2 * self.inv_s_noise_one
+        Qff = Kuf.T @ Kinv @ Kuf + np.diag(noise)
+        grads = [dF.T @ Kinv @ Kuf - Kuf.T @ Kinv @ dU @ Kinv @ Kuf + Kuf.T @ Kinv @ dF
+                 for dU, dF in zip(dKuu, dKuf)]
+        grads += [np.diag(2 * se * self.inv_e_noise_one), np.diag(2 * sf * self.inv_f_noise_one),
+                  np.diag(2 * ss * self.inv_s_noise_one)]
+        lu = lu_factor(Qff)
+        Qinv = lu_solve(lu, np.eye(n))
+        self.complexity_penalty = -0.5 * float(np.sum(np.log(np.abs(np.diag(lu[0])))))
+        Qy = Qinv @ self.y
+        self.data_fit = -0.5 * float(self.y @ Qy)
+        self.constant_term = -n * np.log(2 * np.pi) / 2
+        self.log_marginal_likelihood = self.complexity_penalty + self.data_fit + self.constant_term
+        g = np.zeros(len(hyps))
+        for q, dQ in enumerate(grads):
+            QdQ = Qinv @ dQ
+            g[q] = (-np.trace(QdQ) + float(self.y @ QdQ @ Qy)) / 2
+        self.likelihood_gradient = g
+        return self.log_marginal_likelihood
+
+    # Device models ---------------------------------------------------------
     def _model(self, k, ctx):
-        key = (k, id(ctx))
+        key = (k, ctx.h.value)
         m = self._models.get(key)
-        if m is None:
+        if m is None or m.ctx is not ctx:
             m = _DeviceModel(self, k, ctx)
             self._models[key] = m
         return m
@@ -601,18 +1037,47 @@ class SparseGP:
         structure.mean_efs = mean
         structure.local_uncertainties = local
 
+    def compute_cluster_uncertainties(self, structure):
+        """sparse_gp.cpp:93-139: per kernel, the local variances of the
+        structure's clusters in species-sorted order."""
+        if self.L_inv.shape[0] != self.Kuu.shape[0] or self.L_inv.shape[0] != self.n_sparse:
+            raise RuntimeError("L_inv must be up to date to evaluate uncertainties. Please "
+                               "call update_matrices_QR and try again.")
+        _, local = self._run(structure, _lib.PREDICT_MEAN | _lib.PREDICT_LOCAL)
+        return local
+
     def _dense(self, structure, kind):
+        """DTC / SOR: mean from every kernel; variances K_self - diag(K^T
+        Kuu^-1 K) + diag(K^T Sigma K) (DTC) or diag(K^T R_inv R_inv^T K)
+        (SOR) with K stacked over kernels (sparse_gp.cpp:667-729). A single
+        kernel runs whole on the GPU; with several, each kernel's matrix K_k
+        and self kernel come from the GPU and the stacked products are FP64
+        on the host."""
         self._check_state()
-        if self.n_kernels != 1:
-            raise ValueError("DTC and SOR predictions take a model with one kernel")
         if self.R_inv.shape != (self.n_sparse, self.n_sparse):
             raise RuntimeError("R_inv (or Sigma) must be set for DTC and SOR predictions")
         n = structure.noa
-        mean = np.zeros(1 + 3 * n + 6)
-        var = np.zeros(1 + 3 * n + 6)
-        m = self._model(0, structure._ctx)
-        _lib.check(_lib.load().fm_predict_dense(m.h, structure._h, kind, _lib.ptr(mean),
-                                                _lib.ptr(var)))
+        if self.n_kernels == 1:
+            mean = np.zeros(1 + 3 * n + 6)
+            var = np.zeros(1 + 3 * n + 6)
+            m = self._model(0, structure._ctx)
+            _lib.check(_lib.load().fm_predict_dense(m.h, structure.descriptors[0]._handle,
+                                                    kind, _lib.ptr(mean), _lib.ptr(var)))
+        else:
+            Ks, kself = [], np.zeros(1 + 3 * n + 6)
+            for k, kern in enumerate(self.kernels):
+                vals = structure.descriptors[k]
+                Ks.append(kern.envs_struc(self.sparse_descriptors[k], vals))
+                kself += kern.self_kernel_struc(vals)
+            K = np.concatenate(Ks, axis=0)
+            mean = K.T @ self.alpha
+            if kind == _lib.DENSE_SOR:
+                V = K.T @ self.R_inv
+                var = np.sum(V * V, axis=1)
+            else:
+                Q1 = self.L_inv @ K
+                Q2 = self.R_inv.T @ K
+                var = kself - np.sum(Q1 * Q1, axis=0) + np.sum(Q2 * Q2, axis=0)
         structure.mean_efs = mean
         structure.variance_efs = var
 
@@ -624,6 +1089,40 @@ class SparseGP:
         """sparse_gp.cpp:694-729: variance K_self - diag(K^T Kuu^-1 K) +
         diag(K^T Sigma K)."""
         self._dense(structure, _lib.DENSE_DTC)
+
+    # Files (sparse_gp.cpp:1200-1466) ---------------------------------------
+    def _calculator(self, kernel_index, calc):
+        """The descriptor calculator the files describe: the first training
+        structure's (sparse_gp.cpp:1225-1227), or the one given."""
+        if calc is not None:
+            return calc
+        if self.training_structures:
+            return self.training_structures[0].descriptor_calculators[kernel_index]
+        raise ValueError("no training structure holds the descriptor calculator; pass calc=")
+
+    def write_mapping_coefficients(self, file_name, contributor, kernel_index, calc=None):
+        """sparse_gp.cpp:1200-1264 (pair_style flare coefficient file)."""
+        from . import mapping
+        mapping.write_mapping_coefficients(self, file_name, contributor, kernel_index,
+                                           self._calculator(kernel_index, calc))
+
+    def write_varmap_coefficients(self, file_name, contributor, kernel_index, calc=None):
+        """sparse_gp.cpp:1266-1339 (power-1 variance map)."""
+        from . import mapping
+        mapping.write_varmap_coefficients(self, file_name, contributor, kernel_index,
+                                          self._calculator(kernel_index, calc))
+
+    def write_L_inverse(self, file_name, contributor, calcs=None):
+        """sparse_gp.cpp:1341-1402."""
+        from . import mapping
+        if calcs is None:
+            calcs = [self._calculator(k, None) for k in range(self.n_kernels)]
+        mapping.write_L_inverse(self, file_name, contributor, calcs)
+
+    def write_sparse_descriptors(self, file_name, contributor):
+        """sparse_gp.cpp:1404-1466."""
+        from . import mapping
+        mapping.write_sparse_descriptors(self, file_name, contributor)
 
 
 class _DeviceModel:
@@ -662,6 +1161,7 @@ class _DeviceModel:
         h = _lib.C.c_void_p()
         _lib.check(_lib.load().fm_model_create(ctx.h, _lib.C.byref(d), _lib.C.byref(h)))
         self.h = h
+        self.ctx = ctx  # the context outlives the models made on it
 
     @classmethod
     def from_sparse_set(cls, ctx, sigma, power, cd):
@@ -671,6 +1171,7 @@ class _DeviceModel:
 
     def __del__(self):
         try:
-            _lib.load().fm_model_destroy(self.h)
+            if getattr(self, "h", None) is not None and self.ctx.h is not None:
+                _lib.load().fm_model_destroy(self.h)
         except Exception:
             pass

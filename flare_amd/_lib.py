@@ -159,6 +159,24 @@ class Context:
     def synchronize(self):
         check(self.L.fm_context_synchronize(self.h))
 
+    def close(self):
+        """Destroys the context (its stream and buffer pool). Models and
+        structures made on it must be released first."""
+        if self.h is not None:
+            check(self.L.fm_context_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        # Objects made on a context hold a reference to it, so it is unused
+        # here; at interpreter exit the order of finalisation is arbitrary and
+        # the process releases the device anyway.
+        try:
+            import sys
+            if not sys.is_finalizing():
+                self.close()
+        except Exception:
+            pass
+
     def set_row_tile(self, rows):
         """Row tiles for B3 descriptors computed on this context afterwards
         (0: tile only structures whose descriptors would exceed 32 GB)."""

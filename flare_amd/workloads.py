@@ -5,6 +5,8 @@ tests (there are no datasets in this environment).
                   displacements sigma = 0.05 A, rng seed 1000 + frame.
   fcc_alloy_32k   32,000-atom FCC (a = 4.0 A, 20^3 cells), two species 50/50
                   by random permutation (seed 2), displacements 0.08 A.
+  c1_training_frames  20 of the 108-atom frames labelled by Lennard-Jones
+                  (eps 0.4 eV, sigma 2.6 A, cutoff 7.8 A, shifted to 0).
 """
 import numpy as np
 
@@ -49,14 +51,18 @@ def random_alloy(n_cells, a=4.0, nos=2, sigma=0.08, seed=0, basis=FCC_BASIS):
 
 
 def lennard_jones(cell, positions, eps=0.4, sig=2.6, rc=7.8):
-    """Shifted LJ energy and forces (the config-1 labels), brute force over
-    images; used to label training frames."""
+    """Shifted Lennard-Jones energy, forces and stress (the labels of the
+    108-atom training frames), brute force over the 27 nearest images of
+    the minimum-image vector (cells wider than rc). The stress is in FLARE's
+    order and sign (xx, xy, xz, yy, yz, zz) = -(1/V) dE/d(strain), i.e.
+    minus ASE's stress (otf.py:633-647 converts between them)."""
     pos = np.asarray(positions)
     n = len(pos)
     inv = np.linalg.inv(cell)
     shift = 4 * eps * ((sig / rc) ** 12 - (sig / rc) ** 6)
     E = 0.0
     F = np.zeros_like(pos)
+    W = np.zeros((3, 3))
     reps = [np.array([i, j, k]) for i in (-1, 0, 1) for j in (-1, 0, 1) for k in (-1, 0, 1)]
     for i in range(n):
         d = pos - pos[i]
@@ -69,7 +75,21 @@ def lennard_jones(cell, positions, eps=0.4, sig=2.6, rc=7.8):
             E += 0.5 * np.sum(4 * eps * (x ** 12 - x ** 6) - shift)
             fmag = 24 * eps * (2 * x ** 12 - x ** 6) / rr[m] ** 2
             F[i] -= np.sum(fmag[:, None] * dd[m], axis=0)
-    return E, F
+            W += 0.5 * (fmag[:, None] * dd[m]).T @ dd[m]
+    V = abs(np.linalg.det(cell))
+    S = W[[0, 0, 0, 1, 1, 2], [0, 1, 2, 1, 2, 2]] / V
+    return E, F, S
+
+
+def c1_training_frames(n_frames=20):
+    """The 108-atom FCC training frames (seeds 1000 + k) with their LJ
+    energy, forces and stress: [(cell, species, positions, E, F, S)]."""
+    out = []
+    for k in range(n_frames):
+        cell, species, pos = fcc_al_108(frame=k)
+        E, F, S = lennard_jones(cell, pos)
+        out.append((cell, species, pos, E, F, S))
+    return out
 
 
 def batch_structures(n_structures=512, seed=5, n_min=16, n_max=2048, nos=3):

@@ -1,0 +1,249 @@
+"""The sparse-GP training state built on the GPU path: Kuf columns from
+envs_struc on the GPU, Kuu, and the QR producer of alpha / Sigma / R_inv
+(sparse_gp.cpp:343-641), for the 108-atom FCC model trained on 20
+Lennard-Jones frames with 10 sparse atoms each (200 sparse environments;
+B2 [1, 8, 3], rc 5.0; NormalizedDotProduct sigma 2, power 2; noise 0.01 /
+0.05 / 0.005). Checked against the oracle's kernel matrices and
+predictions at the FP64 bar (1e-10), and through the normal-equation closed
+form of alpha. Also: the reference's AddOrder and SortTest recipes
+(ctests/test_sparse_gp.cpp) with the GPU kernel matrices, the SparseGP
+file writers with the calculator of the first training structure, and DTC
+/ SOR with a trained R_inv and with two kernels."""
+import os
+
+import numpy as np
+import pytest
+
+from helpers import rel_err, var_err
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-10
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def C():
+    from flare_amd import _C_flare
+    return _C_flare
+
+
+def c1_calc(C):
+    return C.B2("chebyshev", "quadratic", [0.0, 5.0], [], [1, 8, 3])
+
+
+@pytest.fixture(scope="module")
+def c1(C):
+    """20 labelled frames, 10 sparse atoms each (seed 7)."""
+    from flare_amd.workloads import c1_training_frames
+    say("building the 108-atom LJ-trained model")
+    calc = c1_calc(C)
+    sgp = C.SparseGP([C.NormalizedDotProduct(2.0, 2.0)], 0.01, 0.05, 0.005)
+    rng = np.random.default_rng(7)
+    frames, picks = c1_training_frames(20), []
+    for cell, species, pos, E, F, S in frames:
+        s = C.Structure(cell, species, pos, 5.0, [calc])
+        s.energy = np.array([[E]])
+        s.forces = F.reshape(-1)
+        s.stresses = S
+        sgp.add_training_structure(s)
+        p = sorted(int(a) for a in rng.choice(len(pos), 10, replace=False))
+        picks.append(p)
+        sgp.add_specific_environments(s, p)
+    sgp.update_matrices_QR()
+    return dict(sgp=sgp, calc=calc, frames=frames, picks=picks)
+
+
+@pytest.fixture(scope="module")
+def c1_oracle(oracle, c1):
+    """The oracle's sparse set and kernel matrices for the same frames."""
+    X, N, descs = [], [], []
+    for (cell, species, pos, *_), p in zip(c1["frames"], c1["picks"]):
+        d = oracle.B2Values(oracle.Neighbors(cell, species, pos, 5.0, "literal"), 1, 8, 3,
+                            np.full((1, 1), 5.0))
+        descs.append(d)
+        order = np.argsort(d.atom_indices[0])  # row of each atom (one species: identity)
+        X.append(d.descriptors[0][order[p]])
+        N.append(d.descriptor_norms[0][order[p]])
+    X, N = np.concatenate(X), np.concatenate(N)
+    return dict(X=X, N=N, descs=descs)
+
+
+def test_sparse_set_and_kuu(C, oracle, c1, c1_oracle):
+    sgp = c1["sgp"]
+    assert sgp.n_sparse == 200 and sgp.n_labels == 20 * (1 + 3 * 108 + 6)
+    cd = sgp.sparse_descriptors[0]
+    assert rel_err(cd.descriptors[0], c1_oracle["X"]) < TOL64
+    from helpers import ndp_kuu
+    Kuu = ndp_kuu(c1_oracle["X"], c1_oracle["N"], 2.0, 2.0)
+    assert rel_err(sgp.Kuu, Kuu) < TOL64
+
+
+def test_kuf_matches_oracle(C, oracle, c1, c1_oracle):
+    """Kuf columns: envs_struc (normalized_dot_product.cpp:117-214) of the
+    final sparse set against every training frame."""
+    sgp = c1["sgp"]
+    m = oracle.Model(2.0, 2.0, [c1_oracle["X"]], [c1_oracle["N"]], np.zeros(200))
+    ref = np.concatenate([m.envs_struc(d) for d in c1_oracle["descs"]], axis=1)
+    assert ref.shape == sgp.Kuf.shape
+    err = rel_err(sgp.Kuf, ref)
+    say(f"Kuf rel err {err:.2e}")
+    assert err < TOL64
+
+
+def test_alpha_normal_equations(C, c1):
+    """update_matrices_QR solves (Kuu + jitter I + Kuf Lambda Kfu) alpha =
+    Kuf Lambda y (sparse_gp.cpp:607-641)."""
+    sgp = c1["sgp"]
+    lam = sgp.noise_vector
+    A = sgp.Kuu + sgp.Kuu_jitter * np.eye(200) + (sgp.Kuf * lam) @ sgp.Kuf.T
+    alpha = np.linalg.solve(A, sgp.Kuf @ (lam * sgp.y))
+    err = rel_err(sgp.alpha, alpha)
+    say(f"alpha vs normal equations {err:.2e}")
+    assert err < 1e-6
+    # The trained model reproduces its LJ labels to within a few noise levels.
+    fit = sgp.Kuf.T @ sgp.alpha
+    f = slice(1, 1 + 3 * 108)
+    rmse_f = np.sqrt(np.mean((fit[f] - sgp.y[f]) ** 2))
+    say(f"training force rmse {rmse_f * 1e3:.2f} meV/A")
+    assert rmse_f < 0.1
+
+
+def test_predictions_match_oracle(C, oracle, c1):
+    """Mean, local variances, DTC and SOR of the trained model on a fresh
+    frame (seed 1100) against the oracle at the FP64 bar."""
+    from flare_amd.workloads import fcc_al_108
+    sgp, calc = c1["sgp"], c1["calc"]
+    cell, species, pos = fcc_al_108(frame=1100)
+    s = C.Structure(cell, species, pos, 5.0, [calc])
+    cd = sgp.sparse_descriptors[0]
+    m = oracle.Model(2.0, 2.0, cd.descriptors, cd.descriptor_norms, sgp.alpha, L_inv=sgp.L_inv,
+                     Kuu_inverse=sgp.Kuu_inverse, Sigma=sgp.Sigma, R_inv=sgp.R_inv)
+    d = oracle.B2Values(oracle.Neighbors(cell, species, pos, 5.0, "literal"), 1, 8, 3,
+                        np.full((1, 1), 5.0))
+    sgp.predict_local_uncertainties(s)
+    mean, var = m.predict(d, "local")
+    assert rel_err(s.mean_efs, mean) < TOL64
+    assert var_err(s.local_uncertainties[0], var, 4.0) < TOL64
+    sgp.predict_DTC(s)
+    _, vd = m.predict(d, "dtc")
+    assert rel_err(s.variance_efs, vd) < TOL64
+    sgp.predict_SOR(s)
+    _, vs = m.predict(d, "sor")
+    assert rel_err(s.variance_efs, vs) < TOL64
+    from flare_amd.workloads import lennard_jones
+    E, F, S = lennard_jones(cell, pos)
+    err_f = np.abs(s.mean_efs[1:-6] - F.reshape(-1)).max()
+    say(f"held-out frame: |dF|max {err_f * 1e3:.1f} meV/A against LJ")
+
+
+def test_add_order_gpu(C):
+    """ctests/test_sparse_gp.cpp AddOrder with the GPU kernel matrices."""
+    from flare_amd.workloads import c1_training_frames
+    calc = c1_calc(C)
+    (cell, species, pos, E, F, S), = c1_training_frames(1)
+    s = C.Structure(cell, species, pos, 5.0, [calc])
+    s.forces = F.reshape(-1)
+    a = C.SparseGP([C.NormalizedDotProduct(2.0, 2.0)], 1.0, 2.0, 3.0)
+    b = C.SparseGP([C.NormalizedDotProduct(2.0, 2.0)], 1.0, 2.0, 3.0)
+    a.add_training_structure(s)
+    a.add_all_environments(s)
+    a.update_matrices_QR()
+    b.add_all_environments(s)
+    b.add_training_structure(s)
+    b.update_matrices_QR()
+    np.testing.assert_array_equal(a.Kuf, b.Kuf)
+    np.testing.assert_array_equal(a.Kuu, b.Kuu)
+
+
+def test_uncertain_environments_sorted(C, c1):
+    """ctests/test_sparse_gp.cpp SortTest / add_uncertain_environments: the
+    added clusters are those of largest local variance."""
+    from flare_amd.workloads import fcc_al_108
+    sgp, calc = c1["sgp"], c1["calc"]
+    cell, species, pos = fcc_al_108(frame=1200)
+    s = C.Structure(cell, species, pos, 5.0, [calc])
+    var = sgp.compute_cluster_uncertainties(s)[0]
+    order = np.argsort(-var, kind="stable")
+    g = C.SparseGP([C.NormalizedDotProduct(2.0, 2.0)], 0.01, 0.05, 0.005)
+    for st, p in zip(sgp.training_structures[:2], [[0, 5], [1, 7]]):
+        g.add_training_structure(st)
+        g.add_specific_environments(st, p)
+    g.update_matrices_QR()
+    var_g = g.compute_cluster_uncertainties(s)[0]
+    top = np.argsort(-var_g, kind="stable")[:5]
+    g.add_uncertain_environments(s, [5])
+    assert g.n_sparse == 9
+    assert g.sparse_indices[0][-1] == [int(s.descriptors[0].atom_indices[0][i]) for i in top]
+    assert np.all(np.diff(var[order]) <= 0)
+
+
+def test_writers_use_training_calculator(C, c1, tmp_path):
+    """SparseGP.write_* (sparse_gp.cpp:1200-1466) take the descriptor
+    calculator of the first training structure; the coefficient file they
+    write reproduces the model's energies and forces."""
+    from flare_amd import mapping
+    from flare_amd.workloads import fcc_al_108
+    sgp, calc = c1["sgp"], c1["calc"]
+    p1, p2 = os.path.join(tmp_path, "a.flare"), os.path.join(tmp_path, "b.flare")
+    sgp.write_mapping_coefficients(p1, "tester", 0)
+    mapping.write_mapping_coefficients(sgp, p2, "tester", 0, calc)
+    a, b = open(p1).read().split("\n"), open(p2).read().split("\n")
+    assert a[1:] == b[1:]  # everything after the date line
+    for name in ("write_L_inverse", "write_sparse_descriptors"):
+        getattr(sgp, name)(p1, "tester")
+        assert os.path.getsize(p1) > 0
+    cell, species, pos = fcc_al_108(frame=1100)
+    s = C.Structure(cell, species, pos, 5.0, [calc])
+    sgp.predict_mean(s)
+    sgp.write_mapping_coefficients(p1, "tester", 0)
+    pot = mapping.MappedPotential(p1)
+    e, f, v = pot.compute(cell, species, pos)
+    assert np.sum(e) == pytest.approx(s.mean_efs[0], rel=1e-10)
+    assert rel_err(f.reshape(-1), s.mean_efs[1:-6]) < 1e-10
+
+
+def test_dtc_two_kernels(C, oracle):
+    """DTC / SOR with two kernels (sparse_gp.cpp:667-729: K stacked over
+    kernels, self kernels summed) against the oracle's per-kernel matrices."""
+    from helpers import random_cell
+    cell, species, pos = random_cell(20, 41, nos=2, scale=7.0, skew=0.3)
+    cuts = [np.array([[4.2, 3.8], [3.8, 4.0]]), np.full((2, 2), 3.6)]
+    calcs = [C.B2("chebyshev", "quadratic", [0.0, 4.2], [], [2, 4, 3], cuts[0]),
+             C.B2("chebyshev", "quadratic", [0.0, 3.6], [], [2, 3, 2], cuts[1])]
+    s = C.Structure(cell, species, pos, 4.2, calcs)
+    rng = np.random.default_rng(3)
+    s.energy = rng.normal(size=1)
+    s.forces = rng.normal(size=60)
+    s.stresses = rng.normal(size=6)
+    sgp = C.SparseGP([C.NormalizedDotProduct(1.5, 2.0), C.NormalizedDotProduct(0.8, 1.0)],
+                     0.1, 0.2, 0.3)
+    sgp.add_training_structure(s)
+    sgp.add_specific_environments(s, [0, 2, 4, 7, 9, 11])
+    sgp.update_matrices_QR()
+    t_cell, t_species, t_pos = random_cell(16, 42, nos=2, scale=6.5, skew=0.3)
+    t = C.Structure(t_cell, t_species, t_pos, 4.2, calcs)
+    nb = oracle.Neighbors(t_cell, t_species, t_pos, 4.2, "literal")
+    Ks, kself, mean = [], 0, 0
+    o = 0
+    for k, (calc, cut, kern) in enumerate(zip(calcs, cuts, sgp.kernels)):
+        d = oracle.B2Values(nb, 2, *calc.descriptor_settings[1:], cut)
+        cd = sgp.sparse_descriptors[k]
+        m = oracle.Model(kern.sigma, kern.power, cd.descriptors, cd.descriptor_norms,
+                         sgp.alpha[o:o + cd.n_clusters])
+        Ks.append(m.envs_struc(d))
+        kself = kself + m.self_kernel(d)
+        o += cd.n_clusters
+    K = np.concatenate(Ks)
+    mean = K.T @ sgp.alpha
+    dtc = kself - np.einsum("ic,ij,jc->c", K, sgp.Kuu_inverse, K) + np.einsum(
+        "ic,ij,jc->c", K, sgp.Sigma, K)
+    V = K.T @ sgp.R_inv
+    sgp.predict_DTC(t)
+    assert rel_err(t.mean_efs, mean) < TOL64
+    assert rel_err(t.variance_efs, dtc) < 1e-9
+    sgp.predict_SOR(t)
+    assert rel_err(t.variance_efs, np.sum(V * V, axis=1)) < TOL64
