@@ -413,6 +413,17 @@ class Structure:
     def wrap_positions(self):
         return self.wrapped_positions.copy()
 
+    @staticmethod
+    def to_json(file_name, struc):
+        """structure.cpp to_json: the reference's JSON layout."""
+        from . import serialization
+        serialization.dump(serialization.structure_to_json(struc), file_name)
+
+    @staticmethod
+    def from_json(file_name):
+        from . import serialization
+        return serialization.structure_from_json(serialization.load(file_name))
+
     def compute_descriptors(self):
         """structure.cpp:45-50. Each calculator's descriptors live in a device
         structure of their own (the first is the one built at construction;
@@ -556,6 +567,9 @@ class SparseGP:
         """Columns of structure j's labels in an envs_struc matrix K
         (energy, forces of the training atoms, stress; sparse_gp.cpp:423-456)."""
         st = self.training_structures[j]
+        if self.training_atom_indices[j] is None:
+            raise RuntimeError(f"the force atoms of training structure {j} are not known "
+                               "(a model file does not store them)")
         cols = []
         if np.size(st.energy):
             cols.append(K[:, :1])
@@ -1123,6 +1137,19 @@ class SparseGP:
         """sparse_gp.cpp:1404-1466."""
         from . import mapping
         mapping.write_sparse_descriptors(self, file_name, contributor)
+
+    # Model files (sparse_gp.cpp:1468-1479) ---------------------------------
+    @staticmethod
+    def to_json(file_name, sgp):
+        """Writes the model in the reference's JSON layout."""
+        from . import serialization
+        serialization.dump(serialization.sparse_gp_to_json(sgp), file_name)
+
+    @staticmethod
+    def from_json(file_name, context=None):
+        """Reads a model written by to_json (here or by the reference)."""
+        from . import serialization
+        return serialization.sparse_gp_from_json(serialization.load(file_name), context)
 
 
 class _DeviceModel:

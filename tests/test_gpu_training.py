@@ -247,3 +247,129 @@ def test_dtc_two_kernels(C, oracle):
     assert rel_err(t.variance_efs, dtc) < 1e-9
     sgp.predict_SOR(t)
     assert rel_err(t.variance_efs, np.sum(V * V, axis=1)) < TOL64
+
+
+def test_json_roundtrip_trained(C, c1, tmp_path):
+    """A trained model written with SparseGP.to_json and read back predicts
+    bit for bit the same, keeps its kernel matrices, and keeps training: one
+    more frame and sparse set gives the same state in both."""
+    from flare_amd.workloads import c1_training_frames, fcc_al_108
+    sgp, calc = c1["sgp"], c1["calc"]
+    path = str(tmp_path / "c1.json")
+    C.SparseGP.to_json(path, sgp)
+    g = C.SparseGP.from_json(path)
+    assert g.n_strucs == 20 and g.n_sparse == 200
+    np.testing.assert_array_equal(g.Kuf, sgp.Kuf)
+    cell, species, pos = fcc_al_108(frame=1100)
+    a = C.Structure(cell, species, pos, 5.0, [calc])
+    b = C.Structure(cell, species, pos, 5.0, [calc])
+    sgp.predict_local_uncertainties(a)
+    g.predict_local_uncertainties(b)
+    np.testing.assert_array_equal(a.mean_efs, b.mean_efs)
+    np.testing.assert_array_equal(a.local_uncertainties[0], b.local_uncertainties[0])
+    # Continue training both.
+    (cell, species, pos, E, F, S), = c1_training_frames(21)[20:]
+    for model in (sgp, g):
+        s = C.Structure(cell, species, pos, 5.0, [model.training_structures[0]
+                                                   .descriptor_calculators[0]])
+        s.energy, s.forces, s.stresses = np.array([E]), F.reshape(-1), S
+        model.add_training_structure(s)
+        model.add_specific_environments(s, [3, 50, 99])
+        model.update_matrices_QR()
+    assert rel_err(g.Kuf, sgp.Kuf) < 1e-14
+    assert rel_err(g.alpha, sgp.alpha) < 1e-9
+
+
+class Atoms:
+    """The ASE Atoms fields the calculator reads."""
+
+    def __init__(self, numbers, cell, positions):
+        self.numbers, self.cell, self.positions = np.asarray(numbers), cell, positions
+
+
+@pytest.fixture(scope="module")
+def wrapper(C):
+    """An SGP_Wrapper trained with update_db on three LJ frames (atomic
+    number 13 coded as species 0)."""
+    from flare_amd.sgp_wrapper import SGP_Wrapper
+    from flare_amd.workloads import c1_training_frames
+    gp = SGP_Wrapper([C.NormalizedDotProduct(2.0, 2.0)], [c1_calc(C)], 5.0, 0.01, 0.05, 0.005,
+                     {13: 0}, variance_type="local", single_atom_energies={0: -1.5})
+    for k, (cell, species, pos, E, F, S) in enumerate(c1_training_frames(3)):
+        gp.update_db(Atoms([13] * len(pos), cell, pos), F, custom_range=[0, 9 + k, 40, 77],
+                     energy=E + len(pos) * -1.5, stress=S, mode="specific", update_qr=False,
+                     rel_e_noise=1.0, rel_f_noise=0.5 + k, rel_s_noise=1.0)
+    gp.sparse_gp.update_matrices_QR()
+    return gp
+
+
+def test_wrapper_model_file(C, wrapper, tmp_path):
+    """write_model / from_file (sparse_gp.py:120-320) rebuild the same model
+    by replaying the training."""
+    from flare_amd.sgp_wrapper import SGP_Wrapper
+    from flare_amd.workloads import fcc_al_108
+    path = str(tmp_path / "gp.json")
+    wrapper.write_model(path)
+    gp2, kernels = SGP_Wrapper.from_file(path)
+    np.testing.assert_array_equal(gp2.sparse_gp.Kuf, wrapper.sparse_gp.Kuf)
+    np.testing.assert_array_equal(gp2.sparse_gp.noise_vector, wrapper.sparse_gp.noise_vector)
+    assert rel_err(gp2.sparse_gp.alpha, wrapper.sparse_gp.alpha) < 1e-12
+    assert gp2.single_atom_energies == {0: -1.5} and gp2.species_map == {13: 0}
+    from flare_amd.calculator import SGP_Calculator
+    cell, species, pos = fcc_al_108(frame=1100)
+    r1 = SGP_Calculator(wrapper).calculate(Atoms([13] * 108, cell, pos))
+    r2 = SGP_Calculator(gp2).calculate(Atoms([13] * 108, cell, pos))
+    for k in r1:
+        assert rel_err(np.atleast_1d(r2[k]), np.atleast_1d(r1[k])) < 1e-12, k
+
+
+def test_calculator_stress_and_energy(C, wrapper):
+    """ASE stress = (1/V) dE/d(strain) (Voigt xx, yy, zz, yz, xz, xy) against
+    central differences of the calculator's own energy; energy includes the
+    single-atom energies; forces = -dE/dx."""
+    from flare_amd.calculator import SGP_Calculator
+    from flare_amd.workloads import fcc_al_108
+    calc = SGP_Calculator(wrapper)
+    cell, species, pos = fcc_al_108(frame=1100)
+    r = dict(calc.calculate(Atoms([13] * 108, cell, pos)))
+    s = C.Structure(cell, [0] * 108, pos, 5.0, wrapper.descriptor_calculators)
+    wrapper.sparse_gp.predict_mean(s)
+    assert r["energy"] == pytest.approx(s.mean_efs[0] + 108 * -1.5, rel=1e-14)
+    V = abs(np.linalg.det(cell))
+    h = 1e-5
+    voigt = {(0, 0): 0, (1, 1): 1, (2, 2): 2, (1, 2): 3, (0, 2): 4, (0, 1): 5}
+    for (a, b), q in voigt.items():
+        es = []
+        for t in (h, -h):
+            c, p = cell.copy(), pos.copy()
+            c[:, a] += c[:, b] * t
+            p[:, a] += p[:, b] * t
+            es.append(calc.calculate(Atoms([13] * 108, c, p))["energy"])
+        assert r["stress"][q] * V == pytest.approx((es[0] - es[1]) / (2 * h), rel=1e-5, abs=1e-7)
+    for a, c_ in ((0, 0), (50, 2)):
+        es = []
+        for t in (h, -h):
+            p = pos.copy()
+            p[a, c_] += t
+            es.append(calc.calculate(Atoms([13] * 108, cell, p))["energy"])
+        assert r["forces"][a, c_] == pytest.approx(-(es[0] - es[1]) / (2 * h), abs=1e-6)
+    assert r["stds"].shape == (108, 3) and np.all(r["stds"][:, 1:] == 0)
+
+
+def test_build_map_std_identity(C, wrapper, tmp_path):
+    """build_map (calculator.py:169-185) writes the coefficient file and the
+    L_inv / sparse-descriptor files; compute flare/std/atom on those files
+    gives the calculator's local stds (tests/test_lammps.py:148-260)."""
+    from flare_amd.calculator import SGP_Calculator
+    from flare_amd.mapping import StdAtom
+    from flare_amd.workloads import fcc_al_108
+    calc = SGP_Calculator(wrapper)
+    base = str(tmp_path / "lmp.flare")
+    calc.build_map(base, "tester")
+    for f in (base, str(tmp_path / "L_inv_lmp.flare"), str(tmp_path / "sparse_desc_lmp.flare")):
+        assert os.path.getsize(f) > 0
+    cell, species, pos = fcc_al_108(frame=1100)
+    r = calc.calculate(Atoms([13] * 108, cell, pos))
+    std = StdAtom(str(tmp_path / "L_inv_lmp.flare"), str(tmp_path / "sparse_desc_lmp.flare"))
+    sig = std.compute(cell, [0] * 108, pos)
+    assert np.max(np.abs(sig - r["stds"][:, 0])) < 1e-9

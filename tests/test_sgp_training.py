@@ -184,3 +184,47 @@ def test_set_hyperparameters(C, oracle, strucs):
     assert a.log_marginal_likelihood == pytest.approx(b.log_marginal_likelihood, abs=1e-8)
     np.testing.assert_allclose(a.alpha, b.alpha, rtol=1e-8, atol=1e-12)
     assert b.kernels[0].sigma == 1.5 and b.force_noise == 1.5
+
+
+def test_json_layout_roundtrip(C, oracle, strucs, tmp_path):
+    """SparseGP::to_json / from_json (sparse_gp.cpp:1468-1479): the keys of
+    the nlohmann member list (sparse_gp.h:102-108), Eigen matrices as lists
+    of rows, empty ones as null; reading restores the state exactly."""
+    import json
+    sgp = trained(C, oracle, strucs)
+    sgp.compute_likelihood_gradient_stable()
+    sgp.training_structures = []  # structures need the GPU to rebuild
+    path = str(tmp_path / "sgp.json")
+    C.SparseGP.to_json(path, sgp)
+    j = json.load(open(path))
+    assert set(j) == {
+        "hyperparameters", "kernels", "Kuu_kernels", "Kuf_kernels", "Kuu", "Kuf", "n_kernels",
+        "Kuu_jitter", "Sigma", "Kuu_inverse", "R_inv", "L_inv", "alpha", "R_inv_diag",
+        "L_diag", "sparse_descriptors", "training_structures", "sparse_indices",
+        "noise_vector", "y", "label_count", "n_energy_labels", "n_force_labels",
+        "n_stress_labels", "n_sparse", "n_labels", "n_strucs", "energy_noise", "force_noise",
+        "stress_noise", "log_marginal_likelihood", "data_fit", "complexity_penalty",
+        "trace_term", "constant_term", "likelihood_gradient"}
+    assert j["kernels"][0] == {"sigma": 1.7, "sig2": 1.7 ** 2, "power": 2.0,
+                               "kernel_name": "NormalizedDotProduct",
+                               "kernel_hyperparameters": [1.7]}
+    assert len(j["Kuu"]) == sgp.n_sparse and len(j["Kuu"][0]) == sgp.n_sparse
+    cd = j["sparse_descriptors"][0]
+    assert set(cd) == {"descriptors", "descriptor_norms", "cutoff_values", "n_clusters_by_type",
+                       "cumulative_type_count", "n_descriptors", "n_types", "n_clusters"}
+    g = C.SparseGP.from_json(path)
+    for name in ("alpha", "L_inv", "Kuu_inverse", "Sigma", "R_inv", "Kuu", "Kuf", "y",
+                 "noise_vector", "R_inv_diag", "L_diag", "hyperparameters"):
+        np.testing.assert_array_equal(getattr(g, name), getattr(sgp, name), err_msg=name)
+    for a, b in zip(g.sparse_descriptors[0].descriptors, sgp.sparse_descriptors[0].descriptors):
+        np.testing.assert_array_equal(a, b)
+    assert g.n_sparse == sgp.n_sparse and g.n_labels == sgp.n_labels
+    assert g.log_marginal_likelihood == sgp.log_marginal_likelihood
+    assert g.kernels[0].sigma == 1.7 and g.kernels[0].power == 2.0
+    # An empty model writes its empty matrices as null and reads them back.
+    e = C.SparseGP([C.NormalizedDotProduct(1.0, 1.0)], 1.0, 1.0, 1.0)
+    C.SparseGP.to_json(path, e)
+    j = json.load(open(path))
+    assert j["Kuu"] is None and j["alpha"] is None
+    e2 = C.SparseGP.from_json(path)
+    assert e2.Kuu.shape == (0, 0) and e2.alpha.size == 0
