@@ -1006,7 +1006,9 @@ class SparseGP:
 
     # Device models ---------------------------------------------------------
     def _model(self, k, ctx):
-        key = (k, ctx.h.value)
+        # Each cached model holds its context, so an id cannot be reused by
+        # another context while its entry exists.
+        key = (k, id(ctx))
         m = self._models.get(key)
         if m is None or m.ctx is not ctx:
             m = _DeviceModel(self, k, ctx)
@@ -1156,6 +1158,15 @@ class _DeviceModel:
     """One kernel's resident model on a context."""
 
     def __init__(self, sgp, k, ctx):
+        d, self._keep = self.describe(sgp, k)
+        h = _lib.C.c_void_p()
+        _lib.check(_lib.load().fm_model_create(ctx.h, _lib.C.byref(d), _lib.C.byref(h)))
+        self.h = h
+        self.ctx = ctx
+
+    @staticmethod
+    def describe(sgp, k):
+        """fm_model_desc of kernel k (and the arrays it points into)."""
         kern = sgp.kernels[k]
         if not isinstance(kern, NormalizedDotProduct):
             raise ValueError("only NormalizedDotProduct kernels are supported on the GPU path")
@@ -1167,17 +1178,18 @@ class _DeviceModel:
             Linv = sgp.L_inv[off:off + M, off:off + M]
         if sgp.n_kernels == 1 and sgp.R_inv.shape == (M, M) and M:
             Rinv = sgp.R_inv
-        self._create(ctx, kern.sigma, kern.power, cd, sgp.alpha[off:off + M], Linv, Rinv)
+        return _DeviceModel._desc(kern.sigma, kern.power, cd, sgp.alpha[off:off + M], Linv, Rinv)
 
-    def _create(self, ctx, sigma, power, cd, alpha, Linv=None, Rinv=None):
+    @staticmethod
+    def _desc(sigma, power, cd, alpha, Linv=None, Rinv=None):
         M, nd = cd.n_clusters, cd.n_descriptors
         desc = _lib.f64(np.concatenate(cd.descriptors, axis=0) if M else np.zeros((0, nd)))
         norms = _lib.f64(np.concatenate(cd.descriptor_norms) if M else np.zeros(0))
         alpha = _lib.f64(alpha)
         Linv = _lib.f64(Linv) if Linv is not None else None
         Rinv = _lib.f64(Rinv) if Rinv is not None else None
-        self._keep = (desc, norms, alpha, Linv, Rinv)
         nbt = _lib.i32(cd.n_clusters_by_type)
+        keep = (desc, norms, alpha, Linv, Rinv, nbt)
         P = _lib.C.POINTER(_lib.C.c_double)
         d = _lib.ModelDesc(float(sigma), float(power), cd.n_types, nd,
                            nbt.ctypes.data_as(_lib.C.POINTER(_lib.C.c_int32)),
@@ -1185,6 +1197,10 @@ class _DeviceModel:
                            alpha.ctypes.data_as(P),
                            Linv.ctypes.data_as(P) if Linv is not None else None,
                            Rinv.ctypes.data_as(P) if Rinv is not None else None)
+        return d, keep
+
+    def _create(self, ctx, sigma, power, cd, alpha, Linv=None, Rinv=None):
+        d, self._keep = self._desc(sigma, power, cd, alpha, Linv, Rinv)
         h = _lib.C.c_void_p()
         _lib.check(_lib.load().fm_model_create(ctx.h, _lib.C.byref(d), _lib.C.byref(h)))
         self.h = h
@@ -1198,7 +1214,7 @@ class _DeviceModel:
 
     def __del__(self):
         try:
-            if getattr(self, "h", None) is not None and self.ctx.h is not None:
+            if getattr(self, "h", None) is not None:
                 _lib.load().fm_model_destroy(self.h)
         except Exception:
             pass

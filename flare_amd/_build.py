@@ -24,6 +24,7 @@ SOURCES = {
     "fm_mapped.hip": [],
     "fm_dtc.hip": [],
     "fm_b3.hip": [],
+    "fm_server.hip": [],
 }
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -42,7 +42,12 @@ EXPORTS = [
     "fm_batch_create", "fm_batch_create_device", "fm_batch_info",
     "fm_structure_get_b2_dervs", "fm_envs_struc", "fm_self_kernel_struc", "fm_predict_dense",
     "fm_structure_compute_b3", "fm_context_set_row_tile",
+    "fm_server_create", "fm_server_destroy", "fm_submit", "fm_poll",
+    "fm_last_kernel_timings",
 ]
+
+DESCRIPTOR_B2 = 2
+DESCRIPTOR_B3 = 3
 
 
 class FlareMIError(RuntimeError):
@@ -69,6 +74,14 @@ class ModelDesc(C.Structure):
                 ("sparse_descriptors", C.POINTER(C.c_double)),
                 ("sparse_norms", C.POINTER(C.c_double)), ("alpha", C.POINTER(C.c_double)),
                 ("L_inv", C.POINTER(C.c_double)), ("R_inv", C.POINTER(C.c_double))]
+
+
+class ServerDesc(C.Structure):
+    _fields_ = [("n_kernels", C.c_int), ("models", C.POINTER(ModelDesc)),
+                ("descriptors", C.POINTER(B2Settings)), ("descriptor_kinds", C.POINTER(C.c_int32)),
+                ("cutoff", C.c_double), ("form", C.c_int), ("flags", C.c_int),
+                ("n_workers", C.c_int), ("hbm_budget", C.c_int64),
+                ("max_batch_atoms", C.c_int32), ("max_density", C.c_double)]
 
 
 _lib = None
@@ -106,6 +119,7 @@ def load():
         L.fm_model_destroy.argtypes = [vp]
         L.fm_predict.argtypes = [vp, vp, ip, ip, vp, vp, vp]
         L.fm_last_timings.argtypes = [vp, vp, vp]
+        L.fm_last_kernel_timings.argtypes = [vp, ip, vp, vp]
         L.fm_model_mapping_coefficients.argtypes = [vp, vp, C.POINTER(C.c_int64)]
         L.fm_mapped_create.argtypes = [vp, C.POINTER(MappedDesc), pvp]
         L.fm_mapped_read_file.argtypes = [vp, C.c_char_p, pvp]
@@ -116,6 +130,10 @@ def load():
         L.fm_batch_create_device.argtypes = [vp, ip, vp, vp, vp, vp, dp, pvp]
         L.fm_batch_info.argtypes = [vp, ip, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                     C.POINTER(C.c_double)]
+        L.fm_server_create.argtypes = [ip, C.POINTER(ServerDesc), pvp]
+        L.fm_server_destroy.argtypes = [vp]
+        L.fm_submit.argtypes = [vp, vp, ip, vp, vp, C.POINTER(C.c_int64)]
+        L.fm_poll.argtypes = [vp, C.c_int64, ip, C.POINTER(C.c_int), vp, vp]
         for name in EXPORTS:
             getattr(L, name).restype = C.c_char_p if name == "fm_last_error" else C.c_int
         _lib = L
@@ -160,16 +178,17 @@ class Context:
         check(self.L.fm_context_synchronize(self.h))
 
     def close(self):
-        """Destroys the context (its stream and buffer pool). Models and
-        structures made on it must be released first."""
+        """Releases the context. Structures, models and potentials made on it
+        stay usable; the library deletes the context (its streams and buffer
+        pool) when the last of them is destroyed."""
         if self.h is not None:
             check(self.L.fm_context_destroy(self.h))
             self.h = None
 
     def __del__(self):
-        # Objects made on a context hold a reference to it, so it is unused
-        # here; at interpreter exit the order of finalisation is arbitrary and
-        # the process releases the device anyway.
+        # The library defers the deletion until the objects made on the
+        # context are gone, whatever order the collector finalises them in;
+        # at interpreter exit the process releases the device anyway.
         try:
             import sys
             if not sys.is_finalizing():
@@ -190,6 +209,16 @@ class Context:
 
     STAGES = ["neighbors", "descriptors", "kernel_contraction", "gradient_contraction",
               "variance", "adjoint_gather"]
+
+    KERNELS = ["descriptor", "slice", "kernel_contraction", "gradient_contraction",
+               "variance_contraction", "adjoint_dEdc", "bond_forces", "gather"]
+
+    def kernel_timings(self):
+        """{kernel: (ms summed over launches, launches)} of the last calls."""
+        ms = np.zeros(len(self.KERNELS))
+        n = np.zeros(len(self.KERNELS), np.int32)
+        check(self.L.fm_last_kernel_timings(self.h, len(self.KERNELS), ptr(ms), ptr(n)))
+        return {k: (float(a), int(b)) for k, a, b in zip(self.KERNELS, ms, n)}
 
     def timings(self):
         """{stage: (ms, launches)} of the stages last run on this context."""

@@ -901,6 +901,7 @@ void b3_order_model(fm_model m, fm_structure s) {
   FM_HIP(hipStreamSynchronize(st));
   m->b3_ntri = t.n_tri;
   m->b3_nls = t.n_ls;
+  m->xq_ready = false;  // slices follow X's new column order
 }
 
 void b3_rows_to_reference(fm_structure s, int r0, int rows, void *out) {
@@ -949,7 +950,7 @@ static size_t b3_pair_lds(const B3Dev &d, size_t esz, bool adjoint, int &nt) {
 }
 
 void launch_b3(fm_structure s) {
-  launch_b3_rows(s, 0, s->n, false, s->B.as<void>(), s->Bs[0].as<void>(), s->Bs[1].as<void>());
+  launch_b3_rows(s, 0, s->n, false, s->B.as<void>(), nullptr, nullptr);
 }
 
 void launch_b3_rows(fm_structure s, int r0, int n, bool tile, void *Bout, void *B0, void *B1) {

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <memory>
 #include <vector>
 
 #include "fm_gemm.hpp"
@@ -28,6 +29,14 @@ int guarded(F &&f) {
     return FM_ERR_INVALID;
   }
 }
+
+}  // namespace
+
+namespace fm {
+void set_last_error(const std::string &msg) { g_last_error = msg; }
+}  // namespace fm
+
+namespace {
 
 // Cofactor inverse and first-row determinant of a 3x3 matrix, the cell
 // algebra of structure.cpp:16-21 (cell_transpose_inverse, cell_dot_inverse,
@@ -183,6 +192,24 @@ void fm_context_s::mark_end(int s, hipStream_t on) {
   (void)hipEventRecord(ev_end[s], on ? on : stream);
   recorded[s] = true;
 }
+void fm_context_s::kt_clear(int first, int last) {
+  for (int k = first; k <= last; k++) kt[k].used = 0;
+}
+void fm_context_s::kt_begin(int k, hipStream_t on) {
+  KTimes &t = kt[k];
+  while ((int)t.ev.size() < 2 * (t.used + 1)) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    t.ev.push_back(e);
+  }
+  (void)hipEventRecord(t.ev[2 * t.used], on ? on : stream);
+}
+void fm_context_s::kt_end(int k, hipStream_t on) {
+  KTimes &t = kt[k];
+  if ((int)t.ev.size() < 2 * (t.used + 1)) return;
+  (void)hipEventRecord(t.ev[2 * t.used + 1], on ? on : stream);
+  t.used++;
+}
 
 extern "C" {
 
@@ -208,15 +235,16 @@ int fm_context_create(int device, fm_context *out) {
   });
 }
 
-int fm_context_destroy(fm_context ctx) {
-  return guarded([&] {
-    if (!ctx) return;
+// Releases the streams, events and pinned words of a context and deletes it.
+static void context_delete(fm_context ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     for (int s = 0; s < fm::ST_COUNT; s++) {
       (void)hipEventDestroy(ctx->ev_begin[s]);
       (void)hipEventDestroy(ctx->ev_end[s]);
     }
+    for (auto &t : ctx->kt)
+      for (hipEvent_t e : t.ev) (void)hipEventDestroy(e);
     hipStream_t st = ctx->stream, side = ctx->side;
     (void)hipStreamSynchronize(side);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
@@ -225,6 +253,23 @@ int fm_context_destroy(fm_context ctx) {
     delete ctx;
     (void)hipStreamDestroy(side);
     (void)hipStreamDestroy(st);
+}
+
+// An object made on ctx: counted until ctx_release.
+static void ctx_acquire(fm_context ctx) { ctx->life.fetch_add(2); }
+
+// An object made on ctx is gone: delete ctx if it was the last one and the
+// context's destroy call came before.
+static void ctx_release(fm_context ctx) {
+  if (ctx->life.fetch_sub(2) == 3) context_delete(ctx);
+}
+
+int fm_context_destroy(fm_context ctx) {
+  return guarded([&] {
+    if (!ctx) return;
+    int64_t old = ctx->life.fetch_or(1);
+    FM_CHECK(!(old & 1), FM_ERR_STATE, "context destroyed twice");
+    if (old == 0) context_delete(ctx);
   });
 }
 
@@ -350,6 +395,7 @@ static int structure_create(fm_context ctx, int n_struct, const double *cells,
       delete s;
       throw;
     }
+    ctx_acquire(ctx);
     *out = s;
   });
 }
@@ -392,9 +438,11 @@ int fm_batch_info(fm_structure s, int k, int *atom_offset, int *sweep, double *v
 int fm_structure_destroy(fm_structure s) {
   return guarded([&] {
     if (!s) return;
-    (void)hipStreamSynchronize(s->ctx->stream);
+    fm_context ctx = s->ctx;
+    (void)hipStreamSynchronize(ctx->stream);
     fm::free_b3_state(s);
     delete s;
+    ctx_release(ctx);
   });
 }
 
@@ -475,6 +523,7 @@ static void compute_descriptor(fm_structure s, const fm_b2_settings *st_in, int 
     FM_HIP(hipMemcpyAsync(s->pair_table.as<int>(), pairs.data(), pairs.size() * sizeof(int),
                           hipMemcpyHostToDevice, stream));
     ctx->mark_begin(fm::ST_DESC);
+    ctx->kt_clear(fm::KT_DESC, fm::KT_DESC);
     if (s->n > 0) fm::build_bonds(s);
     else {
       s->n_by_type.assign(s->nos, 0);
@@ -485,23 +534,17 @@ static void compute_descriptor(fm_structure s, const fm_b2_settings *st_in, int 
     // B3 rows too large to keep are formed in row tiles inside fm_predict.
     s->row_tile = 0;
     if (kind == 3 && s->n > 0) {
-      size_t per_row = (size_t)s->nd * (esz + (form == FM_FORM_CHOSEN ? 4 : 0));
+      size_t per_row = (size_t)s->nd * esz;
       if (ctx->row_tile > 0) {
         s->row_tile = ctx->row_tile;
       } else if ((size_t)s->n * per_row > (32ull << 30)) {
-        size_t tile_row = (size_t)s->nd * (2 * esz + 4);  // B, G (and pieces) per row
+        // B, G and (chosen form) the int8 slices of B, per row.
+        size_t tile_row = (size_t)s->nd * (2 * esz + (form == FM_FORM_CHOSEN ? fm::kI8Slices : 0));
         s->row_tile = std::max<int64_t>(256, (int64_t)((8ull << 30) / tile_row) / 256 * 256);
       }
     }
     size_t rows_kept = s->row_tile > 0 ? 0 : (size_t)s->n;
     s->B.alloc(pool, std::max<size_t>(1, rows_kept * s->nd * esz));
-    size_t split_bytes = rows_kept * fm::split_ld(s->nd) * sizeof(uint16_t);
-    for (auto &piece : s->Bs) {
-      if (form == FM_FORM_CHOSEN)
-        piece.alloc(pool, std::max<size_t>(1, split_bytes));
-      else
-        piece.release();
-    }
     s->norm.alloc(pool, std::max<size_t>(1, (size_t)s->n * sizeof(double)));
     s->nn.alloc(pool, std::max<size_t>(1, (size_t)s->n * sizeof(double)));
     if (s->n > 0) {
@@ -639,6 +682,7 @@ int fm_model_create(fm_context ctx, const fm_model_desc *d, fm_model *out) {
     FM_HIP(hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
     auto *m = new fm_model_s();
+    std::unique_ptr<fm_model_s> owner(m);  // deleted if a check below throws
     m->ctx = ctx;
     m->sigma = d->sigma;
     m->power = d->power;
@@ -757,6 +801,8 @@ int fm_model_create(fm_context ctx, const fm_model_desc *d, fm_model *out) {
                             hipMemcpyHostToDevice, st));
     }
     FM_HIP(hipStreamSynchronize(st));
+    owner.release();
+    ctx_acquire(ctx);
     *out = m;
   });
 }
@@ -764,8 +810,10 @@ int fm_model_create(fm_context ctx, const fm_model_desc *d, fm_model *out) {
 int fm_model_destroy(fm_model m) {
   return guarded([&] {
     if (!m) return;
-    (void)hipStreamSynchronize(m->ctx->stream);
+    fm_context ctx = m->ctx;
+    (void)hipStreamSynchronize(ctx->stream);
     delete m;
+    ctx_release(ctx);
   });
 }
 
@@ -889,6 +937,7 @@ int fm_mapped_create(fm_context ctx, const fm_mapped_desc *d, fm_mapped *out) {
       delete mp;
       throw;
     }
+    ctx_acquire(ctx);
     *out = mp;
   });
 }
@@ -908,6 +957,7 @@ int fm_mapped_read_file(fm_context ctx, const char *path, fm_mapped *out) {
       delete mp;
       throw;
     }
+    ctx_acquire(ctx);
     *out = mp;
   });
 }
@@ -929,8 +979,10 @@ int fm_mapped_info(fm_mapped mp, int *power, int *normalized, int *n_species, in
 int fm_mapped_destroy(fm_mapped mp) {
   return guarded([&] {
     if (!mp) return;
-    (void)hipStreamSynchronize(mp->ctx->stream);
+    fm_context ctx = mp->ctx;
+    (void)hipStreamSynchronize(ctx->stream);
     delete mp;
+    ctx_release(ctx);
   });
 }
 
@@ -941,6 +993,25 @@ int fm_mapped_compute(fm_mapped mp, fm_structure s, double *atom_energy, double 
     FM_CHECK(mp->ctx == s->ctx, FM_ERR_INVALID, "potential and structure belong to different contexts");
     FM_HIP(hipSetDevice(s->ctx->device));
     fm::mapped_compute(mp, s, atom_energy, forces, virial);
+  });
+}
+
+int fm_last_kernel_timings(fm_context ctx, int n, double *ms_out, int32_t *launches) {
+  return guarded([&] {
+    FM_CHECK(ctx && n >= 0, FM_ERR_INVALID, "null argument");
+    FM_HIP(hipStreamSynchronize(ctx->stream));
+    FM_HIP(hipStreamSynchronize(ctx->side));
+    for (int k = 0; k < n; k++) {
+      double total = 0;
+      int used = k < fm::KT_COUNT ? ctx->kt[k].used : 0;
+      for (int i = 0; i < used; i++) {
+        float ms = 0;
+        FM_HIP(hipEventElapsedTime(&ms, ctx->kt[k].ev[2 * i], ctx->kt[k].ev[2 * i + 1]));
+        total += ms;
+      }
+      if (ms_out) ms_out[k] = total;
+      if (launches) launches[k] = used;
+    }
   });
 }
 

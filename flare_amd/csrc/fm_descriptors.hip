@@ -273,10 +273,22 @@ struct SplitOut {
   __bf16 *p[2];
 };
 
-template <typename T, int LC, int CH>
+// The chosen form's outputs of the FP64 descriptor pass (k_b2 with MIXED):
+// the B row and c in FP32 (for the adjoint) and the FP64 row as kI8Slices
+// int8 slices with a power-of-two scale (the kernel contraction's operand,
+// see k_slice_rows in fm_gemm.hip for the slicing).
+struct MixedOut {
+  float *B, *C;
+  int8_t *q;
+  double *scale;
+  long long ldq, slice_stride;
+};
+
+template <typename T, int LC, int CH, bool MIXED = false>
 __global__ __launch_bounds__(64 * kAtomsPerBlock) void k_b2(B2Args a, T *B, double *norm,
                                                               double *nn, SplitOut Bs, int ldp,
-                                                              T *Cg, int want_b2, int work) {
+                                                              T *Cg, int want_b2, int work,
+                                                              MixedOut mo) {
   extern __shared__ __align__(16) unsigned char smem[];
   int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   int i = blockIdx.x * (blockDim.x >> 6) + w;
@@ -285,7 +297,11 @@ __global__ __launch_bounds__(64 * kAtomsPerBlock) void k_b2(B2Args a, T *B, doub
   T *c = reinterpret_cast<T *>(smem) + (size_t)w * work;  // work: elements per atom
   T *tab = c + nc;
   accumulate_c<T, LC, CH>(a, i, lane, c, tab);
-  for (int k = lane; k < nc; k += 64) Cg[(size_t)i * nc + k] = c[k];
+  if constexpr (MIXED) {
+    for (int k = lane; k < nc; k += 64) mo.C[(size_t)i * nc + k] = (float)c[k];
+  } else {
+    for (int k = lane; k < nc; k += 64) Cg[(size_t)i * nc + k] = c[k];
+  }
   if (!want_b2) return;
   // B_l = C_l C_l^T on the matrix cores (C_l[n][m] = c[n, l^2 + m]); the
   // upper triangle n2 >= n1 belongs at p(n1, n2) (L + 1) + l of the row. FP64
@@ -326,7 +342,7 @@ __global__ __launch_bounds__(64 * kAtomsPerBlock) void k_b2(B2Args a, T *B, doub
           if (n1 < nr && n2 < nr && n2 >= n1) {
             int k = (n1 * nr - (n1 * (n1 - 1)) / 2 + (n2 - n1)) * L1 + l;
             T v = acc[q];
-            if constexpr (sizeof(T) == 4)
+            if constexpr (sizeof(T) == 4 || MIXED)
               bst[k] = v;
             else
               Brow[k] = v;
@@ -355,6 +371,43 @@ __global__ __launch_bounds__(64 * kAtomsPerBlock) void k_b2(B2Args a, T *B, doub
       }
       *reinterpret_cast<bf16x8 *>(Bs.p[0] + (size_t)row * ldp + 8 * k8) = h0;
       *reinterpret_cast<bf16x8 *>(Bs.p[1] + (size_t)row * ldp + 8 * k8) = h1;
+    }
+  }
+  if constexpr (MIXED) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    float *B32 = mo.B + (size_t)row * a.nd;
+    double mx = 0;
+    for (int k = lane; k < a.nd; k += 64) {
+      B32[k] = (float)bst[k];
+      mx = fmax(mx, fabs((double)bst[k]));
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+    int E = 0;
+    if (mx > 0) frexp(mx, &E);  // mx = m 2^E, m in [0.5, 1)
+    double inv = ldexp(1.0, -E);
+    if (lane == 0) mo.scale[row] = ldexp(1.0, E);
+    int8_t *q = mo.q + (size_t)row * mo.ldq;
+    for (int k0 = 4 * lane; k0 < mo.ldq; k0 += 256) {
+      int packed[kI8Slices];
+#pragma unroll
+      for (int p = 0; p < kI8Slices; p++) packed[p] = 0;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        int k = k0 + j;
+        double r = k < a.nd ? (double)bst[k] * inv : 0.0;
+#pragma unroll
+        for (int p = 0; p < kI8Slices; p++) {
+          double v = rint(ldexp(r, 6 + 7 * p));
+          r -= ldexp(v, -(6 + 7 * p));
+          packed[p] |= ((int)v & 0xff) << (8 * j);
+        }
+      }
+#pragma unroll
+      for (int p = 0; p < kI8Slices; p++)
+        *reinterpret_cast<int *>(q + p * mo.slice_stride + k0) = packed[p];
     }
   }
   s2 = wave_sum(s2);
@@ -412,29 +465,43 @@ __device__ __forceinline__ void stage_row(const T *src, T *dst, int n, int lane,
 // in flight together), a fixed order per lane and a fixed wave reduction.
 template <typename T>
 __device__ __forceinline__ T row_dot(const T *G, const T *B, int n, int lane, int nt = 64) {
+  // Lane l takes the E-element groups v = l, l + nt, ... and the tail
+  // elements nv E + l, ...: the same elements in the same order whether a
+  // group arrives as one 16-byte load or as E scalar ones, so the sum does
+  // not depend on where the rows sit in memory (an atom's row is placed
+  // differently alone and in a batch).
   constexpr int E = 16 / sizeof(T);
+  using V = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
   T acc = 0;
-  int done = 0;
+  int nv = n / E;
   if (((reinterpret_cast<uintptr_t>(G) | reinterpret_cast<uintptr_t>(B)) & 15) == 0) {
-    using V = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
     const V *gv = reinterpret_cast<const V *>(G), *bv = reinterpret_cast<const V *>(B);
-    int nv = n / E;
 #pragma unroll 8
     for (int v = lane; v < nv; v += nt) {
       V x = gv[v], y = bv[v];
       const T *xs = reinterpret_cast<const T *>(&x), *ys = reinterpret_cast<const T *>(&y);
 #pragma unroll
-      for (int e = 0; e < E; e++) acc += xs[e] * ys[e];
+      for (int e = 0; e < E; e++) acc = fma(xs[e], ys[e], acc);
     }
-    done = nv * E;
+  } else {
+#pragma unroll 4
+    for (int v = lane; v < nv; v += nt)
+#pragma unroll
+      for (int e = 0; e < E; e++) acc = fma(G[E * v + e], B[E * v + e], acc);
   }
-  for (int k = done + lane; k < n; k += nt) acc += G[k] * B[k];
+  for (int k = nv * E + lane; k < n; k += nt) acc = fma(G[k], B[k], acc);
   return wave_sum(acc);
 }
 
 // w = sg G - sb B (the atom's dE/dB row) into LDS, n elements: 16-byte
 // loads of G and B when both rows are 16-byte aligned, four vectors of each
-// in flight per lane; B is not read when sb is zero.
+// in flight per lane; B is not read when sb is zero. Every element is the
+// same fma(sg, G, -sb B) on either path.
+template <typename T>
+__device__ __forceinline__ T w_elem(T sg, T g, T sb, T b) {
+  return fma(sg, g, -(sb * b));
+}
+
 template <typename T>
 __device__ __forceinline__ void stage_w(const T *G, const T *B, T sg, T sb, T *w, int n,
                                         int lane, int nt = 64) {
@@ -454,7 +521,7 @@ __device__ __forceinline__ void stage_w(const T *G, const T *B, T sg, T sb, T *w
         const T *xs = reinterpret_cast<const T *>(&x), *ys = reinterpret_cast<const T *>(&y);
         T *rs = reinterpret_cast<T *>(&r);
 #pragma unroll
-        for (int e = 0; e < E; e++) rs[e] = sg * xs[e] - sb * ys[e];
+        for (int e = 0; e < E; e++) rs[e] = w_elem(sg, xs[e], sb, ys[e]);
         wv[v] = r;
       }
     } else {
@@ -470,7 +537,8 @@ __device__ __forceinline__ void stage_w(const T *G, const T *B, T sg, T sb, T *w
     }
     done = nv * E;
   }
-  for (int k = done + lane; k < n; k += nt) w[k] = withb ? sg * G[k] - sb * B[k] : sg * G[k];
+  for (int k = done + lane; k < n; k += nt)
+    w[k] = withb ? w_elem(sg, G[k], sb, B[k]) : sg * G[k];
 }
 
 // FP32 rows: G . B and w = sg G - sb B (sb = (G . B) s3) in one pass over
@@ -499,14 +567,14 @@ __device__ __forceinline__ void stage_w_onepass(const float *G, const float *B, 
       const float *xs = reinterpret_cast<const float *>(&g);
       const float *ys = reinterpret_cast<const float *>(&b[j]);
 #pragma unroll
-      for (int e = 0; e < E; e++) acc += xs[e] * ys[e];
+      for (int e = 0; e < E; e++) acc = fma(xs[e], ys[e], acc);
       wv[v] = g;
     }
   }
   int kt = nv * E + t2;
   bool tail = live && kt < n;
   float gt = tail ? G[kt] : 0.f, bt = tail ? B[kt] : 0.f;
-  acc += gt * bt;
+  acc = fma(gt, bt, acc);
   acc = wave_sum(acc);
   if (lane == 0) red[w] = acc;
   __syncthreads();
@@ -519,11 +587,11 @@ __device__ __forceinline__ void stage_w_onepass(const float *G, const float *B, 
       float *rs = reinterpret_cast<float *>(&r);
       const float *ys = reinterpret_cast<const float *>(&b[j]);
 #pragma unroll
-      for (int e = 0; e < E; e++) rs[e] = sg * rs[e] - sb * ys[e];
+      for (int e = 0; e < E; e++) rs[e] = w_elem(sg, rs[e], sb, ys[e]);
       wv[v] = r;
     }
   }
-  if (tail) wl[kt] = sg * gt - sb * bt;
+  if (tail) wl[kt] = w_elem(sg, gt, sb, bt);
 }
 
 // Longest row for the one-pass staging of k_dEdc<float, LC, true>.
@@ -847,8 +915,13 @@ void launch_b2(fm_structure s, bool want_b2) {
   upload_knorm(st, s->lmax);
   upload_rcp(st);
   B2Args a = make_args(s);
+  // The chosen form's B2 rows are computed in FP64 (mixed: FP32 row and c
+  // for the adjoint, int8 slices of the FP64 row for the kernel
+  // contraction); c alone (the B3 bond pass) stays FP32.
+  bool mixed = want_b2 && s->form != FM_FORM_FP64;
   size_t esz = s->form == FM_FORM_FP64 ? sizeof(double) : sizeof(float);
-  // FP32 stages the B row after c (over the bond tables).
+  size_t csz = s->form == FM_FORM_FP64 || mixed ? sizeof(double) : sizeof(float);
+  // FP32 and the mixed pass stage the B row after c (over the bond tables).
   auto work_of = [&](int ch) {
     size_t w = b2_work(a.nrad * a.H, a.H, a.N, ch);
     if (want_b2 && s->form != FM_FORM_FP64)
@@ -857,14 +930,27 @@ void launch_b2(fm_structure s, bool want_b2) {
   };
   bool wide = 3 * work_of(64) <= 4 * work_of(32);
   size_t work = work_of(wide ? 64 : 32);
-  size_t per = work * esz;
+  size_t per = work * csz;
   int apb = atoms_per_block(per);
   size_t lds = apb * per;
   int nblk = (s->n + apb - 1) / apb;
   s->C.alloc(&ctx->pool, std::max<size_t>(1, (size_t)s->n * a.nrad * a.H * esz));
+  MixedOut mo{};
+  if (mixed) {
+    s->ldq = i8_ld(a.nd);
+    s->Bq.alloc(&ctx->pool, std::max<size_t>(1, (size_t)kI8Slices * s->n * s->ldq));
+    s->Bq_scale.alloc(&ctx->pool, std::max<size_t>(1, (size_t)s->n * sizeof(double)));
+    mo = MixedOut{s->B.as<float>(), s->C.as<float>(), s->Bq.as<int8_t>(),
+                  s->Bq_scale.as<double>(), s->ldq, (long long)s->n * s->ldq};
+  } else {
+    s->Bq.release();
+    s->Bq_scale.release();
+  }
+  s->has_q = mixed;
   if (nblk == 0) return;
+  // No BF16 copies of the rows: the chosen form's kernel contraction slices
+  // the FP32 rows itself.
   SplitOut none{{nullptr, nullptr}};
-  SplitOut pieces{{s->Bs[0].as<__bf16>(), s->Bs[1].as<__bf16>()}};
   // The unrolled sweep (compile-time l_max) is faster for the FP64 kernel;
   // for FP32 its extra registers cost more occupancy than it saves at
   // l_max 6.
@@ -875,20 +961,30 @@ void launch_b2(fm_structure s, bool want_b2) {
         constexpr int LC = decltype(lc)::value;
         hipLaunchKernelGGL((k_b2<double, LC, CH>), dim3(nblk), dim3(64 * apb), lds, st, a,
                            s->B.as<double>(), s->norm.as<double>(), s->nn.as<double>(), none, 0,
-                           s->C.as<double>(), (int)want_b2, (int)work);
+                           s->C.as<double>(), (int)want_b2, (int)work, mo);
+      });
+    else if (mixed)
+      with_lmax(a.lmax, [&](auto lc) {
+        constexpr int LC = decltype(lc)::value;
+        hipLaunchKernelGGL((k_b2<double, LC, CH, true>), dim3(nblk), dim3(64 * apb), lds, st,
+                           a, (double *)nullptr, s->norm.as<double>(), s->nn.as<double>(), none,
+                           0, (double *)nullptr, (int)want_b2, (int)work, mo);
       });
     else
       with_lmax(a.lmax <= 4 ? a.lmax : -1, [&](auto lc) {
         constexpr int LC = decltype(lc)::value;
         hipLaunchKernelGGL((k_b2<float, LC, CH>), dim3(nblk), dim3(64 * apb), lds, st, a,
-                           s->B.as<float>(), s->norm.as<double>(), s->nn.as<double>(), pieces,
-                           (int)split_ld(s->nd), s->C.as<float>(), (int)want_b2, (int)work);
+                           s->B.as<float>(), s->norm.as<double>(), s->nn.as<double>(), none, 0,
+                           s->C.as<float>(), (int)want_b2, (int)work, mo);
       });
   };
-  if (wide)
-    go(std::integral_constant<int, 64>{});
-  else
-    go(std::integral_constant<int, 32>{});
+  {
+    KScope timer(ctx, KT_DESC);
+    if (wide)
+      go(std::integral_constant<int, 64>{});
+    else
+      go(std::integral_constant<int, 32>{});
+  }
   FM_HIP(hipGetLastError());
 }
 
@@ -918,7 +1014,8 @@ void launch_adjoint(fm_structure s, const void *G, int64_t g_ld, void *fbond,
   // occupancy there; with three or more blocks they would).
   bool one_pass = !f64 && project && !energy && a.nd <= kOnePassRow &&
                   3 * (size_t)apb1 * per1 > (size_t)160 * 1024;
-  if (s->desc_kind == 2)
+  if (s->desc_kind == 2) {
+    KScope timer(ctx, KT_DEDC);
     with_lmax(a.lmax, [&](auto lc) {
       constexpr int LC = decltype(lc)::value;
       if (f64)
@@ -937,6 +1034,7 @@ void launch_adjoint(fm_structure s, const void *G, int64_t g_ld, void *fbond,
                            (const float *)G, (long long)g_ld, aa, s->C.as<float>(),
                            U.as<float>());
     });
+  }
   FM_HIP(hipGetLastError());
   launch_bond_forces(s, U.as<void>(), fbond, virial_atom, nullptr, s->n, empty_norm, empty_nn);
 }
@@ -955,6 +1053,7 @@ void launch_bond_forces(fm_structure s, const void *U, void *fbond, double *viri
   size_t per2 = (size_t)a.nos * a.H * pad4(a.N) * esz;
   int apb2 = atoms_per_block(per2);
   int nblk2 = (n_list + apb2 - 1) / apb2;
+  KScope timer(s->ctx, KT_BOND);
   if (f64)
     hipLaunchKernelGGL((k_bond_forces<double, -1>), dim3(nblk2), dim3(64 * apb2), apb2 * per2,
                        st, a, s->norm.as<double>(), s->nn.as<double>(), aa, (const double *)U,

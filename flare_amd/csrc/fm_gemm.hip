@@ -488,8 +488,238 @@ __global__ __launch_bounds__(256) void k_gemm_split(int M, int N, int Kpad, Spli
   run_epilogue<EP, float, float, TM, TN>(e, t, row0, col0, M, N, lane, wr, wc, red);
 }
 
+// ---------------------------------------------------------------------------
+// Exact-integer kernel contraction (chosen form). Every row x of either
+// operand is scaled by a power of two s = 2^E with max|x| < s, and r = x / s
+// is cut into kI8Slices signed 8-bit slices, r = sum_p a_p 2^-(6 + 7p) + tail
+// (|a_p| <= 64, |tail| <= 2^-(7 kI8Slices)): the first slice takes 6 bits and
+// every further one 7 (rounding to nearest, so each residual is at most half
+// a unit of the previous slice). The product x . y is then
+//   s_x s_y sum_{p + q < kI8Slices} 2^-(12 + 7(p + q)) (A_p . B_q),
+// each A_p . B_q exact on the int8 matrix cores (int32 accumulation; the
+// terms of equal p + q share one accumulator). Dropping the products with
+// p + q >= kI8Slices leaves about 34 significant bits, so the kernel values
+// k = sigma^2 S^p that the local variance amplifies by L_inv carry FP64-like
+// accuracy, where FP32 accumulation of BF16 pieces carries about 16 bits.
+// ---------------------------------------------------------------------------
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+// One wave per row: the row maximum, its power-of-two scale and the slices
+// of 4 consecutive values per lane (zeros past nd up to ldq).
+template <typename T>
+__global__ __launch_bounds__(256) void k_slice_rows(const T *X, long long ldx, int n, int nd,
+                                                    int8_t *Q, long long ldq,
+                                                    long long slice_stride, double *scale) {
+  int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (row >= n) return;
+  const T *x = X + (size_t)row * ldx;
+  double mx = 0;
+  for (int k = lane; k < nd; k += 64) mx = fmax(mx, fabs((double)x[k]));
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+  int E = 0;
+  if (mx > 0) frexp(mx, &E);  // mx = m 2^E, m in [0.5, 1)
+  double inv = ldexp(1.0, -E);
+  if (lane == 0) scale[row] = ldexp(1.0, E);
+  int8_t *q = Q + (size_t)row * ldq;
+  for (int k0 = 4 * lane; k0 < ldq; k0 += 256) {
+    int packed[kI8Slices];
+#pragma unroll
+    for (int p = 0; p < kI8Slices; p++) packed[p] = 0;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      int k = k0 + j;
+      double r = k < nd ? (double)x[k] * inv : 0.0;
+#pragma unroll
+      for (int p = 0; p < kI8Slices; p++) {
+        double a = rint(ldexp(r, 6 + 7 * p));
+        r -= ldexp(a, -(6 + 7 * p));
+        packed[p] |= ((int)a & 0xff) << (8 * j);
+      }
+    }
+#pragma unroll
+    for (int p = 0; p < kI8Slices; p++)
+      *reinterpret_cast<int *>(q + p * slice_stride + k0) = packed[p];
+  }
+}
+
+struct I8Ptrs {
+  const int8_t *p[kI8Slices];
+};
+
+// An int32 accumulator sums at most kI8Slices products per k of |a b| <=
+// 64^2 = 4096: 65536 k keep it below 2^31 (5 x 4096 x 65536 = 1.3e9).
+constexpr int kI8Flush = 65536;
+
+// (64 TMb) x 64 tiles, 4 waves (2 x 2), each (32 TMb) x 32 of
+// v_mfma_i32_32x32x32_i8 (lane l: operand row l % 32, bytes k = 16 (l / 32)
+// .. + 15; result column l % 32, rows 8 (j / 4) + 4 (l / 32) + j % 4). LDS
+// rows of 64 k-bytes padded to 80, which spreads a fragment read over the
+// banks. Epilogue as EpKernel, in FP64 from the exact sums: S, k (FP64), w as
+// BF16 pairs, energy partials per column tile.
+template <int TMb, bool LONG>
+__global__ __launch_bounds__(256) void k_gemm_i8(int M, int N, int Kq, I8Ptrs Ag, long long lda,
+                                                 const double *sa, I8Ptrs Bg, long long ldb,
+                                                 const double *sb, EpArgs e) {
+  constexpr int S = kI8Slices, BM = 64 * TMb, BN = 64, BK = 64, LDB = 80;
+  __shared__ __align__(16) int8_t As[S][BM * LDB];
+  __shared__ __align__(16) int8_t Bs[S][BN * LDB];
+  __shared__ double red[2 * BM];
+  int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  int wr = wid >> 1, wc = wid & 1;
+  int tx, ty;
+  xcd_tile(tx, ty);
+  int row0 = ty * BM, col0 = tx * BN;
+  v16i acc[S][TMb];
+  // With LONG (Kq > kI8Flush) the int32 sums of the slice products
+  // (p + q = d) are folded into one FP64 sum every kI8Flush k, before they
+  // could overflow; otherwise once, in the epilogue.
+  double sacc[TMb][16];
+  auto clear = [&]() {
+#pragma unroll
+    for (int d = 0; d < S; d++)
+#pragma unroll
+      for (int a = 0; a < TMb; a++) acc[d][a] = v16i{};
+  };
+  auto fold = [&]() {
+#pragma unroll
+    for (int a = 0; a < TMb; a++)
+#pragma unroll
+      for (int j = 0; j < 16; j++) {
+        double sum = 0;
+#pragma unroll
+        for (int d = S - 1; d >= 0; d--) sum += ldexp((double)acc[d][a][j], -(12 + 7 * d));
+        sacc[a][j] += sum;
+      }
+  };
+#pragma unroll
+  for (int a = 0; a < TMb; a++)
+#pragma unroll
+    for (int j = 0; j < 16; j++) sacc[a][j] = 0;
+  clear();
+  // Loader: thread tid takes 16 bytes (k offset 16 (tid % 4)) of rows
+  // tid / 4 + 64 j; rows past M or N read row 0 and load zeros.
+  int lr = tid >> 2, lk = 16 * (tid & 3);
+  v4i ra[S][TMb], rb[S];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int p = 0; p < S; p++) {
+#pragma unroll
+      for (int j = 0; j < TMb; j++) {
+        int r = row0 + lr + 64 * j;
+        const v4i *src = reinterpret_cast<const v4i *>(Ag.p[p] + (size_t)(r < M ? r : 0) * lda + k0 + lk);
+        ra[p][j] = r < M ? *src : v4i{};
+      }
+      int c = col0 + lr;
+      const v4i *src = reinterpret_cast<const v4i *>(Bg.p[p] + (size_t)(c < N ? c : 0) * ldb + k0 + lk);
+      rb[p] = c < N ? *src : v4i{};
+    }
+  };
+  gload(0);
+#pragma unroll 1
+  for (int k0 = 0; k0 < Kq; k0 += BK) {
+    if constexpr (LONG) {
+      if (k0 > 0 && k0 % kI8Flush == 0) {
+        fold();
+        clear();
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < S; p++) {
+#pragma unroll
+      for (int j = 0; j < TMb; j++)
+        *reinterpret_cast<v4i *>(&As[p][(lr + 64 * j) * LDB + lk]) = ra[p][j];
+      *reinterpret_cast<v4i *>(&Bs[p][lr * LDB + lk]) = rb[p];
+    }
+    __syncthreads();
+    if (k0 + BK < Kq) gload(k0 + BK);
+#pragma unroll
+    for (int ks = 0; ks < BK; ks += 32) {
+      int kk = ks + 16 * (lane >> 5);
+      v4i af[S][TMb], bf[S];
+#pragma unroll
+      for (int p = 0; p < S; p++) {
+#pragma unroll
+        for (int a = 0; a < TMb; a++)
+          af[p][a] = *reinterpret_cast<const v4i *>(
+              &As[p][(wr * 32 * TMb + 32 * a + (lane & 31)) * LDB + kk]);
+        bf[p] = *reinterpret_cast<const v4i *>(&Bs[p][(wc * 32 + (lane & 31)) * LDB + kk]);
+      }
+#pragma unroll
+      for (int p = 0; p < S; p++)
+#pragma unroll
+        for (int q = 0; q < S - p; q++)
+#pragma unroll
+          for (int a = 0; a < TMb; a++)
+            acc[p + q][a] =
+                __builtin_amdgcn_mfma_i32_32x32x32_i8(af[p][a], bf[q], acc[p + q][a], 0, 0, 0);
+    }
+  }
+  fold();
+  // Epilogue.
+  int pi = e.power == 2.0 ? 2 : (e.power == 1.0 ? 1 : 0);
+  int c = col0 + wc * 32 + (lane & 31);
+  bool cin = c < N;
+  double a_c = cin ? e.alpha[c] : 0.0, s_c = cin ? sb[c] : 0.0;
+  double rsum[TMb][16];
+#pragma unroll
+  for (int a = 0; a < TMb; a++)
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+      int r = row0 + wr * 32 * TMb + 32 * a + 8 * (j / 4) + 4 * (lane >> 5) + (j & 3);
+      rsum[a][j] = 0;
+      if (r >= M) continue;
+      if (cin) {
+        double sum = sacc[a][j];
+        double nrm = e.row_norm[r];
+        bool empty = nrm < kEmptyThresh;
+        double s = empty ? 0.0 : sum * sa[r] * s_c / nrm;
+        double kv = empty ? 0.0 : e.sig2 * fpow(s, e.power, pi);
+        double pm1 = pi == 2 ? s : (pi == 1 ? 1.0 : fpow(s, e.power - 1, 0));
+        double wv = empty ? 0.0 : e.sig2 * a_c * e.power * pm1;
+        float wf = (float)wv;
+        __bf16 h = (__bf16)wf;
+        reinterpret_cast<__bf16 *>(e.Wh)[(size_t)r * e.ldw + c] = h;
+        reinterpret_cast<__bf16 *>(e.Wl)[(size_t)r * e.ldw + c] = (__bf16)(wf - (float)h);
+        if (e.K) e.K_as<double>()[(size_t)r * e.ldk + c] = kv;
+        rsum[a][j] = kv * a_c;
+      } else if (c < e.ldw) {
+        // Padding columns [N, ldw) of w are zero: the reduction tail of the
+        // gradient contraction.
+        reinterpret_cast<__bf16 *>(e.Wh)[(size_t)r * e.ldw + c] = (__bf16)0.0f;
+        reinterpret_cast<__bf16 *>(e.Wl)[(size_t)r * e.ldw + c] = (__bf16)0.0f;
+      }
+    }
+  // Row sums over the 32 columns of each half-wave, then over the two wave
+  // columns through LDS: one energy partial per (column tile, row).
+#pragma unroll
+  for (int a = 0; a < TMb; a++)
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+      double v = rsum[a][j];
+#pragma unroll
+      for (int off = 16; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+      rsum[a][j] = v;
+    }
+  if ((lane & 31) == 0) {
+#pragma unroll
+    for (int a = 0; a < TMb; a++)
+#pragma unroll
+      for (int j = 0; j < 16; j++)
+        red[wc * BM + wr * 32 * TMb + 32 * a + 8 * (j / 4) + 4 * (lane >> 5) + (j & 3)] =
+            rsum[a][j];
+  }
+  __syncthreads();
+  for (int q = tid; q < BM; q += blockDim.x) {
+    int r = row0 + q;
+    if (r < M) e.part[(size_t)(col0 / BN) * e.part_ld + r] = red[q] + red[BM + q];
+  }
+}
+
 int gemm_col_tiles(int form, int N) {
-  int bn = form == FM_FORM_FP64 ? F64_BN : B16_BN;
+  int bn = form == FM_FORM_FP64 ? F64_BN : kI8TileN;
   return (N + bn - 1) / bn;
 }
 
@@ -631,7 +861,59 @@ void split_shape(int i, hipStream_t st, int M, int N, int Kpad, const SplitPtrs 
   }
 }
 
+// The integer contraction runs 128 x 64 tiles: the 64-row tile spills
+// registers, and the column tile is fixed at 64 (the energy partials are
+// per column tile).
+
+template <int TMb>
+void i8_launch(hipStream_t st, int M, int N, int Kq, const I8Ptrs &a, long long lda,
+               const double *sa, const I8Ptrs &b, long long ldb, const double *sb,
+               const EpArgs &e) {
+  dim3 grid((N + kI8TileN - 1) / kI8TileN, (M + 64 * TMb - 1) / (64 * TMb));
+  if (Kq > kI8Flush)
+    hipLaunchKernelGGL((k_gemm_i8<TMb, true>), grid, dim3(256), 0, st, M, N, Kq, a, lda, sa, b,
+                       ldb, sb, e);
+  else
+    hipLaunchKernelGGL((k_gemm_i8<TMb, false>), grid, dim3(256), 0, st, M, N, Kq, a, lda, sa, b,
+                       ldb, sb, e);
+}
+
 }  // namespace
+
+void slice_rows(hipStream_t st, const float *X, long long ldx, int n, int nd, int8_t *Q,
+                long long ldq, long long slice_stride, double *scale) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_slice_rows<float>, dim3((n + 3) / 4), dim3(256), 0, st, X, ldx, n, nd, Q,
+                     ldq, slice_stride, scale);
+  FM_HIP(hipGetLastError());
+}
+
+void slice_rows(hipStream_t st, const double *X, long long ldx, int n, int nd, int8_t *Q,
+                long long ldq, long long slice_stride, double *scale) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_slice_rows<double>, dim3((n + 3) / 4), dim3(256), 0, st, X, ldx, n, nd,
+                     Q, ldq, slice_stride, scale);
+  FM_HIP(hipGetLastError());
+}
+
+void gemm_i8_kernel(hipStream_t st, int M, int N, int Kq, const SliceOperand &A,
+                    const SliceOperand &B, const EpArgs &e) {
+  if (M <= 0 || N <= 0) return;
+  FM_CHECK(Kq % kI8Pad == 0 && A.ld % kI8Pad == 0 && B.ld % kI8Pad == 0 && Kq <= A.ld &&
+               Kq <= B.ld,
+           FM_ERR_INVALID, "sliced operands must be padded to 64 bytes");
+  FM_CHECK(e.Wh && e.Wl && e.alpha && e.row_norm && e.part, FM_ERR_INVALID,
+           "integer contraction: missing epilogue outputs");
+  I8Ptrs a{}, b{};
+  for (int p = 0; p < kI8Slices; p++) {
+    a.p[p] = A.q + p * A.slice_stride;
+    b.p[p] = B.q + p * B.slice_stride;
+    FM_CHECK(((uintptr_t)a.p[p] | (uintptr_t)b.p[p]) % 16 == 0, FM_ERR_INVALID,
+             "sliced operands must be 16-byte aligned");
+  }
+  i8_launch<2>(st, M, N, Kq, a, A.ld, A.scale, b, B.ld, B.scale, e);
+  FM_HIP(hipGetLastError());
+}
 
 void gemm_f64(hipStream_t st, int ep, int M, int N, int K, const double *A, long long lda,
               const double *B, long long ldb, int lower_tri, const EpArgs &e) {
@@ -659,6 +941,9 @@ void gemm_f64(hipStream_t st, int ep, int M, int N, int K, const double *A, long
 void gemm_split(hipStream_t st, int ep, int np, int M, int N, int Kpad, const SplitOperand &A,
                 const SplitOperand &B, const EpArgs &e) {
   if (M <= 0 || N <= 0) return;
+  // The kernel contraction runs on int8 slices (gemm_i8_kernel); the split
+  // form serves the gradient contraction.
+  FM_CHECK(ep == EP_STORE, FM_ERR_INVALID, "split-BF16 contraction: EP_STORE only");
   FM_CHECK(np == 2, FM_ERR_INVALID, "split operands have 2 pieces");
   FM_CHECK(Kpad % kSplitPad == 0 && A.ld % kSplitPad == 0 && B.ld % kSplitPad == 0 &&
                Kpad <= A.ld && Kpad <= B.ld,
@@ -672,9 +957,7 @@ void gemm_split(hipStream_t st, int ep, int np, int M, int N, int Kpad, const Sp
     b.p[pc] = reinterpret_cast<const __bf16 *>(B.piece[pc]);
   }
   auto launch = [&](int i) {
-    if (ep == EP_KERNEL) split_shape<EP_KERNEL>(i, st, M, N, Kpad, a, A.ld, b, B.ld, e);
-    else if (ep == EP_STORE) split_shape<EP_STORE>(i, st, M, N, Kpad, a, A.ld, b, B.ld, e);
-    else split_shape<EP_SUMSQ>(i, st, M, N, Kpad, a, A.ld, b, B.ld, e);
+    split_shape<EP_STORE>(i, st, M, N, Kpad, a, A.ld, b, B.ld, e);
     FM_HIP(hipGetLastError());
   };
   launch(tuned(1, ep, M, N, Kpad, st, kSplitShapes, 7, launch));

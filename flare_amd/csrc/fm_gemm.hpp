@@ -50,7 +50,8 @@ void gemm_f64(hipStream_t st, int ep, int M, int N, int K, const double *A, long
 // EP_SUMSQ only (the variance contraction on the split form's k matrix).
 void gemm_f64(hipStream_t st, int ep, int M, int N, int K, const double *A, long long lda,
               const float *B, long long ldb, int lower_tri, const EpArgs &e);
-// Split-BF16 x3 contraction on operands stored as two BF16 pieces of FP32
+// Split-BF16 x3 contraction (EP_STORE: the gradient contraction G = W X_hat
+// of the chosen form) on operands stored as two BF16 pieces of FP32
 // values, x = x_0 + x_1 with x_0 = bf16(x), x_1 = bf16(x - x_0); it sums
 // x_1 y_0 + x_0 y_1 + x_0 y_0 (3 MFMAs per product, about 16 significant
 // bits; FP32 accumulation). Rows of A and B are K-contiguous with leading
@@ -65,6 +66,26 @@ struct SplitOperand {
 };
 void gemm_split(hipStream_t st, int ep, int np, int M, int N, int Kpad, const SplitOperand &A,
                 const SplitOperand &B, const EpArgs &e);
+// Exact-integer kernel contraction (the chosen form's S = B X_hat^T with the
+// EP_KERNEL epilogue, k in FP64 and w as BF16 pairs): both operands as
+// kI8Slices int8 slices of their rows, slice p at q + p slice_stride, rows of
+// ld bytes (a multiple of kI8Pad, zero padded), with a power-of-two scale per
+// row. The slices come from slice_rows.
+constexpr int kI8Slices = 5;
+constexpr int kI8Pad = 64;
+constexpr int kI8TileN = 64;  // column tile: the energy partials are per column tile
+inline long long i8_ld(long long k) { return (k + kI8Pad - 1) / kI8Pad * kI8Pad; }
+struct SliceOperand {
+  const int8_t *q = nullptr;
+  long long ld = 0, slice_stride = 0;
+  const double *scale = nullptr;
+};
+void slice_rows(hipStream_t st, const float *X, long long ldx, int n, int nd, int8_t *Q,
+                long long ldq, long long slice_stride, double *scale);
+void slice_rows(hipStream_t st, const double *X, long long ldx, int n, int nd, int8_t *Q,
+                long long ldq, long long slice_stride, double *scale);
+void gemm_i8_kernel(hipStream_t st, int M, int N, int Kq, const SliceOperand &A,
+                    const SliceOperand &B, const EpArgs &e);
 // Host-side split of FP32 values into np BF16 pieces, rounding to nearest
 // even as the device conversion does.
 void split_bf16_host(const float *x, size_t n, int np, uint16_t *const *pieces);

@@ -8,6 +8,7 @@
 #include <map>
 #include <stdexcept>
 #include <string>
+#include <atomic>
 #include <vector>
 
 #include "../../include/flare_mi.h"
@@ -28,6 +29,10 @@ struct Error : std::runtime_error {
       throw ::fm::Error(FM_ERR_HIP, std::string(#call) + ": " +              \
                                         hipGetErrorString(e_));               \
   } while (0)
+
+// Sets the calling thread's fm_last_error() message (entry points that do
+// not go through the C ABI's error wrapper).
+void set_last_error(const std::string &msg);
 
 #define FM_CHECK(cond, code, msg)                     \
   do {                                                \
@@ -105,6 +110,19 @@ struct CellArgs {
 };
 
 enum Stage { ST_NEIGH = 0, ST_DESC, ST_SGEMM, ST_GGEMM, ST_VAR, ST_ADJ, ST_COUNT };
+// Per-kernel timers (fm_last_kernel_timings): each launch of these kernels
+// is bracketed by an event pair on the stream it runs on.
+enum KernelTimer {
+  KT_DESC = 0,  // B2 descriptor kernel (bond basis, c, B)
+  KT_SLICE,     // int8 slicing of descriptor rows (chosen form)
+  KT_SGEMM,     // kernel contraction S = B X_hat^T with its epilogue
+  KT_GGEMM,     // gradient contraction G = W X_hat
+  KT_VAR,       // variance contraction Q = L_s^-1 K^T, sum of squares
+  KT_DEDC,      // adjoint u = dE/dc
+  KT_BOND,      // per-bond forces and virials
+  KT_GATHER,    // per-atom force gather and reductions
+  KT_COUNT
+};
 
 }  // namespace fm
 
@@ -129,7 +147,36 @@ struct fm_context_s {
   int launches[fm::ST_COUNT] = {};
   void mark_begin(int s, hipStream_t on = nullptr);
   void mark_end(int s, hipStream_t on = nullptr);
+  // Kernel timers: event pairs of the launches since the last kt_clear.
+  struct KTimes {
+    std::vector<hipEvent_t> ev;
+    int used = 0;
+  };
+  KTimes kt[fm::KT_COUNT];
+  void kt_clear(int first, int last);
+  void kt_begin(int k, hipStream_t on = nullptr);
+  void kt_end(int k, hipStream_t on = nullptr);
+  // Lifetime: 2 x (structures, models and potentials made on the context)
+  // + 1 once fm_context_destroy was called. The context is deleted by
+  // whichever comes last: the destroy call with no objects left, or the
+  // destruction of the last object after the destroy call. Objects return
+  // their buffers to the context's pool, so the order the caller releases
+  // them in does not matter.
+  std::atomic<int64_t> life{0};
 };
+
+namespace fm {
+// Brackets the launches made in its scope with a kernel timer.
+struct KScope {
+  fm_context c;
+  int k;
+  hipStream_t s;
+  KScope(fm_context c_, int k_, hipStream_t s_ = nullptr) : c(c_), k(k_), s(s_) {
+    c->kt_begin(k, s);
+  }
+  ~KScope() { c->kt_end(k, s); }
+};
+}  // namespace fm
 
 struct fm_structure_s {
   fm_context ctx = nullptr;
@@ -172,12 +219,15 @@ struct fm_structure_s {
   std::vector<int> n_by_type, nb_by_type, type_off;
   fm::DevBuf d_cutoffs, bond_off, bond_q, in_off, in_bond, row_of_atom,
       atom_of_row, pair_table, B, norm, nn;
-  // Chosen form: the descriptors also as two BF16 pieces (n x split_ld(d)),
-  // the operand of the kernel contraction.
-  fm::DevBuf Bs[2];
   // Density coefficients c (atom-major, n x nrad H, descriptor precision),
   // kept for the adjoint.
   fm::DevBuf C;
+  // Chosen form, B2: the FP64 descriptor rows as int8 slices (kI8Slices x n x
+  // ldq, species-sorted rows) with a power-of-two scale per row, written by
+  // the descriptor pass; the kernel contraction's operand.
+  fm::DevBuf Bq, Bq_scale;
+  long long ldq = 0;
+  bool has_q = false;
   // Dense descriptor derivatives (FP64, single structure): bonds in the
   // species-sorted order of DescriptorValues, f in [sb_off[row], sb_off[row+1])
   // for species-sorted row `row`; bonds of species t are [nbt_off[t],
@@ -201,6 +251,11 @@ struct fm_model_s {
   // padded), alpha (zero for empty sparse environments) and the L_inv block
   // (L_rows x M_s, starting at the species' first row).
   std::vector<fm::DevBuf> X, XT, Xs0, Xs1, XTs0, XTs1, alpha, L;
+  // Chosen form: X-hat rows as int8 slices with their power-of-two scales
+  // (the kernel contraction's operand), made from X on first use and again
+  // after the B3 reordering of X.
+  std::vector<fm::DevBuf> Xq, Xq_scale;
+  bool xq_ready = false;
   std::vector<double> host_alpha;
   // Non-zero once the sparse set has been permuted to the device order of
   // B3 descriptors (l-set major) for (n_tri, n_ls) = (b3_ntri, b3_nls).

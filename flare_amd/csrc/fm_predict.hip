@@ -126,6 +126,7 @@ static void gather_forces(fm_structure s, bool f64, DevBuf &fbond, DevBuf &vir, 
   hipStream_t st = s->ctx->stream;
   Pool *pool = &s->ctx->pool;
   DevBuf blk(pool, (size_t)s->n_chunks * 7 * sizeof(double));
+  KScope timer(s->ctx, KT_GATHER);
   if (f64)
     hipLaunchKernelGGL(k_gather<double>, dim3(s->n_chunks), dim3(256), 0, st,
                        s->d_chunk_begin.as<int>(), s->d_chunk_end.as<int>(),
@@ -258,9 +259,63 @@ static int species_tile_rows(fm_context ctx, int T, int Ms, int form) {
           hipSuccess ||
       cus <= 0)
     return T;
-  int per_round = 3 * cus / ((Ms + 127) / 128);
+  // The integer contraction: 128 x 64 tiles, two workgroups per CU.
+  int per_round = 2 * cus / ((Ms + kI8TileN - 1) / kI8TileN);
   int rounds = per_round > 0 ? (T / 128) / per_round : 0;
   return rounds > 0 ? rounds * per_round * 128 : T;
+}
+
+// The model's sparse rows as int8 slices for the chosen form's kernel
+// contraction, made on the context stream from the FP64 rows on first use.
+static void model_slices(fm_model m, hipStream_t st) {
+  if (m->xq_ready) return;
+  int nd = m->nd;
+  long long ldq = i8_ld(nd);
+  Pool *pool = &m->ctx->pool;
+  m->Xq.resize(m->nos);
+  m->Xq_scale.resize(m->nos);
+  for (int t = 0; t < m->nos; t++) {
+    int Ms = m->M_by_type[t];
+    if (Ms == 0) continue;
+    m->Xq[t].alloc(pool, (size_t)kI8Slices * Ms * ldq);
+    m->Xq_scale[t].alloc(pool, (size_t)Ms * sizeof(double));
+    slice_rows(st, m->X[t].as<double>(), nd, Ms, nd, m->Xq[t].as<int8_t>(), ldq,
+               (long long)Ms * ldq, m->Xq_scale[t].as<double>());
+  }
+  m->xq_ready = true;
+}
+
+// Chosen form: `cnt` FP32 descriptor rows (leading dimension nd) as int8
+// slices in Aq / Aq_scale (for rows without slices of their FP64 values).
+static SliceOperand slice_fp32_rows(fm_context ctx, hipStream_t st, const float *rows, int cnt,
+                                    int nd, DevBuf &Aq, DevBuf &Aq_scale) {
+  long long ldq = i8_ld(nd);
+  {
+    KScope timer(ctx, KT_SLICE, st);
+    slice_rows(st, rows, nd, cnt, nd, Aq.as<int8_t>(), ldq, (long long)cnt * ldq,
+               Aq_scale.as<double>());
+  }
+  SliceOperand A;
+  A.q = Aq.as<int8_t>();
+  A.ld = ldq;
+  A.slice_stride = (long long)cnt * ldq;
+  A.scale = Aq_scale.as<double>();
+  return A;
+}
+
+// Chosen form: kernel contraction of `cnt` sliced descriptor rows against
+// species t's sparse set.
+static void kernel_contraction_i8(fm_model m, hipStream_t st, int t, const SliceOperand &A,
+                                  int cnt, int nd, const EpArgs &ek) {
+  long long ldq = i8_ld(nd);
+  int Ms = m->M_by_type[t];
+  SliceOperand Bx;
+  Bx.q = m->Xq[t].as<int8_t>();
+  Bx.ld = ldq;
+  Bx.slice_stride = (long long)Ms * ldq;
+  Bx.scale = m->Xq_scale[t].as<double>();
+  KScope timer(m->ctx, KT_SGEMM, st);
+  gemm_i8_kernel(st, cnt, Ms, (int)ldq, A, Bx, ek);
 }
 
 // B3 structures too large to keep their descriptors (s->row_tile > 0): each
@@ -301,21 +356,22 @@ static void predict_tiled(fm_model m, fm_structure s, int flags, int form, doubl
   DevBuf U(pool, std::max<size_t>(1, (size_t)n * nc * esz));
   DevBuf fbond(pool, std::max<size_t>(1, (size_t)3 * s->n_bonds * esz));
   DevBuf vir(pool, std::max<size_t>(1, (size_t)6 * n * sizeof(double)));
-  // Tile buffers.
-  long long ldd = split_ld(nd), ldm = split_ld(std::max(Mmax, 1));
-  size_t kld = f64 ? (size_t)Mmax : ((size_t)Mmax + 3) / 4 * 4;
+  // Tile buffers. k is FP64 in both forms (rows padded to 16 bytes).
+  long long ldm = split_ld(std::max(Mmax, 1));
+  size_t kld = ((size_t)Mmax + 1) / 2 * 2;
   DevBuf Bt(pool, std::max<size_t>(1, (size_t)T * nd * esz));
   DevBuf Gt(pool, std::max<size_t>(1, (size_t)T * nd * esz));
-  DevBuf Bt0, Bt1, Wt, Wh, Wl, Kt;
+  DevBuf Tq, Tq_scale, Wt, Wh, Wl, Kt;
   if (f64) {
     Wt.alloc(pool, std::max<size_t>(1, (size_t)T * Mmax * sizeof(double)));
   } else {
-    Bt0.alloc(pool, std::max<size_t>(1, (size_t)T * ldd * sizeof(uint16_t)));
-    Bt1.alloc(pool, std::max<size_t>(1, (size_t)T * ldd * sizeof(uint16_t)));
+    model_slices(m, st);
+    Tq.alloc(pool, std::max<size_t>(1, (size_t)kI8Slices * T * i8_ld(nd)));
+    Tq_scale.alloc(pool, std::max<size_t>(1, (size_t)T * sizeof(double)));
     Wh.alloc(pool, std::max<size_t>(1, (size_t)T * ldm * sizeof(uint16_t)));
     Wl.alloc(pool, std::max<size_t>(1, (size_t)T * ldm * sizeof(uint16_t)));
   }
-  if (want_var) Kt.alloc(pool, std::max<size_t>(1, (size_t)T * kld * esz));
+  if (want_var) Kt.alloc(pool, std::max<size_t>(1, (size_t)T * kld * sizeof(double)));
   double sig2 = m->sigma * m->sigma;
   ctx->mark_begin(ST_SGEMM);
   for (int t = 0; t < nos; t++) {
@@ -325,8 +381,7 @@ static void predict_tiled(fm_model m, fm_structure s, int flags, int form, doubl
     int Ts = species_tile_rows(ctx, T, Ms, form);
     for (int c0 = 0; c0 < ns; c0 += Ts) {
       int cnt = std::min(Ts, ns - c0), r0 = ro + c0;
-      launch_b3_rows(s, r0, cnt, true, Bt.as<void>(), f64 ? nullptr : Bt0.as<void>(),
-                     f64 ? nullptr : Bt1.as<void>());
+      launch_b3_rows(s, r0, cnt, true, Bt.as<void>(), nullptr, nullptr);
       double *ep = Epart.as<double>() + eo[t] + c0;
       if (Ms == 0) {
         hipLaunchKernelGGL(k_fill, dim3((cnt + 255) / 256), dim3(256), 0, st, ep,
@@ -347,37 +402,27 @@ static void predict_tiled(fm_model m, fm_structure s, int flags, int form, doubl
           ek.Wl = Wl.as<uint16_t>();
           ek.ldw = split_ld(Ms);
         }
-        size_t kl = f64 ? (size_t)Ms : ((size_t)Ms + 3) / 4 * 4;
+        size_t kl = ((size_t)Ms + 1) / 2 * 2;
         if (want_var) {
           ek.K = Kt.as<void>();
           ek.ldk = (long long)kl;
         }
         ek.part = ep;
         ek.part_ld = ns;
-        if (f64) {
+        if (f64)
           gemm_f64(st, EP_KERNEL, cnt, Ms, nd, Bt.as<double>(), nd, m->X[t].as<double>(), nd, 0,
                    ek);
-        } else {
-          SplitOperand A, Bx;
-          A.piece[0] = Bt0.as<uint16_t>();
-          A.piece[1] = Bt1.as<uint16_t>();
-          A.ld = ldd;
-          Bx.piece[0] = m->Xs0[t].as<uint16_t>();
-          Bx.piece[1] = m->Xs1[t].as<uint16_t>();
-          Bx.ld = ldd;
-          gemm_split(st, EP_KERNEL, 2, cnt, Ms, (int)ldd, A, Bx, ek);
-        }
+        else
+          kernel_contraction_i8(m, st, t,
+                                slice_fp32_rows(ctx, st, Bt.as<float>(), cnt, nd, Tq, Tq_scale),
+                                cnt, nd, ek);
         // Variance contraction of the tile.
         if (want_var && m->L_rows[t] > 0) {
           EpArgs eq;
           eq.part = Qpart.as<double>() + qo[t] + c0;
           eq.part_ld = ns;
-          if (f64)
-            gemm_f64(st, EP_SUMSQ, m->L_rows[t], cnt, Ms, m->L[t].as<double>(), Ms,
-                     Kt.as<double>(), (long long)kl, m->L_lower[t], eq);
-          else
-            gemm_f64(st, EP_SUMSQ, m->L_rows[t], cnt, Ms, m->L[t].as<double>(), Ms,
-                     Kt.as<float>(), (long long)kl, m->L_lower[t], eq);
+          gemm_f64(st, EP_SUMSQ, m->L_rows[t], cnt, Ms, m->L[t].as<double>(), Ms,
+                   Kt.as<double>(), (long long)kl, m->L_lower[t], eq);
         }
         // Gradient contraction of the tile.
         EpArgs eg;
@@ -439,6 +484,7 @@ void predict(fm_model m, fm_structure s, int flags, int form, double *mean_efs,
   bool want_var = (flags & FM_PREDICT_LOCAL) != 0;
   FM_CHECK(!want_var || m->has_L, FM_ERR_STATE,
            "L_inv must be up to date to evaluate uncertainties");
+  ctx->kt_clear(KT_SLICE, KT_GATHER);
   if (s->desc_kind == 3 && s->row_tile > 0)
     return predict_tiled(m, s, flags, form, mean_efs, local_var, local_energy);
   bool f64 = form == FM_FORM_FP64;
@@ -449,13 +495,13 @@ void predict(fm_model m, fm_structure s, int flags, int form, double *mean_efs,
   // Per-species slices of the atom-major w and k matrices (n_s x M_s each)
   // and of the energy / variance partials.
   // The chosen form keeps w as BF16 pairs with padded rows (ws = split_ld).
-  // k is FP32 in the chosen form, with rows padded to 4 values (16 bytes).
+  // k is FP64 in both forms, rows padded to 2 values (16 bytes).
   std::vector<size_t> wo(nos + 1, 0), eo(nos + 1, 0), qo(nos + 1, 0), so(nos + 1, 0),
       ko(nos + 1, 0), kld(nos, 0);
   for (int t = 0; t < nos; t++) {
     size_t ns = s->n_by_type[t], Ms = m->M_by_type[t];
     wo[t + 1] = wo[t] + ns * Ms;
-    kld[t] = f64 ? Ms : (Ms + 3) / 4 * 4;
+    kld[t] = (Ms + 1) / 2 * 2;
     ko[t + 1] = ko[t] + ns * kld[t];
     so[t + 1] = so[t] + ns * (size_t)split_ld((long long)Ms);
     eo[t + 1] = eo[t] + ns * (size_t)std::max(1, gemm_col_tiles(form, (int)Ms));
@@ -470,9 +516,21 @@ void predict(fm_model m, fm_structure s, int flags, int form, double *mean_efs,
     Wl.alloc(pool, std::max<size_t>(1, so[nos] * sizeof(uint16_t)));
   }
   DevBuf Kb;
-  // k in FP64, or in FP32 for the split form (whose k is an FP32 value).
-  size_t ksz = f64 ? sizeof(double) : sizeof(float);
+  size_t ksz = sizeof(double);
   if (want_var) Kb.alloc(pool, std::max<size_t>(1, ko[nos] * ksz));
+  // Chosen form: the descriptor rows' int8 slices, made by the descriptor
+  // pass from the FP64 rows (B2), or here from the FP32 rows one species at
+  // a time (B3).
+  DevBuf Aq, Aq_scale;
+  if (!f64) {
+    model_slices(m, st);
+    if (!s->has_q) {
+      int nmax = 0;
+      for (int t = 0; t < nos; t++) nmax = std::max(nmax, s->n_by_type[t]);
+      Aq.alloc(pool, std::max<size_t>(1, (size_t)kI8Slices * nmax * i8_ld(nd)));
+      Aq_scale.alloc(pool, std::max<size_t>(1, (size_t)nmax * sizeof(double)));
+    }
+  }
   DevBuf G(pool, std::max<size_t>(1, (size_t)n * nd * esz));
   DevBuf Epart(pool, std::max<size_t>(1, eo[nos] * sizeof(double)));
   DevBuf Qpart(pool, std::max<size_t>(1, qo[nos] * sizeof(double)));
@@ -510,18 +568,21 @@ void predict(fm_model m, fm_structure s, int flags, int form, double *mean_efs,
     }
     ek.part = Epart.as<double>() + eo[t];
     ek.part_ld = ns;
-    if (f64)
+    if (f64) {
+      KScope timer(ctx, KT_SGEMM);
       gemm_f64(st, EP_KERNEL, ns, Ms, nd, s->B.as<double>() + (size_t)ro * nd, nd,
                m->X[t].as<double>(), nd, 0, ek);
-    else {
-      long long ldd = split_ld(nd);
-      SplitOperand A, Bx;
-      for (int pc = 0; pc < 2; pc++) A.piece[pc] = s->Bs[pc].as<uint16_t>() + (size_t)ro * ldd;
-      A.ld = ldd;
-      Bx.piece[0] = m->Xs0[t].as<uint16_t>();
-      Bx.piece[1] = m->Xs1[t].as<uint16_t>();
-      Bx.ld = ldd;
-      gemm_split(st, EP_KERNEL, 2, ns, Ms, (int)ldd, A, Bx, ek);
+    } else {
+      SliceOperand A;
+      if (s->has_q) {
+        A.q = s->Bq.as<int8_t>() + (size_t)ro * s->ldq;
+        A.ld = s->ldq;
+        A.slice_stride = (long long)n * s->ldq;
+        A.scale = s->Bq_scale.as<double>() + ro;
+      } else {
+        A = slice_fp32_rows(ctx, st, s->B.as<float>() + (size_t)ro * nd, ns, nd, Aq, Aq_scale);
+      }
+      kernel_contraction_i8(m, st, t, A, ns, nd, ek);
     }
     ctx->launches[ST_SGEMM]++;
   }
@@ -540,6 +601,7 @@ void predict(fm_model m, fm_structure s, int flags, int form, double *mean_efs,
     EpArgs eg;
     eg.W = Grow;
     eg.ldw = nd;
+    KScope timer(ctx, KT_GGEMM);
     if (f64)
       gemm_f64(st, EP_STORE, ns, nd, Ms, W.as<double>() + wo[t], Ms, m->XT[t].as<double>(), Ms,
                0, eg);
@@ -595,12 +657,9 @@ void predict(fm_model m, fm_structure s, int flags, int form, double *mean_efs,
       EpArgs eq;
       eq.part = Qpart.as<double>() + qo[t];
       eq.part_ld = ns;
-      if (f64)
-        gemm_f64(sd, EP_SUMSQ, m->L_rows[t], ns, Ms, m->L[t].as<double>(), Ms,
-                 Kb.as<double>() + ko[t], (long long)kld[t], m->L_lower[t], eq);
-      else
-        gemm_f64(sd, EP_SUMSQ, m->L_rows[t], ns, Ms, m->L[t].as<double>(), Ms,
-                 Kb.as<float>() + ko[t], (long long)kld[t], m->L_lower[t], eq);
+      KScope timer(ctx, KT_VAR, sd);
+      gemm_f64(sd, EP_SUMSQ, m->L_rows[t], ns, Ms, m->L[t].as<double>(), Ms,
+               Kb.as<double>() + ko[t], (long long)kld[t], m->L_lower[t], eq);
       ctx->launches[ST_VAR]++;
     }
     for (int t = 0; t < nos; t++) {

@@ -51,7 +51,10 @@ typedef struct fm_mapped_s *fm_mapped;
 const char *fm_last_error(void);
 
 /* Device context: one HIP stream plus a caching pool of device buffers.
- * No reference counterpart (the reference is a CPU library). */
+ * No reference counterpart (the reference is a CPU library).
+ * fm_context_destroy releases the caller's handle: structures, models and
+ * potentials made on the context stay valid, and the context is deleted
+ * when the last of them is destroyed (or at once if there are none). */
 int fm_context_create(int device, fm_context *out);
 int fm_context_destroy(fm_context ctx);
 int fm_context_synchronize(fm_context ctx);
@@ -246,6 +249,17 @@ int fm_mapped_compute(fm_mapped mp, fm_structure s, double *atom_energy,
 #define FM_N_STAGES 6
 int fm_last_timings(fm_context ctx, double *ms, int32_t *launches);
 
+/* Device-side durations of individual kernels in the last calls on a
+ * context (HIP event pairs around each launch, on the stream it runs on):
+ * the sum over launches in milliseconds and the number of launches, for
+ * n of the kernels [0] B2 descriptor (bond basis, c, B; last descriptor
+ * computation), [1] int8 slicing of descriptor rows (chosen form), [2]
+ * kernel contraction S with its epilogue, [3] gradient contraction G, [4]
+ * variance contraction, [5] adjoint u = dE/dc, [6] bond forces, [7] force
+ * gather ([1]-[7]: last prediction). No reference counterpart. */
+#define FM_N_KERNEL_TIMERS 8
+int fm_last_kernel_timings(fm_context ctx, int n, double *ms, int32_t *launches);
+
 /* Row tiles for B3 structures whose descriptors are too large to keep
  * (B3 at 10^6 atoms is 296 GB in FP32): for B3 descriptors computed on this
  * context afterwards, rows > 0 forces tiles of that many species-sorted rows;
@@ -255,6 +269,54 @@ int fm_last_timings(fm_context ctx, double *ms, int32_t *launches);
  * (results equal the untiled ones bit for bit) and cannot be read back with
  * fm_structure_get_b2. No reference counterpart (b3.cpp keeps every row). */
 int fm_context_set_row_tile(fm_context ctx, int64_t rows);
+
+/* Serving: the batched and serving entry points of the drop-in boundary.
+ * No reference counterpart (the reference evaluates one Structure per call,
+ * sparse_gp.cpp:643-757; this serves the same predictions to many callers).
+ * A server keeps every kernel's model resident on n_workers contexts (HIP
+ * streams, one worker thread each) and batches waiting requests
+ * continuously: a worker takes the oldest requests whose device footprint
+ * fits hbm_budget beside the batches in flight on all workers, evaluates
+ * them as one batch (descriptor calculator k feeds kernel k, as
+ * Structure::descriptors and SparseGP::kernels pair up) and completes them.
+ * A request is sized from its atom count and the neighbours within cutoff
+ * at max_density atoms / A^3 (or its own density if higher); once its batch
+ * is built the measured neighbour count replaces the estimate. Results do
+ * not depend on the other requests of a batch. fm_submit and fm_poll may be
+ * called from any host threads. */
+#define FM_DESCRIPTOR_B2 2
+#define FM_DESCRIPTOR_B3 3
+typedef struct fm_server_s *fm_server;
+typedef struct {
+  int n_kernels;
+  const fm_model_desc *models;          /* one per kernel */
+  const fm_b2_settings *descriptors;    /* one per kernel */
+  const int32_t *descriptor_kinds;      /* FM_DESCRIPTOR_B2 / _B3 per kernel; NULL: B2 */
+  double cutoff;                        /* neighbour-list cutoff */
+  int form;                             /* FM_FORM_FP64 or FM_FORM_CHOSEN */
+  int flags;                            /* FM_PREDICT_MEAN [| FM_PREDICT_LOCAL] */
+  int n_workers;                        /* 1 .. 16 */
+  int64_t hbm_budget;                   /* device bytes of the batches in flight */
+  int32_t max_batch_atoms;              /* atoms of one batch at most; 0: no cap */
+  double max_density;                   /* atoms / A^3 for sizing requests; 0: 0.1 */
+} fm_server_desc;
+int fm_server_create(int device, const fm_server_desc *desc, fm_server *out);
+/* Finishes the queued requests, then stops the workers and frees the server
+ * (results not yet polled are discarded). */
+int fm_server_destroy(fm_server srv);
+/* Queues one periodic structure (inputs are copied; cell 3 x 3 row-major,
+ * positions N x 3). FM_ERR_CAPACITY when the request alone exceeds the
+ * budget. */
+int fm_submit(fm_server srv, const double *cell, int n_atoms, const int32_t *species,
+              const double *positions, int64_t *request);
+/* Waits up to timeout_ms (0: returns at once, < 0: until done) for a
+ * request. *done = 1 once it has finished: its outputs are copied (mean_efs
+ * 1 + 3N + 6; local_var n_kernels x N, each kernel's variances in the
+ * structure's species-sorted order; either may be NULL), the request is
+ * forgotten, and the return value is FM_OK or the error the evaluation
+ * raised. *done = 0: still queued or running, FM_OK. */
+int fm_poll(fm_server srv, int64_t request, int timeout_ms, int *done, double *mean_efs,
+            double *local_var);
 
 #ifdef __cplusplus
 }

@@ -86,7 +86,7 @@ def test_b3_batch_and_chosen_form(C, oracle, system):
             s = C.Structure(*st, 4.6, [system["calc"]], form=form)
             sgp.predict_local_uncertainties(s)
             assert np.array_equal(means[k], s.mean_efs), (form, k)
-            assert np.array_equal(vars_[k], s.local_uncertainties[0]), (form, k)
+            assert np.array_equal(vars_[k][0], s.local_uncertainties[0]), (form, k)
     s64 = C.Structure(*structs[0], 4.6, [system["calc"]])
     s32 = C.Structure(*structs[0], 4.6, [system["calc"]], form=1)
     sgp.predict_mean(s64)

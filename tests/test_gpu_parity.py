@@ -221,8 +221,13 @@ BUDGET_CASES = [[2, 12, 6], [3, 10, 6], [2, 16, 6]]
 
 @pytest.mark.parametrize("settings", BUDGET_CASES)
 def test_chosen_form_error_budget(C, oracle, settings):
-    """FP32 descriptors + split-BF16x3 contraction against the FP64 path on
-    the same model: energy per atom, force and variance errors."""
+    """The chosen form (FP32 descriptors, the kernel contraction on int8
+    slices with exact integer sums, the gradient contraction split-BF16x3,
+    FP64 variance contraction) against the FP64 path on the same model:
+    energy per atom and force (bar 1 meV/A), and sigma: |d sigma| / sigma_f
+    and the relative error of atoms with sigma >= 0.01 sigma_f (bar 1e-3;
+    closer to the sparse set sigma^2 is a cancellation of terms of size
+    sigma_f^2 whose relative error is unbounded in any precision)."""
     from flare_amd.workloads import FCC_BASIS, lattice
     cell, pos = lattice(FCC_BASIS, 4.0, (6, 6, 6))
     rng = np.random.default_rng(8)
@@ -245,11 +250,17 @@ def test_chosen_form_error_budget(C, oracle, settings):
     v32 = s32.local_uncertainties[0]
     sd64 = np.sqrt(np.abs(v64))
     sd32 = np.sqrt(np.abs(v32))
-    sig_rel = np.max(np.abs(sd32 - sd64) / np.maximum(sd64, 1e-12))
+    sig_f = 2.0
+    big = sd64 >= 0.01 * sig_f
+    sig_abs = np.max(np.abs(sd32 - sd64)) / sig_f
+    sig_rel = np.max(np.abs(sd32 - sd64)[big] / sd64[big]) if big.any() else 0.0
     say(f"chosen form: dE {dE * 1e3:.3e} meV/atom, dF {dF * 1e3:.3e} meV/A, "
-        f"sigma rel {sig_rel:.3e}, |F|max {np.abs(s64.mean_efs[1:1 + 3 * n]).max():.3e}")
+        f"|dsigma|/sigma_f {sig_abs:.3e}, sigma rel (sigma >= 0.01 sigma_f, "
+        f"{big.mean():.0%} of atoms) {sig_rel:.3e}")
     assert dF < 1e-3  # 1 meV/A
     assert dE < 1e-3
+    assert sig_abs < 1e-3
+    assert sig_rel < 1e-3
 
 
 @pytest.mark.parametrize("shape", range(7))
