@@ -128,7 +128,9 @@ def kernel_rooflines(r, steps, n, n_by, M_by, d, nrad_h, f64):
       gradient contraction 2 n_s M_s d flops
       variance contraction M_s^2 n_s flops (lower-triangular L_s^-1 times K^T)
       descriptor kernel    n (40 nb + s (nrad H + d) + 16) bytes: neighbour
-                           entries in, c and B rows and norms out
+                           entries in, c and B rows and norms out; the
+                           chosen form also writes the int8 slices of the
+                           row and its scale, n (5 d + 8) bytes
       adjoint dE/dc        n (s (2 d + 2 nrad H) + 16) bytes: G, B, c in, u out
       bond forces          n (nb (40 + 3 s) + s nrad H + 48) bytes
       row slicing          n (4 d + 5 d + 8) bytes
@@ -142,7 +144,8 @@ def kernel_rooflines(r, steps, n, n_by, M_by, d, nrad_h, f64):
                                PEAK["fp64"] if f64 else PEAK_INT8 / I8_PRODUCTS),
         "gradient_contraction": ("mfma", kf, "TFLOP/s", PEAK["fp64"] if f64 else PEAK["bf16"] / 3),
         "variance_contraction": ("mfma", vf, "TFLOP/s", PEAK["fp64"]),
-        "descriptor": ("hbm", n * (40 * nb + s * (nrad_h + d) + 16), "GB/s", HBM_PEAK),
+        "descriptor": ("hbm", n * (40 * nb + s * (nrad_h + d) + 16 + (0 if f64 else 5 * d + 8)),
+                       "GB/s", HBM_PEAK),
         "adjoint_dEdc": ("hbm", n * (s * (2 * d + 2 * nrad_h) + 16), "GB/s", HBM_PEAK),
         "bond_forces": ("hbm", n * (nb * (40 + 3 * s) + s * nrad_h + 48), "GB/s", HBM_PEAK),
         "slice": ("hbm", n * (9 * d + 8), "GB/s", HBM_PEAK),
@@ -170,7 +173,7 @@ def kernel_rooflines(r, steps, n, n_by, M_by, d, nrad_h, f64):
 
 
 def batched_leg(lib, ctx, hip, steps, warmup):
-    """Config 3: 512 structures (16-2048 atoms, 3 species, bulk/slab cells)
+    """512 structures (16-2048 atoms, 3 species, bulk/slab cells)
     in one batch per step; B2 [3, 8, 4] rc 5.5 (d = 1500); 1500 sparse
     environments from the first structures (seed 6); FP64 vs chosen."""
     from flare_amd import _C_flare as CF
@@ -283,7 +286,7 @@ def batched_leg(lib, ctx, hip, steps, warmup):
 
 
 def large_leg(lib, ctx, hip, steps, warmup):
-    """Config 5 (B2 kernel): a 1,048,576-atom three-species BCC alloy
+    """The 1M-atom B2 workload: a 1,048,576-atom three-species BCC alloy
     (a = 3.2 A, 128 x 64 x 64 cells, displacements 0.1 A, seed 8); B2
     [3, 10, 6] rc 5.5 (d = 3255); NDP sigma 2, p 2; 4096 sparse environments
     (seed 9) drawn from a 16^3-cell sample of the same alloy (seed 11; the
@@ -374,7 +377,7 @@ def run_model_forms(lib, ctx, parts, cell, sp_dev, pos_dev, n, form, steps, warm
 
 
 def large_b3_leg(lib, ctx, hip, steps, warmup):
-    """Config 5 as one model: the 1,048,576-atom three-species BCC alloy of
+    """The 1M-atom B2 + B3 workload as one model: the 1,048,576-atom three-species BCC alloy of
     the B2 leg (a = 3.2 A, 128 x 64 x 64 cells, displacements 0.1 A, seed 8)
     with two kernels, NDP sigma 2, p 2 on B2 [3, 10, 6] rc 5.5 (d = 3255)
     and NDP sigma 2, p 2 on B3 [3, 6, 4] rc 5.5 (d = 74,100); each kernel's
@@ -432,7 +435,7 @@ def large_b3_leg(lib, ctx, hip, steps, warmup):
 
 
 def serving_leg(sgp, calc, batched, n_requests=2000, n_fp64=200):
-    """Config 4: Poisson stream of requests (sizes log-uniform 32-8192
+    """Served stream: Poisson stream of requests (sizes log-uniform 32-8192
     atoms, the batched model), served with continuous batching in the
     chosen form at 0.7x the batched throughput (three workers, batches of
     at most 32,768 atoms) after an untimed warm-up stream; FP64 drop-in
@@ -578,10 +581,9 @@ def main():
     ap.add_argument("--workloads", default="all",
                     choices=["single", "batched", "serving", "large", "large_b3", "all"],
                     help="single: the 32k-atom structure only; all: also the batched, "
-                         "served and 1M-atom B2 workloads; batched / large / large_b3: that "
-                         "workload alone (its own JSON line), serving: the batched workload "
-                         "and the served stream at its rate; large_b3 (the 1M-atom alloy "
-                         "with a B2 and a B3 kernel) runs only when named")
+                         "served, 1M-atom B2 and 1M-atom B2 + B3 workloads; batched / "
+                         "large / large_b3: that workload alone (its own JSON line), "
+                         "serving: the batched workload and the served stream at its rate")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a 512 MiB buffer three times (the known-size kernel "
                          "that FETCH_SIZE / WRITE_SIZE counters are scaled by)")
@@ -682,10 +684,11 @@ def main():
         "data": "synthetic",
         "config": {"workload": "32,000-atom 2-species FCC alloy, B2 [2,12,6] rc 6.0 (d=2100), "
                                "2,000 sparse envs, NDP sigma 2 p 2; E/F/stress/local sigma",
-                   "form": "chosen: FP32 descriptors + adjoint; kernel contraction on int8 "
-                           "slices (5 per operand, 15 products, exact int32 sums, FP64 "
-                           "epilogue); gradient contraction split-BF16x3 (FP32 accumulate); "
-                           "FP64 variance contraction",
+                   "form": "chosen: descriptors computed in FP64, kept as FP32 rows for "
+                           "the gradient contraction and adjoint and as int8 slices (5 per "
+                           "operand) for the kernel contraction (15 products, exact int32 "
+                           "sums, FP64 epilogue); gradient contraction split-BF16x3 (FP32 "
+                           "accumulate); FP32 adjoint; FP64 variance contraction",
                    "n_atoms": n, "n_sparse": int(sum(M_by)), "n_descriptors": d},
         "p50_ms": chosen["p50_ms"],
         "p99_ms": chosen["p99_ms"],
@@ -705,6 +708,8 @@ def main():
         out["serving"] = serving_leg(bsgp, bcalc, out["batched"])
         log("1M-atom workload ...")
         out["large_b2"] = large_leg(lib, ctx, hip, 2, 1)
+        log("1M-atom B2 + B3 workload ...")
+        out["large_b3"] = large_b3_leg(lib, ctx, hip, 1, 1)
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
         log(f"cpu baseline on {threads} threads ...")

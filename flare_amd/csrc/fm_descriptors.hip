@@ -2,15 +2,19 @@
 // and its adjoint (the arithmetic of lammps_descriptor.cpp:250-315 and
 // pair_flare.cpp:141-174, driven by dE/dB from the kernel contraction).
 //
-// Descriptor: one wavefront per atom; lane L owns the harmonics
-// lm = L + 64 c. Per neighbour species the lane keeps c[n, lm] for all radial
-// n in registers and accumulates g_n(r) Y_lm(r_hat) over that species' bonds
-// in list order (so each coefficient sums its terms in the reference's
-// order); the coefficients then sit in LDS for the power spectrum
-// B[n1, n2, l] and are also written to HBM for the adjoint.
-// Adjoint: u = dE/dc per atom (lanes over lm), then the bond forces with
-// lanes over bonds, each lane sweeping all (l, m) of its bond by recursion.
-// Dense dc/dr and dB/dr are never formed.
+// Descriptor (k_b2): an atom's bonds are taken in chunks of 32; lanes write
+// the radial values g_n and all harmonics Y_lm of the chunk into LDS tables
+// grouped by neighbour species, and the density coefficients accumulate as
+// c_t += G_t^T Y_t on the matrix cores (16x16x4, FP32 or FP64) over that
+// species' rows only. The power spectrum B_l = C_l C_l^T is a second MFMA
+// product, stored straight to the atom's species-sorted row with |B| and
+// B.B; c goes to HBM for the adjoint. The mixed variant computes all of
+// this in FP64 and writes the FP32 rows and c the adjoint reads plus the
+// int8 slices of the FP64 row that the exact kernel contraction consumes.
+// Adjoint: u = dE/dc per atom (w = dE/dB and c staged in LDS, u_l = W_l C_l
+// on the matrix cores), then the bond forces with lanes over bonds, each
+// lane sweeping all (l, m) of its bond by recursion. Dense dc/dr and dB/dr
+// are never formed.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

@@ -13,12 +13,15 @@
 //             variance term |L_inv k|^2 (sparse_gp.cpp:121-131); A is lower
 //             triangular so each row tile stops at its last nonzero column.
 //
-// FP64 form: v_mfma_f64_16x16x4f64, 64 x 64 tiles. Chosen form: split-BF16
-// x3 on v_mfma_f32_16x16x32_bf16 (a = a_hi + a_lo, ab ~ a_hi b_hi +
-// a_hi b_lo + a_lo b_hi, FP32 accumulation), 128 x 128 tiles, on operands
-// split once where they are produced; the three products of a k-step run
+// FP64 form: v_mfma_f64_16x16x4f64, 64 x 64 tiles, for all three. Chosen
+// form: EpKernel is exact on int8 MFMA (v_mfma_i32_32x32x32_i8) over five
+// power-of-two-scaled slices of each operand row, int32 accumulation per
+// slice-pair diagonal and an FP64 epilogue (k_gemm_i8); EpStore is
+// split-BF16 x3 on v_mfma_f32_16x16x32_bf16 (a = a_hi + a_lo, ab ~ a_hi b_hi
+// + a_hi b_lo + a_lo b_hi, FP32 accumulation), 128 x 128 tiles, on operands
+// split once where they are produced, the three products of a k-step run
 // over all 16 accumulators of a wave before the next, so consecutive MFMAs
-// are independent.
+// are independent; EpSumSq stays FP64.
 #include <hip/hip_runtime.h>
 
 #include <cstdlib>
@@ -925,17 +928,6 @@ void gemm_f64(hipStream_t st, int ep, int M, int N, int K, const double *A, long
     FM_HIP(hipGetLastError());
   };
   launch(tuned(0, ep, M, N, K, st, kF64Shapes, 4, launch));
-}
-
-void gemm_f64(hipStream_t st, int ep, int M, int N, int K, const double *A, long long lda,
-              const float *B, long long ldb, int lower_tri, const EpArgs &e) {
-  if (M <= 0 || N <= 0) return;
-  FM_CHECK(ep == EP_SUMSQ, FM_ERR_INVALID, "FP32 B operand: variance contraction only");
-  auto launch = [&](int i) {
-    f64_shape<EP_SUMSQ>(i, st, M, N, K, A, lda, B, ldb, lower_tri, e);
-    FM_HIP(hipGetLastError());
-  };
-  launch(tuned(2, ep, M, N, K, st, kF64Shapes, 4, launch));
 }
 
 void gemm_split(hipStream_t st, int ep, int np, int M, int N, int Kpad, const SplitOperand &A,

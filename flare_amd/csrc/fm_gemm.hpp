@@ -14,14 +14,13 @@ struct EpArgs {
   double sig2 = 1, power = 2;
   const double *row_norm = nullptr;  // |B_i| per row (atom) of the block
   const double *alpha = nullptr;     // alpha_j per column (sparse env)
-  // Optional k matrix (rows x ldk) in the accumulator's type: FP64, or FP32
-  // in the split form (whose k is an FP32 value, stored exactly).
+  // Optional k matrix (rows x ldk), FP64.
   void *K = nullptr;
   long long ldk = 0;
-  // EP_KERNEL writes w, EP_STORE writes C, into W (rows x ldw). In the
-  // split-BF16 contraction EP_KERNEL instead writes w as the BF16 pair
-  // (hi, lo) of its FP32 value into Wh / Wl (rows x ldw), zero in the
-  // padding columns [N, ldw).
+  // EP_KERNEL writes w, EP_STORE writes C, into W (rows x ldw). In the int8
+  // contraction EP_KERNEL instead writes w as the BF16 pair (hi, lo) of its
+  // FP32 value into Wh / Wl (rows x ldw), zero in the padding columns
+  // [N, ldw), the operand of the split-BF16 gradient contraction.
   void *W = nullptr;
   void *Wh = nullptr, *Wl = nullptr;
   long long ldw = 0;
@@ -46,10 +45,6 @@ int gemm_row_tiles(int form, int M);
 
 void gemm_f64(hipStream_t st, int ep, int M, int N, int K, const double *A, long long lda,
               const double *B, long long ldb, int lower_tri, const EpArgs &e);
-// The same FP64 contraction with B stored in FP32 (widened to FP64 on load);
-// EP_SUMSQ only (the variance contraction on the split form's k matrix).
-void gemm_f64(hipStream_t st, int ep, int M, int N, int K, const double *A, long long lda,
-              const float *B, long long ldb, int lower_tri, const EpArgs &e);
 // Split-BF16 x3 contraction (EP_STORE: the gradient contraction G = W X_hat
 // of the chosen form) on operands stored as two BF16 pieces of FP32
 // values, x = x_0 + x_1 with x_0 = bf16(x), x_1 = bf16(x - x_0); it sums

@@ -5,7 +5,7 @@ tests (there are no datasets in this environment).
                   displacements sigma = 0.05 A, rng seed 1000 + frame.
   fcc_alloy_32k   32,000-atom FCC (a = 4.0 A, 20^3 cells), two species 50/50
                   by random permutation (seed 2), displacements 0.08 A.
-  c1_training_frames  20 of the 108-atom frames labelled by Lennard-Jones
+  lj_training_frames  20 of the 108-atom frames labelled by Lennard-Jones
                   (eps 0.4 eV, sigma 2.6 A, cutoff 7.8 A, shifted to 0).
 """
 import numpy as np
@@ -24,7 +24,7 @@ def lattice(basis, a, reps):
 
 
 def fcc_al_108(frame=0, sigma=0.05):
-    """Config 1 frames: seed 1000 + frame."""
+    """108-atom FCC aluminium frames: seed 1000 + frame."""
     cell, pos = lattice(FCC_BASIS, 4.05, (3, 3, 3))
     rng = np.random.default_rng(1000 + frame)
     pos = pos + rng.normal(scale=sigma, size=pos.shape)
@@ -32,7 +32,7 @@ def fcc_al_108(frame=0, sigma=0.05):
 
 
 def fcc_alloy_32k(sigma=0.08):
-    """Config 2 structure: 32,000 atoms, species by permutation (seed 2)."""
+    """32,000-atom two-species FCC alloy, species by permutation (seed 2)."""
     cell, pos = lattice(FCC_BASIS, 4.0, (20, 20, 20))
     n = len(pos)
     rng = np.random.default_rng(2)
@@ -81,7 +81,7 @@ def lennard_jones(cell, positions, eps=0.4, sig=2.6, rc=7.8):
     return E, F, S
 
 
-def c1_training_frames(n_frames=20):
+def lj_training_frames(n_frames=20):
     """The 108-atom FCC training frames (seeds 1000 + k) with their LJ
     energy, forces and stress: [(cell, species, positions, E, F, S)]."""
     out = []
@@ -93,7 +93,7 @@ def c1_training_frames(n_frames=20):
 
 
 def batch_structures(n_structures=512, seed=5, n_min=16, n_max=2048, nos=3):
-    """Config 3 structures: atom counts log-uniform over [n_min, n_max];
+    """Batch of structures of uneven sizes: atom counts log-uniform over [n_min, n_max];
     species uniform over nos; density 0.05-0.09 atoms/A^3; a 50% coin flip
     between bulk (random triclinic cell) and slab (10-15 A of vacuum along
     the third lattice vector). Sites come from an FCC lattice scaled to the

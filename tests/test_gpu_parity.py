@@ -127,8 +127,8 @@ def oracle_model(oracle, sgp):
 
 
 PRED_CASES = [
-    # c1 shape: 108-atom FCC, B2 [1, 8, 3], rc 5, NDP sigma 2 p 2, sparse from other frames.
-    ("c1", lambda: fcc(4.05, (3, 3, 3), 0.05, 1100), (1, 8, 3), np.full((1, 1), 5.0), 2.0, 2.0),
+    # 108-atom model: 108-atom FCC, B2 [1, 8, 3], rc 5, NDP sigma 2 p 2, sparse from other frames.
+    ("al108", lambda: fcc(4.05, (3, 3, 3), 0.05, 1100), (1, 8, 3), np.full((1, 1), 5.0), 2.0, 2.0),
     ("two_species_asym_p1", lambda: random_cell(30, 12, nos=2, scale=7.5, skew=0.5), (2, 5, 4),
      np.array([[4.5, 4.0], [4.2, 3.9]]), 1.5, 1.0),
     ("three_species_p3", lambda: random_cell(36, 16, nos=3, scale=8.0, skew=0.6), (3, 4, 4),
@@ -143,8 +143,8 @@ def test_predict_fp64(C, oracle, name, make, settings, cut, sigma, power):
     rc = float(cut.max())
     calc = C.B2("chebyshev", "quadratic", [0.0, rc], [], list(settings), cut)
     # Sparse set from a second seeded structure of the same kind.
-    cell2, species2, pos2 = make() if name != "c1" else fcc(4.05, (3, 3, 3), 0.05, 1000)
-    if name != "c1":
+    cell2, species2, pos2 = make() if name != "al108" else fcc(4.05, (3, 3, 3), 0.05, 1000)
+    if name != "al108":
         pos2 = pos2 + np.random.default_rng(99).normal(scale=0.1, size=pos2.shape)
     sp = gpu_structure(C, cell2, species2, pos2, rc, calc)
     picks = [list(range(0, n, 2)) for n in sp.descriptors[0].n_clusters_by_type]
