@@ -38,7 +38,7 @@ EXPORTS = [
     "fm_structure_compute_b2", "fm_structure_b2_sizes", "fm_structure_get_b2",
     "fm_model_create", "fm_model_destroy", "fm_predict", "fm_last_timings",
     "fm_model_mapping_coefficients", "fm_mapped_create", "fm_mapped_read_file",
-    "fm_mapped_info", "fm_mapped_destroy", "fm_mapped_compute",
+    "fm_mapped_info", "fm_mapped_destroy", "fm_mapped_compute", "fm_mapped_compute_noalloc",
     "fm_batch_create", "fm_batch_create_device", "fm_batch_info",
     "fm_structure_get_b2_dervs", "fm_envs_struc", "fm_self_kernel_struc", "fm_predict_dense",
     "fm_structure_compute_b3", "fm_context_set_row_tile",
@@ -126,6 +126,7 @@ def load():
         L.fm_mapped_info.argtypes = [vp] * 8
         L.fm_mapped_destroy.argtypes = [vp]
         L.fm_mapped_compute.argtypes = [vp, vp, vp, vp, vp]
+        L.fm_mapped_compute_noalloc.argtypes = [vp, vp, vp, vp, vp]
         L.fm_batch_create.argtypes = [vp, ip, vp, vp, vp, vp, dp, pvp]
         L.fm_batch_create_device.argtypes = [vp, ip, vp, vp, vp, vp, dp, pvp]
         L.fm_batch_info.argtypes = [vp, ip, C.POINTER(C.c_int), C.POINTER(C.c_int),

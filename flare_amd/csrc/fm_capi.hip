@@ -196,6 +196,7 @@ void fm_context_s::kt_clear(int first, int last) {
   for (int k = first; k <= last; k++) kt[k].used = 0;
 }
 void fm_context_s::kt_begin(int k, hipStream_t on) {
+  if (!kt_on) return;
   KTimes &t = kt[k];
   while ((int)t.ev.size() < 2 * (t.used + 1)) {
     hipEvent_t e;
@@ -205,6 +206,7 @@ void fm_context_s::kt_begin(int k, hipStream_t on) {
   (void)hipEventRecord(t.ev[2 * t.used], on ? on : stream);
 }
 void fm_context_s::kt_end(int k, hipStream_t on) {
+  if (!kt_on) return;
   KTimes &t = kt[k];
   if ((int)t.ev.size() < 2 * (t.used + 1)) return;
   (void)hipEventRecord(t.ev[2 * t.used + 1], on ? on : stream);
@@ -219,6 +221,7 @@ int fm_context_create(int device, fm_context *out) {
   return guarded([&] {
     FM_CHECK(out, FM_ERR_INVALID, "null output handle");
     FM_HIP(hipSetDevice(device));
+    fm::scratch_setup(device);
     auto *c = new fm_context_s();
     c->device = device;
     FM_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -993,6 +996,16 @@ int fm_mapped_compute(fm_mapped mp, fm_structure s, double *atom_energy, double 
     FM_CHECK(mp->ctx == s->ctx, FM_ERR_INVALID, "potential and structure belong to different contexts");
     FM_HIP(hipSetDevice(s->ctx->device));
     fm::mapped_compute(mp, s, atom_energy, forces, virial);
+  });
+}
+
+int fm_mapped_compute_noalloc(fm_mapped mp, fm_structure s, double *atom_energy,
+                              double *forces, double *virial) {
+  return guarded([&] {
+    FM_CHECK(mp && s, FM_ERR_INVALID, "null argument");
+    FM_CHECK(mp->ctx == s->ctx, FM_ERR_INVALID, "potential and structure belong to different contexts");
+    FM_HIP(hipSetDevice(s->ctx->device));
+    fm::mapped_compute_noalloc(mp, s, atom_energy, forces, virial);
   });
 }
 

@@ -994,7 +994,7 @@ void launch_b2(fm_structure s, bool want_b2) {
 
 void launch_adjoint(fm_structure s, const void *G, int64_t g_ld, void *fbond,
                     double *virial_atom, int project, double empty_norm, double empty_nn,
-                    const double *energy, double power) {
+                    const double *energy, double power, void *u_buf) {
   fm_context ctx = s->ctx;
   hipStream_t st = ctx->stream;
   B2Args a = make_args(s);
@@ -1004,7 +1004,11 @@ void launch_adjoint(fm_structure s, const void *G, int64_t g_ld, void *fbond,
   bool f64 = s->form == FM_FORM_FP64;
   size_t esz = f64 ? sizeof(double) : sizeof(float);
   size_t nc = (size_t)a.nrad * a.H;
-  DevBuf U(&ctx->pool, std::max<size_t>(1, (size_t)s->n * nc * esz));
+  DevBuf U;
+  if (!u_buf) {
+    U.alloc(&ctx->pool, adjoint_u_bytes(s));
+    u_buf = U.as<void>();
+  }
   AdjArgs aa{project, empty_norm, empty_nn, energy, power};
   // dE/dc (B2): c and w rows in LDS per atom (each padded to 4 values).
   size_t per1 = ((size_t)pad4((int)nc) + (size_t)pad4(a.nd)) * esz;
@@ -1012,7 +1016,7 @@ void launch_adjoint(fm_structure s, const void *G, int64_t g_ld, void *fbond,
   int nblk1 = (s->n + apb1 - 1) / apb1;
   if (s->n == 0) return;
   if (s->desc_kind == 3)
-    launch_dEdc_b3(s, G, g_ld, project, empty_norm, empty_nn, U.as<void>());
+    launch_dEdc_b3(s, G, g_ld, project, empty_norm, empty_nn, u_buf);
   // FP32 projection from the rows in one pass over G and B where LDS
   // already holds the CU to two blocks (the one-pass registers cost no
   // occupancy there; with three or more blocks they would).
@@ -1026,21 +1030,27 @@ void launch_adjoint(fm_structure s, const void *G, int64_t g_ld, void *fbond,
         hipLaunchKernelGGL((k_dEdc<double, LC>), dim3(nblk1), dim3(128 * apb1), apb1 * per1, st,
                            a, s->B.as<double>(), s->norm.as<double>(), s->nn.as<double>(),
                            (const double *)G, (long long)g_ld, aa, s->C.as<double>(),
-                           U.as<double>());
+                           (double *)u_buf);
       else if (one_pass)
         hipLaunchKernelGGL((k_dEdc<float, LC, true>), dim3(nblk1), dim3(128 * apb1),
                            apb1 * per1, st, a, s->B.as<float>(), s->norm.as<double>(),
                            s->nn.as<double>(), (const float *)G, (long long)g_ld, aa,
-                           s->C.as<float>(), U.as<float>());
+                           s->C.as<float>(), (float *)u_buf);
       else
         hipLaunchKernelGGL((k_dEdc<float, LC>), dim3(nblk1), dim3(128 * apb1), apb1 * per1, st,
                            a, s->B.as<float>(), s->norm.as<double>(), s->nn.as<double>(),
                            (const float *)G, (long long)g_ld, aa, s->C.as<float>(),
-                           U.as<float>());
+                           (float *)u_buf);
     });
   }
   FM_HIP(hipGetLastError());
-  launch_bond_forces(s, U.as<void>(), fbond, virial_atom, nullptr, s->n, empty_norm, empty_nn);
+  launch_bond_forces(s, u_buf, fbond, virial_atom, nullptr, s->n, empty_norm, empty_nn);
+}
+
+size_t adjoint_u_bytes(fm_structure s) {
+  B2Args a = make_args(s);
+  size_t esz = s->form == FM_FORM_FP64 ? sizeof(double) : sizeof(float);
+  return std::max<size_t>(1, (size_t)s->n * a.nrad * a.H * esz);
 }
 
 void launch_bond_forces(fm_structure s, const void *U, void *fbond, double *virial_atom,

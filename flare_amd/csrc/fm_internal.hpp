@@ -153,6 +153,9 @@ struct fm_context_s {
     int used = 0;
   };
   KTimes kt[fm::KT_COUNT];
+  // Off while a call that may not create events runs (kt_begin / kt_end do
+  // nothing then).
+  bool kt_on = true;
   void kt_clear(int first, int last);
   void kt_begin(int k, hipStream_t on = nullptr);
   void kt_end(int k, hipStream_t on = nullptr);
@@ -282,6 +285,14 @@ namespace fm {
 void mapping_coefficients(fm_model m, std::vector<double> &beta, int64_t &beta_size);
 void mapped_compute(fm_mapped mp, fm_structure s, double *atom_energy, double *forces,
                     double *virial);
+// The same evaluation with every working buffer carved from the library's
+// static device scratch (kNoAllocScratchBytes per GPU); no allocation.
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+void mapped_compute_noalloc(fm_mapped mp, fm_structure s, double *atom_energy, double *forces,
+                            double *virial);
+// Looks up the scratch array's device address on the current device (once
+// per device; called when a context is created).
+void scratch_setup(int device);
 void mapped_upload(fm_mapped mp, const double *beta);
 void mapped_parse(const std::string &path, fm_mapped mp, std::vector<double> &beta);
 // Launchers (fm_neighbors.hip)
@@ -320,7 +331,9 @@ void launch_bond_forces(fm_structure s, const void *U, void *fbond, double *viri
 // when given, else from a pass over the G and B rows.
 void launch_adjoint(fm_structure s, const void *G, int64_t g_ld, void *fbond,
                     double *virial_atom, int project, double empty_norm, double empty_nn,
-                    const double *energy = nullptr, double power = 0);
+                    const double *energy = nullptr, double power = 0, void *u_buf = nullptr);
+// Bytes of the u = dE/dc buffer launch_adjoint needs (u_buf).
+size_t adjoint_u_bytes(fm_structure s);
 // fm_predict.cpp
 void predict(fm_model m, fm_structure s, int flags, int form, double *mean_efs,
              double *local_var, double *local_energy);

@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <mutex>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -160,11 +161,19 @@ __global__ void k_mapped_sum(int n_struct, const int *struct_chunk_off, const do
   for (int q = 0; q < 6; q++) vir[6 * (size_t)k + q] = acc[q];
 }
 
-void mapped_compute(fm_mapped mp, fm_structure s, double *atom_energy, double *forces,
-                    double *virial) {
-  fm_context ctx = s->ctx;
-  hipStream_t st = ctx->stream;
-  Pool *pool = &ctx->pool;
+__global__ void k_zero(double *x, long long n) {
+  long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i < n) x[i] = 0;
+}
+
+// Working buffers of one evaluation (device memory, at least the sizes
+// mapped_layout gives).
+struct MappedBufs {
+  double *G, *E_row, *P, *fbond, *vir, *F, *Ea, *vsum, *blk;
+  void *U;
+};
+
+static void mapped_checks(fm_mapped mp, fm_structure s) {
   FM_CHECK(s->has_b2 && s->form == FM_FORM_FP64, FM_ERR_STATE,
            "the structure needs FP64 B2 descriptors for the mapped potential");
   FM_CHECK(s->desc_kind == 2, FM_ERR_INVALID, "pair_style flare evaluates the B2 descriptor");
@@ -174,58 +183,145 @@ void mapped_compute(fm_mapped mp, fm_structure s, double *atom_energy, double *f
   for (size_t k = 0; k < mp->cutoffs.size(); k++)
     FM_CHECK(s->cutoffs[k] == mp->cutoffs[k], FM_ERR_INVALID,
              "structure cutoffs do not match the potential's cutoffs");
-  int n = s->n, d = s->nd;
-  DevBuf G(pool, std::max<size_t>(1, (size_t)n * d * sizeof(double)));
-  DevBuf E_row(pool, std::max<size_t>(1, (size_t)n * sizeof(double)));
-  size_t maxns = 0;
+}
+
+// Byte sizes of the buffers, in MappedBufs order.
+static void mapped_layout(fm_mapped mp, fm_structure s, size_t sz[10]) {
+  size_t n = s->n, d = s->nd, maxns = 0;
   for (int t = 0; t < s->nos; t++) maxns = std::max(maxns, (size_t)s->n_by_type[t]);
-  DevBuf P;
-  if (mp->power == 2) P.alloc(pool, std::max<size_t>(1, maxns * d * sizeof(double)));
+  size_t w = sizeof(double);
+  sz[0] = n * d * w;                                // G
+  sz[1] = n * w;                                    // E_row
+  sz[2] = mp->power == 2 ? maxns * d * w : 0;       // P
+  sz[3] = 3 * (size_t)s->n_bonds * w;               // fbond
+  sz[4] = 6 * n * w;                                // vir
+  sz[5] = 3 * n * w;                                // F
+  sz[6] = n * w;                                    // Ea
+  sz[7] = 6 * (size_t)s->n_struct * w;              // vsum
+  sz[8] = 6 * (size_t)s->n_chunks * w;              // blk
+  sz[9] = adjoint_u_bytes(s);                       // U
+  for (int k = 0; k < 10; k++) sz[k] = std::max<size_t>(sz[k], 1);
+}
+
+// pair_flare.cpp:68-184 on the given buffers; outputs (host or device
+// memory, any may be null) are copied on the context stream, then the
+// stream is synchronised.
+static void mapped_run(fm_mapped mp, fm_structure s, const MappedBufs &b, double *atom_energy,
+                       double *forces, double *virial) {
+  hipStream_t st = s->ctx->stream;
+  int n = s->n, d = s->nd;
   for (int t = 0; t < s->nos; t++) {
     int ns = s->n_by_type[t], ro = s->type_off[t];
     if (ns == 0) continue;
     const double *Bs = s->B.as<double>() + (size_t)ro * d;
     if (mp->power == 2) {
       EpArgs e;
-      e.W = P.as<void>();
+      e.W = b.P;
       e.ldw = d;
       gemm_f64(st, EP_STORE, ns, d, d, Bs, d, mp->beta[t].as<double>(), d, 0, e);
     }
     hipLaunchKernelGGL(k_mapped_energy, dim3((ns + 3) / 4), dim3(256), 0, st, ns, ro, d,
-                       s->B.as<double>(), mp->power == 2 ? P.as<double>() : nullptr,
+                       s->B.as<double>(), mp->power == 2 ? b.P : nullptr,
                        mp->beta[t].as<double>(), mp->power, mp->normalized,
-                       s->norm.as<double>(), s->nn.as<double>(), E_row.as<double>(),
-                       G.as<double>());
+                       s->norm.as<double>(), s->nn.as<double>(), b.E_row, b.G);
   }
-  DevBuf fbond(pool, std::max<size_t>(1, (size_t)3 * s->n_bonds * sizeof(double)));
-  DevBuf vir(pool, std::max<size_t>(1, (size_t)6 * n * sizeof(double)));
-  DevBuf F(pool, std::max<size_t>(1, (size_t)3 * n * sizeof(double)));
-  DevBuf Ea(pool, std::max<size_t>(1, (size_t)n * sizeof(double)));
-  DevBuf vsum(pool, 6 * (size_t)s->n_struct * sizeof(double));
-  FM_HIP(hipMemsetAsync(vsum.as<double>(), 0, 6 * (size_t)s->n_struct * sizeof(double), st));
+  long long nv = 6 * (long long)s->n_struct;
+  hipLaunchKernelGGL(k_zero, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, st, b.vsum, nv);
   if (n > 0) {
-    launch_adjoint(s, G.as<void>(), d, fbond.as<void>(), vir.as<double>(),
-                   mp->normalized ? 1 : 0, -1.0, 1e-8);
-    DevBuf blk(pool, (size_t)s->n_chunks * 6 * sizeof(double));
+    launch_adjoint(s, b.G, d, b.fbond, b.vir, mp->normalized ? 1 : 0, -1.0, 1e-8, nullptr, 0,
+                   b.U);
     hipLaunchKernelGGL(k_mapped_gather, dim3(s->n_chunks), dim3(256), 0, st,
                        s->d_chunk_begin.as<int>(), s->d_chunk_end.as<int>(), s->bond_off.as<int>(),
-                       s->in_off.as<int>(), s->in_bond.as<int>(), fbond.as<double>(),
-                       s->row_of_atom.as<int>(), E_row.as<double>(), vir.as<double>(),
-                       F.as<double>(), Ea.as<double>(), blk.as<double>());
+                       s->in_off.as<int>(), s->in_bond.as<int>(), b.fbond,
+                       s->row_of_atom.as<int>(), b.E_row, b.vir, b.F, b.Ea, b.blk);
     hipLaunchKernelGGL(k_mapped_sum, dim3((s->n_struct + 63) / 64), dim3(64), 0, st, s->n_struct,
-                       s->d_struct_chunk_off.as<int>(), blk.as<double>(), vsum.as<double>());
-    FM_HIP(hipGetLastError());
-    if (atom_energy)
-      FM_HIP(hipMemcpyAsync(atom_energy, Ea.as<double>(), n * sizeof(double),
-                            hipMemcpyDeviceToHost, st));
-    if (forces)
-      FM_HIP(hipMemcpyAsync(forces, F.as<double>(), 3 * (size_t)n * sizeof(double),
-                            hipMemcpyDeviceToHost, st));
+                       s->d_struct_chunk_off.as<int>(), b.blk, b.vsum);
   }
+  FM_HIP(hipGetLastError());
+  if (n > 0 && atom_energy)
+    FM_HIP(hipMemcpyAsync(atom_energy, b.Ea, n * sizeof(double), hipMemcpyDefault, st));
+  if (n > 0 && forces)
+    FM_HIP(hipMemcpyAsync(forces, b.F, 3 * (size_t)n * sizeof(double), hipMemcpyDefault, st));
   if (virial)
-    FM_HIP(hipMemcpyAsync(virial, vsum.as<double>(), 6 * (size_t)s->n_struct * sizeof(double),
-                          hipMemcpyDeviceToHost, st));
+    FM_HIP(hipMemcpyAsync(virial, b.vsum, 6 * (size_t)s->n_struct * sizeof(double),
+                          hipMemcpyDefault, st));
   FM_HIP(hipStreamSynchronize(st));
+}
+
+void mapped_compute(fm_mapped mp, fm_structure s, double *atom_energy, double *forces,
+                    double *virial) {
+  mapped_checks(mp, s);
+  Pool *pool = &s->ctx->pool;
+  size_t sz[10];
+  mapped_layout(mp, s, sz);
+  DevBuf buf[10];
+  for (int k = 0; k < 10; k++) buf[k].alloc(pool, sz[k]);
+  MappedBufs b{buf[0].as<double>(), buf[1].as<double>(), buf[2].as<double>(),
+               buf[3].as<double>(), buf[4].as<double>(), buf[5].as<double>(),
+               buf[6].as<double>(), buf[7].as<double>(), buf[8].as<double>(),
+               buf[9].as<void>()};
+  mapped_run(mp, s, b, atom_energy, forces, virial);
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free evaluation. The working buffers live in one static device
+// array per GPU (kNoAllocScratchBytes, held until the process ends); host
+// code reaches it only through the address hipGetSymbolAddress returns on
+// each device, never through the array's name. Calls on one GPU share the
+// array, so calls that overlap on one GPU race.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+static std::mutex g_scratch_mu;
+static std::vector<unsigned char *> g_scratch;  // per device; null until set up
+
+void scratch_setup(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  FM_CHECK(device >= 0, FM_ERR_INVALID, "negative device index");
+  if ((size_t)device < g_scratch.size() && g_scratch[device]) return;
+  void *p = nullptr;
+  hipError_t rc = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  FM_CHECK(rc == hipSuccess && p, FM_ERR_HIP,
+           std::string("hipGetSymbolAddress of the scratch array failed: ") +
+               hipGetErrorString(rc));
+  if ((size_t)device >= g_scratch.size()) g_scratch.resize(device + 1, nullptr);
+  g_scratch[device] = static_cast<unsigned char *>(p);
+}
+
+static unsigned char *scratch_ptr(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  FM_CHECK(device >= 0 && (size_t)device < g_scratch.size() && g_scratch[device], FM_ERR_STATE,
+           "the scratch array of this device was not set up (create a context on it first)");
+  return g_scratch[device];
+}
+
+void mapped_compute_noalloc(fm_mapped mp, fm_structure s, double *atom_energy, double *forces,
+                            double *virial) {
+  mapped_checks(mp, s);
+  size_t sz[10], off[10], total = 0;
+  mapped_layout(mp, s, sz);
+  for (int k = 0; k < 10; k++) {
+    off[k] = total;
+    total += (sz[k] + 255) / 256 * 256;
+  }
+  FM_CHECK(total <= kNoAllocScratchBytes, FM_ERR_CAPACITY,
+           "the structure needs " + std::to_string(total) +
+               " bytes of working memory; the allocation-free entry point holds " +
+               std::to_string(kNoAllocScratchBytes) + " (use fm_mapped_compute)");
+  unsigned char *base = scratch_ptr(s->ctx->device);
+  auto at = [&](int k) { return static_cast<void *>(base + off[k]); };
+  MappedBufs b{(double *)at(0), (double *)at(1), (double *)at(2), (double *)at(3),
+               (double *)at(4), (double *)at(5), (double *)at(6), (double *)at(7),
+               (double *)at(8), at(9)};
+  fm_context ctx = s->ctx;
+  ctx->kt_on = false;  // the timers would create events
+  try {
+    mapped_run(mp, s, b, atom_energy, forces, virial);
+  } catch (...) {
+    ctx->kt_on = true;
+    throw;
+  }
+  ctx->kt_on = true;
 }
 
 // Builds the device beta matrices: power 2 expands the upper triangle into a

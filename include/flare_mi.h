@@ -239,6 +239,19 @@ int fm_mapped_destroy(fm_mapped mp);
  * Any output may be NULL. */
 int fm_mapped_compute(fm_mapped mp, fm_structure s, double *atom_energy,
                       double *forces, double *virial);
+/* fm_mapped_compute without any allocation: no reference counterpart; it is
+ * for callers that cannot allocate device memory (an MD loop that holds its
+ * structures, code run where hipMalloc is forbidden). Results equal
+ * fm_mapped_compute's bit for bit. The working buffers come from one static
+ * device array of FM_NOALLOC_SCRATCH_BYTES per GPU (64 MiB, held until the
+ * process ends); a structure that needs more returns FM_ERR_CAPACITY.
+ * Outputs may be host or device memory; the call returns when they are
+ * written. Every call on a GPU shares that GPU's array: calls that overlap on
+ * one GPU (from several threads or contexts) race. Kernel timers are not
+ * recorded by this call. */
+#define FM_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int fm_mapped_compute_noalloc(fm_mapped mp, fm_structure s, double *atom_energy,
+                              double *forces, double *virial);
 
 /* Device-side durations (HIP events on the context stream) of the stages
  * last run on a context, in milliseconds, and the number of kernel launches
