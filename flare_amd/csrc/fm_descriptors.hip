@@ -283,9 +283,10 @@ struct SplitOut {
 // see k_slice_rows in fm_gemm.hip for the slicing).
 struct MixedOut {
   float *B, *C;
-  int8_t *q;
+  int8_t *q;  // tiled slice layout, kI8TileA rows per tile
   double *scale;
-  long long ldq, slice_stride;
+  long long nk;        // k steps of kI8Pad (the padded reduction length / kI8Pad)
+  const int *shift;    // per species: slice row = species-sorted row + shift
 };
 
 template <typename T, int LC, int CH, bool MIXED = false>
@@ -393,8 +394,9 @@ __global__ __launch_bounds__(64 * kAtomsPerBlock) void k_b2(B2Args a, T *B, doub
     if (mx > 0) frexp(mx, &E);  // mx = m 2^E, m in [0.5, 1)
     double inv = ldexp(1.0, -E);
     if (lane == 0) mo.scale[row] = ldexp(1.0, E);
-    int8_t *q = mo.q + (size_t)row * mo.ldq;
-    for (int k0 = 4 * lane; k0 < mo.ldq; k0 += 256) {
+    long long qrow = row + mo.shift[a.species[i]];
+    long long Kq = mo.nk * kI8Pad;
+    for (int k0 = 4 * lane; k0 < Kq; k0 += 256) {
       int packed[kI8Slices];
 #pragma unroll
       for (int p = 0; p < kI8Slices; p++) packed[p] = 0;
@@ -411,7 +413,8 @@ __global__ __launch_bounds__(64 * kAtomsPerBlock) void k_b2(B2Args a, T *B, doub
       }
 #pragma unroll
       for (int p = 0; p < kI8Slices; p++)
-        *reinterpret_cast<int *>(q + p * mo.slice_stride + k0) = packed[p];
+        *reinterpret_cast<int *>(mo.q + i8_tile_offset(kI8TileA, mo.nk, qrow, k0, p)) =
+            packed[p];
     }
   }
   s2 = wave_sum(s2);
@@ -941,14 +944,29 @@ void launch_b2(fm_structure s, bool want_b2) {
   s->C.alloc(&ctx->pool, std::max<size_t>(1, (size_t)s->n * a.nrad * a.H * esz));
   MixedOut mo{};
   if (mixed) {
+    // Each species' rows start a row tile of the slices.
     s->ldq = i8_ld(a.nd);
-    s->Bq.alloc(&ctx->pool, std::max<size_t>(1, (size_t)kI8Slices * s->n * s->ldq));
+    s->q_tile_row.assign(s->nos, 0);
+    std::vector<int> &shift = s->q_shift_host;  // kept with the structure for the async copy
+    shift.assign(s->nos, 0);
+    long long qrows = 0;
+    for (int t = 0; t < s->nos; t++) {
+      s->q_tile_row[t] = qrows;
+      shift[t] = (int)(qrows - s->type_off[t]);
+      qrows += (s->n_by_type[t] + kI8TileA - 1) / kI8TileA * kI8TileA;
+    }
+    FM_CHECK(qrows < (1LL << 31), FM_ERR_CAPACITY, "too many descriptor rows for the slices");
+    s->Bq.alloc(&ctx->pool, std::max<size_t>(1, i8_tiled_bytes(qrows, kI8TileA, s->ldq)));
     s->Bq_scale.alloc(&ctx->pool, std::max<size_t>(1, (size_t)s->n * sizeof(double)));
+    s->q_shift.alloc(&ctx->pool, (size_t)s->nos * sizeof(int));
+    FM_HIP(hipMemcpyAsync(s->q_shift.as<int>(), shift.data(), shift.size() * sizeof(int),
+                          hipMemcpyHostToDevice, st));
     mo = MixedOut{s->B.as<float>(), s->C.as<float>(), s->Bq.as<int8_t>(),
-                  s->Bq_scale.as<double>(), s->ldq, (long long)s->n * s->ldq};
+                  s->Bq_scale.as<double>(), s->ldq / kI8Pad, s->q_shift.as<int>()};
   } else {
     s->Bq.release();
     s->Bq_scale.release();
+    s->q_shift.release();
   }
   s->has_q = mixed;
   if (nblk == 0) return;

@@ -509,11 +509,12 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 
 // One wave per row: the row maximum, its power-of-two scale and the slices
-// of 4 consecutive values per lane (zeros past nd up to ldq).
+// of 4 consecutive values per lane (zeros past nd up to Kq), stored in the
+// tiled slice layout (tr rows per tile).
 template <typename T>
 __global__ __launch_bounds__(256) void k_slice_rows(const T *X, long long ldx, int n, int nd,
-                                                    int8_t *Q, long long ldq,
-                                                    long long slice_stride, double *scale) {
+                                                    int8_t *Q, long long nk, int tr,
+                                                    double *scale) {
   int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (row >= n) return;
   const T *x = X + (size_t)row * ldx;
@@ -525,8 +526,8 @@ __global__ __launch_bounds__(256) void k_slice_rows(const T *X, long long ldx, i
   if (mx > 0) frexp(mx, &E);  // mx = m 2^E, m in [0.5, 1)
   double inv = ldexp(1.0, -E);
   if (lane == 0) scale[row] = ldexp(1.0, E);
-  int8_t *q = Q + (size_t)row * ldq;
-  for (int k0 = 4 * lane; k0 < ldq; k0 += 256) {
+  long long Kq = nk * kI8Pad;
+  for (int k0 = 4 * lane; k0 < Kq; k0 += 256) {
     int packed[kI8Slices];
 #pragma unroll
     for (int p = 0; p < kI8Slices; p++) packed[p] = 0;
@@ -543,179 +544,224 @@ __global__ __launch_bounds__(256) void k_slice_rows(const T *X, long long ldx, i
     }
 #pragma unroll
     for (int p = 0; p < kI8Slices; p++)
-      *reinterpret_cast<int *>(q + p * slice_stride + k0) = packed[p];
+      *reinterpret_cast<int *>(Q + i8_tile_offset(tr, nk, row, k0, p)) = packed[p];
   }
 }
-
-struct I8Ptrs {
-  const int8_t *p[kI8Slices];
-};
 
 // An int32 accumulator sums at most kI8Slices products per k of |a b| <=
 // 64^2 = 4096: 65536 k keep it below 2^31 (5 x 4096 x 65536 = 1.3e9).
 constexpr int kI8Flush = 65536;
 
-// (64 TMb) x 64 tiles, 4 waves (2 x 2), each (32 TMb) x 32 of
+// 16-byte chunk position of k chunk c (0..3) in row r of an LDS slice block
+// (rows of 64 bytes, no padding): an XOR swizzle that keeps the fragment
+// reads (32 rows at one chunk) and the contiguous stores free of bank
+// conflicts for 8- and 16-lane phases.
+__device__ __forceinline__ int i8_chunk(int r, int c) { return c ^ (((r >> 1) ^ (r >> 3)) & 3); }
+
+// 128 x 64 tiles (kI8TileA x kI8TileN), 8 waves (4 x 2), each 32 x 32 of
 // v_mfma_i32_32x32x32_i8 (lane l: operand row l % 32, bytes k = 16 (l / 32)
-// .. + 15; result column l % 32, rows 8 (j / 4) + 4 (l / 32) + j % 4). LDS
-// rows of 64 k-bytes padded to 80, which spreads a fragment read over the
-// banks. Epilogue as EpKernel, in FP64 from the exact sums: S, k (FP64), w as
-// BF16 pairs, energy partials per column tile.
-template <int TMb, bool LONG>
-__global__ __launch_bounds__(256) void k_gemm_i8(int M, int N, int Kq, I8Ptrs Ag, long long lda,
-                                                 const double *sa, I8Ptrs Bg, long long ldb,
+// .. + 15; result column l % 32, rows 8 (j / 4) + 4 (l / 32) + j % 4); two
+// waves per SIMD, so one wave's loads, LDS traffic and barrier waits overlap
+// the other's MFMAs. One k step (64) of the workgroup's row tile and column
+// tile is one contiguous run of each operand (the tiled slice layout), loaded
+// with 16-byte vectors into registers one step ahead and stored to one of two
+// LDS buffers after the step before it computes: one barrier per k step. The slice
+// products run in rounds (round r: slice r of A against each slice of B it
+// pairs with), so consecutive MFMAs accumulate into different diagonals.
+// Epilogue as EpKernel, in FP64 from the exact sums: S, k (FP64), w as BF16
+// pairs, energy partials per column tile; k and w are staged in LDS and
+// stored as whole rows of the tile.
+template <bool LONG>
+__global__ __launch_bounds__(512) void k_gemm_i8(int M, int N, int nk, const int8_t *Ag,
+                                                 const double *sa, const int8_t *Bg,
                                                  const double *sb, EpArgs e) {
-  constexpr int S = kI8Slices, BM = 64 * TMb, BN = 64, BK = 64, LDB = 80;
-  __shared__ __align__(16) int8_t As[S][BM * LDB];
-  __shared__ __align__(16) int8_t Bs[S][BN * LDB];
+  constexpr int S = kI8Slices, BM = kI8TileA, BN = kI8TileN, NT = 512;
+  constexpr int A_SL = BM * kI8Pad, B_SL = BN * kI8Pad;  // bytes of one slice block
+  constexpr int A_ST = S * A_SL, B_ST = S * B_SL;        // bytes of one k step
+  constexpr int NA = A_ST / (16 * NT);                   // 16-byte loads per thread (A)
+  constexpr int NB = (B_ST + 16 * NT - 1) / (16 * NT);   // (B; the last by some threads)
+  static_assert(A_ST % (16 * NT) == 0 && B_ST % (16 * NT / 2) == 0, "load split");
+  static_assert(BM * BN * (8 + 2 + 2) <= 2 * (A_ST + B_ST), "epilogue staging fits");
+  __shared__ __align__(16) int8_t smem[2 * (A_ST + B_ST)];
   __shared__ double red[2 * BM];
+  int8_t *As = smem, *Bs = smem + 2 * A_ST;  // [2][A_ST], [2][B_ST]
   int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   int wr = wid >> 1, wc = wid & 1;
   int tx, ty;
   xcd_tile(tx, ty);
   int row0 = ty * BM, col0 = tx * BN;
-  v16i acc[S][TMb];
+  const int8_t *Ab = Ag + (size_t)ty * nk * A_ST + 16 * tid;
+  const int8_t *Bb = Bg + (size_t)tx * nk * B_ST + 16 * tid;
+  v16i acc[S];
   // With LONG (Kq > kI8Flush) the int32 sums of the slice products
   // (p + q = d) are folded into one FP64 sum every kI8Flush k, before they
-  // could overflow; otherwise once, in the epilogue.
-  double sacc[TMb][16];
+  // could overflow; otherwise once, after the loop.
+  constexpr int SJ = LONG ? 16 : 1;
+  double sacc[SJ];
   auto clear = [&]() {
 #pragma unroll
-    for (int d = 0; d < S; d++)
+    for (int d = 0; d < S; d++) acc[d] = v16i{};
+  };
+  auto diag_sum = [&](int j) {
+    double sum = 0;
 #pragma unroll
-      for (int a = 0; a < TMb; a++) acc[d][a] = v16i{};
+    for (int d = S - 1; d >= 0; d--) sum += ldexp((double)acc[d][j], -(12 + 7 * d));
+    return sum;
   };
   auto fold = [&]() {
+    if constexpr (LONG) {
 #pragma unroll
-    for (int a = 0; a < TMb; a++)
-#pragma unroll
-      for (int j = 0; j < 16; j++) {
-        double sum = 0;
-#pragma unroll
-        for (int d = S - 1; d >= 0; d--) sum += ldexp((double)acc[d][a][j], -(12 + 7 * d));
-        sacc[a][j] += sum;
-      }
-  };
-#pragma unroll
-  for (int a = 0; a < TMb; a++)
-#pragma unroll
-    for (int j = 0; j < 16; j++) sacc[a][j] = 0;
-  clear();
-  // Loader: thread tid takes 16 bytes (k offset 16 (tid % 4)) of rows
-  // tid / 4 + 64 j; rows past M or N read row 0 and load zeros.
-  int lr = tid >> 2, lk = 16 * (tid & 3);
-  v4i ra[S][TMb], rb[S];
-  auto gload = [&](int k0) {
-#pragma unroll
-    for (int p = 0; p < S; p++) {
-#pragma unroll
-      for (int j = 0; j < TMb; j++) {
-        int r = row0 + lr + 64 * j;
-        const v4i *src = reinterpret_cast<const v4i *>(Ag.p[p] + (size_t)(r < M ? r : 0) * lda + k0 + lk);
-        ra[p][j] = r < M ? *src : v4i{};
-      }
-      int c = col0 + lr;
-      const v4i *src = reinterpret_cast<const v4i *>(Bg.p[p] + (size_t)(c < N ? c : 0) * ldb + k0 + lk);
-      rb[p] = c < N ? *src : v4i{};
+      for (int j = 0; j < 16; j++) sacc[j] += diag_sum(j);
     }
   };
-  gload(0);
-#pragma unroll 1
-  for (int k0 = 0; k0 < Kq; k0 += BK) {
+#pragma unroll
+  for (int j = 0; j < SJ; j++) sacc[j] = 0;
+  clear();
+  v4i ra0[NA], rb0[NB];
+  auto gload = [&](auto &ra, auto &rb, int ks) {
+#pragma unroll
+    for (int j = 0; j < NA; j++)
+      ra[j] = *reinterpret_cast<const v4i *>(Ab + (size_t)ks * A_ST + 16 * NT * j);
+#pragma unroll
+    for (int j = 0; j < NB; j++)
+      if (16 * NT * j + 16 * tid < B_ST)
+        rb[j] = *reinterpret_cast<const v4i *>(Bb + (size_t)ks * B_ST + 16 * NT * j);
+  };
+  // Thread tid's 16 bytes at offset 16 tid + 16 NT j of a step: slice block,
+  // row (of 64 bytes) and k chunk; stored at the swizzled chunk.
+  auto lstore = [&](const auto &ra, const auto &rb, int buf) {
+#pragma unroll
+    for (int j = 0; j < NA; j++) {
+      int o = 16 * tid + 16 * NT * j, p = o / A_SL, w = o % A_SL, r = w >> 6;
+      *reinterpret_cast<v4i *>(
+          &As[buf * A_ST + p * A_SL + r * 64 + 16 * i8_chunk(r, (w >> 4) & 3)]) = ra[j];
+    }
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+      int o = 16 * tid + 16 * NT * j, p = o / B_SL, w = o % B_SL, r = w >> 6;
+      if (o < B_ST)
+        *reinterpret_cast<v4i *>(
+            &Bs[buf * B_ST + p * B_SL + r * 64 + 16 * i8_chunk(r, (w >> 4) & 3)]) = rb[j];
+    }
+  };
+  auto compute = [&](int buf) {
+    int ra_ = wr * 32 + (lane & 31), rb_ = wc * 32 + (lane & 31);
+#pragma unroll
+    for (int kk = 0; kk < kI8Pad; kk += 32) {
+      int c = (kk >> 4) + (lane >> 5);
+      v4i af[S], bf[S];
+#pragma unroll
+      for (int p = 0; p < S; p++) {
+        af[p] = *reinterpret_cast<const v4i *>(
+            &As[buf * A_ST + p * A_SL + ra_ * 64 + 16 * i8_chunk(ra_, c)]);
+        bf[p] = *reinterpret_cast<const v4i *>(
+            &Bs[buf * B_ST + p * B_SL + rb_ * 64 + 16 * i8_chunk(rb_, c)]);
+      }
+#pragma unroll
+      for (int rd = 0; rd < S; rd++)
+#pragma unroll
+        for (int d = S - 1; d >= rd; d--)
+          acc[d] = __builtin_amdgcn_mfma_i32_32x32x32_i8(af[rd], bf[d - rd], acc[d], 0, 0, 0);
+    }
+  };
+  auto flush = [&](int ks) {
     if constexpr (LONG) {
-      if (k0 > 0 && k0 % kI8Flush == 0) {
+      if (ks > 0 && (ks * kI8Pad) % kI8Flush == 0) {
         fold();
         clear();
       }
     }
+  };
+  // Before step ks: LDS buffer ks % 2 holds step ks, the registers step ks + 1
+  // (issued after the previous step's barrier; the other wave on each SIMD
+  // computes while they land).
+  gload(ra0, rb0, 0);
+  lstore(ra0, rb0, 0);
+  if (nk > 1) gload(ra0, rb0, 1);
+  __syncthreads();
+#pragma unroll 1
+  for (int ks = 0; ks < nk; ks++) {
+    flush(ks);
+    compute(ks & 1);
+    if (ks + 1 < nk) lstore(ra0, rb0, (ks + 1) & 1);  // last read in step ks - 1
     __syncthreads();
-#pragma unroll
-    for (int p = 0; p < S; p++) {
-#pragma unroll
-      for (int j = 0; j < TMb; j++)
-        *reinterpret_cast<v4i *>(&As[p][(lr + 64 * j) * LDB + lk]) = ra[p][j];
-      *reinterpret_cast<v4i *>(&Bs[p][lr * LDB + lk]) = rb[p];
-    }
-    __syncthreads();
-    if (k0 + BK < Kq) gload(k0 + BK);
-#pragma unroll
-    for (int ks = 0; ks < BK; ks += 32) {
-      int kk = ks + 16 * (lane >> 5);
-      v4i af[S][TMb], bf[S];
-#pragma unroll
-      for (int p = 0; p < S; p++) {
-#pragma unroll
-        for (int a = 0; a < TMb; a++)
-          af[p][a] = *reinterpret_cast<const v4i *>(
-              &As[p][(wr * 32 * TMb + 32 * a + (lane & 31)) * LDB + kk]);
-        bf[p] = *reinterpret_cast<const v4i *>(&Bs[p][(wc * 32 + (lane & 31)) * LDB + kk]);
-      }
-#pragma unroll
-      for (int p = 0; p < S; p++)
-#pragma unroll
-        for (int q = 0; q < S - p; q++)
-#pragma unroll
-          for (int a = 0; a < TMb; a++)
-            acc[p + q][a] =
-                __builtin_amdgcn_mfma_i32_32x32x32_i8(af[p][a], bf[q], acc[p + q][a], 0, 0, 0);
-    }
+    if (ks + 2 < nk) gload(ra0, rb0, ks + 2);
   }
   fold();
-  // Epilogue.
+  // Epilogue: values in FP64 per element, staged in LDS (over the operand
+  // buffers, free after the loop's last barrier) as k (BM x BN FP64) and the
+  // two BF16 pieces of w (BM x BN each).
+  double *Kst = reinterpret_cast<double *>(smem);
+  uint16_t *Hst = reinterpret_cast<uint16_t *>(smem + BM * BN * 8);
+  uint16_t *Lst = Hst + BM * BN;
   int pi = e.power == 2.0 ? 2 : (e.power == 1.0 ? 1 : 0);
-  int c = col0 + wc * 32 + (lane & 31);
+  int cc = wc * 32 + (lane & 31), c = col0 + cc;
   bool cin = c < N;
   double a_c = cin ? e.alpha[c] : 0.0, s_c = cin ? sb[c] : 0.0;
-  double rsum[TMb][16];
+  double rsum[16];
 #pragma unroll
-  for (int a = 0; a < TMb; a++)
-#pragma unroll
-    for (int j = 0; j < 16; j++) {
-      int r = row0 + wr * 32 * TMb + 32 * a + 8 * (j / 4) + 4 * (lane >> 5) + (j & 3);
-      rsum[a][j] = 0;
-      if (r >= M) continue;
-      if (cin) {
-        double sum = sacc[a][j];
-        double nrm = e.row_norm[r];
-        bool empty = nrm < kEmptyThresh;
-        double s = empty ? 0.0 : sum * sa[r] * s_c / nrm;
-        double kv = empty ? 0.0 : e.sig2 * fpow(s, e.power, pi);
-        double pm1 = pi == 2 ? s : (pi == 1 ? 1.0 : fpow(s, e.power - 1, 0));
-        double wv = empty ? 0.0 : e.sig2 * a_c * e.power * pm1;
-        float wf = (float)wv;
-        __bf16 h = (__bf16)wf;
-        reinterpret_cast<__bf16 *>(e.Wh)[(size_t)r * e.ldw + c] = h;
-        reinterpret_cast<__bf16 *>(e.Wl)[(size_t)r * e.ldw + c] = (__bf16)(wf - (float)h);
-        if (e.K) e.K_as<double>()[(size_t)r * e.ldk + c] = kv;
-        rsum[a][j] = kv * a_c;
-      } else if (c < e.ldw) {
-        // Padding columns [N, ldw) of w are zero: the reduction tail of the
-        // gradient contraction.
-        reinterpret_cast<__bf16 *>(e.Wh)[(size_t)r * e.ldw + c] = (__bf16)0.0f;
-        reinterpret_cast<__bf16 *>(e.Wl)[(size_t)r * e.ldw + c] = (__bf16)0.0f;
-      }
+  for (int j = 0; j < 16; j++) {
+    int rr = wr * 32 + 8 * (j / 4) + 4 * (lane >> 5) + (j & 3);
+    int r = row0 + rr;
+    double kv = 0.0;
+    float wf = 0.0f;
+    if (r < M && cin) {
+      double sum;
+      if constexpr (LONG) sum = sacc[j];
+      else sum = 0.0 + diag_sum(j);
+      double nrm = e.row_norm[r];
+      bool empty = nrm < kEmptyThresh;
+      double s = empty ? 0.0 : sum * sa[r] * s_c / nrm;
+      kv = empty ? 0.0 : e.sig2 * fpow(s, e.power, pi);
+      double pm1 = pi == 2 ? s : (pi == 1 ? 1.0 : fpow(s, e.power - 1, 0));
+      wf = (float)(empty ? 0.0 : e.sig2 * a_c * e.power * pm1);
     }
+    __bf16 h = (__bf16)wf, l = (__bf16)(wf - (float)h);
+    Kst[rr * BN + cc] = kv;
+    Hst[rr * BN + cc] = __builtin_bit_cast(uint16_t, h);
+    Lst[rr * BN + cc] = __builtin_bit_cast(uint16_t, l);
+    rsum[j] = kv * a_c;
+  }
   // Row sums over the 32 columns of each half-wave, then over the two wave
   // columns through LDS: one energy partial per (column tile, row).
 #pragma unroll
-  for (int a = 0; a < TMb; a++)
+  for (int j = 0; j < 16; j++) {
+    double v = rsum[j];
 #pragma unroll
-    for (int j = 0; j < 16; j++) {
-      double v = rsum[a][j];
-#pragma unroll
-      for (int off = 16; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-      rsum[a][j] = v;
-    }
+    for (int off = 16; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    rsum[j] = v;
+  }
   if ((lane & 31) == 0) {
 #pragma unroll
-    for (int a = 0; a < TMb; a++)
-#pragma unroll
-      for (int j = 0; j < 16; j++)
-        red[wc * BM + wr * 32 * TMb + 32 * a + 8 * (j / 4) + 4 * (lane >> 5) + (j & 3)] =
-            rsum[a][j];
+    for (int j = 0; j < 16; j++)
+      red[wc * BM + wr * 32 + 8 * (j / 4) + 4 * (lane >> 5) + (j & 3)] = rsum[j];
   }
   __syncthreads();
-  for (int q = tid; q < BM; q += blockDim.x) {
+  // k: rows of the tile as 16-byte pairs of columns (ldk even, 16-byte base).
+  if (e.K) {
+    double *K = e.K_as<double>();
+    for (int q = tid; q < BM * (BN / 2); q += NT) {
+      int rr = q / (BN / 2), c2 = 2 * (q % (BN / 2)), r = row0 + rr, cg = col0 + c2;
+      if (r >= M || cg >= N) continue;
+      double *dst = K + (size_t)r * e.ldk + cg;
+      if (cg + 1 < N)
+        *reinterpret_cast<double2 *>(dst) = *reinterpret_cast<const double2 *>(&Kst[rr * BN + c2]);
+      else
+        dst[0] = Kst[rr * BN + c2];
+    }
+  }
+  // w pieces: 8 columns per 16-byte store; the padding columns [N, ldw) are
+  // zero (the reduction tail of the gradient contraction), none past ldw
+  // (a multiple of 32) is written.
+  for (int q = tid; q < BM * (BN / 8); q += NT) {
+    int rr = q / (BN / 8), c8 = 8 * (q % (BN / 8)), r = row0 + rr, cg = col0 + c8;
+    if (r >= M || cg >= e.ldw) continue;
+    *reinterpret_cast<v4i *>(reinterpret_cast<uint16_t *>(e.Wh) + (size_t)r * e.ldw + cg) =
+        *reinterpret_cast<const v4i *>(&Hst[rr * BN + c8]);
+    *reinterpret_cast<v4i *>(reinterpret_cast<uint16_t *>(e.Wl) + (size_t)r * e.ldw + cg) =
+        *reinterpret_cast<const v4i *>(&Lst[rr * BN + c8]);
+  }
+  for (int q = tid; q < BM; q += NT) {
     int r = row0 + q;
     if (r < M) e.part[(size_t)(col0 / BN) * e.part_ld + r] = red[q] + red[BM + q];
   }
@@ -864,57 +910,53 @@ void split_shape(int i, hipStream_t st, int M, int N, int Kpad, const SplitPtrs 
   }
 }
 
-// The integer contraction runs 128 x 64 tiles: the 64-row tile spills
-// registers, and the column tile is fixed at 64 (the energy partials are
-// per column tile).
-
-template <int TMb>
-void i8_launch(hipStream_t st, int M, int N, int Kq, const I8Ptrs &a, long long lda,
-               const double *sa, const I8Ptrs &b, long long ldb, const double *sb,
-               const EpArgs &e) {
-  dim3 grid((N + kI8TileN - 1) / kI8TileN, (M + 64 * TMb - 1) / (64 * TMb));
-  if (Kq > kI8Flush)
-    hipLaunchKernelGGL((k_gemm_i8<TMb, true>), grid, dim3(256), 0, st, M, N, Kq, a, lda, sa, b,
-                       ldb, sb, e);
+// The integer contraction runs 128 x 64 tiles (kI8TileA x kI8TileN): the
+// column tile is fixed at 64 (the energy partials are per column tile).
+void i8_launch(hipStream_t st, int M, int N, int nk, const int8_t *a, const double *sa,
+               const int8_t *b, const double *sb, const EpArgs &e) {
+  dim3 grid((N + kI8TileN - 1) / kI8TileN, (M + kI8TileA - 1) / kI8TileA);
+  if ((long long)nk * kI8Pad > kI8Flush)
+    hipLaunchKernelGGL((k_gemm_i8<true>), grid, dim3(512), 0, st, M, N, nk, a, sa, b, sb, e);
   else
-    hipLaunchKernelGGL((k_gemm_i8<TMb, false>), grid, dim3(256), 0, st, M, N, Kq, a, lda, sa, b,
-                       ldb, sb, e);
+    hipLaunchKernelGGL((k_gemm_i8<false>), grid, dim3(512), 0, st, M, N, nk, a, sa, b, sb, e);
 }
 
 }  // namespace
 
 void slice_rows(hipStream_t st, const float *X, long long ldx, int n, int nd, int8_t *Q,
-                long long ldq, long long slice_stride, double *scale) {
+                long long Kq, int tr, double *scale) {
   if (n <= 0) return;
+  FM_CHECK(Kq % kI8Pad == 0 && Kq >= nd && tr > 0, FM_ERR_INVALID, "bad slice layout");
   hipLaunchKernelGGL(k_slice_rows<float>, dim3((n + 3) / 4), dim3(256), 0, st, X, ldx, n, nd, Q,
-                     ldq, slice_stride, scale);
+                     Kq / kI8Pad, tr, scale);
   FM_HIP(hipGetLastError());
 }
 
 void slice_rows(hipStream_t st, const double *X, long long ldx, int n, int nd, int8_t *Q,
-                long long ldq, long long slice_stride, double *scale) {
+                long long Kq, int tr, double *scale) {
   if (n <= 0) return;
+  FM_CHECK(Kq % kI8Pad == 0 && Kq >= nd && tr > 0, FM_ERR_INVALID, "bad slice layout");
   hipLaunchKernelGGL(k_slice_rows<double>, dim3((n + 3) / 4), dim3(256), 0, st, X, ldx, n, nd,
-                     Q, ldq, slice_stride, scale);
+                     Q, Kq / kI8Pad, tr, scale);
   FM_HIP(hipGetLastError());
 }
 
 void gemm_i8_kernel(hipStream_t st, int M, int N, int Kq, const SliceOperand &A,
                     const SliceOperand &B, const EpArgs &e) {
   if (M <= 0 || N <= 0) return;
-  FM_CHECK(Kq % kI8Pad == 0 && A.ld % kI8Pad == 0 && B.ld % kI8Pad == 0 && Kq <= A.ld &&
-               Kq <= B.ld,
-           FM_ERR_INVALID, "sliced operands must be padded to 64 bytes");
+  FM_CHECK(Kq > 0 && Kq % kI8Pad == 0 && A.Kq == Kq && B.Kq == Kq, FM_ERR_INVALID,
+           "sliced operands must share a reduction length padded to 64");
+  FM_CHECK(A.tr == kI8TileA && B.tr == kI8TileN, FM_ERR_INVALID,
+           "sliced operands must be tiled by 128 (rows) and 64 (columns)");
+  FM_CHECK(A.q && B.q && A.scale && B.scale &&
+               ((uintptr_t)A.q | (uintptr_t)B.q) % 16 == 0,
+           FM_ERR_INVALID, "sliced operands must be 16-byte aligned");
   FM_CHECK(e.Wh && e.Wl && e.alpha && e.row_norm && e.part, FM_ERR_INVALID,
            "integer contraction: missing epilogue outputs");
-  I8Ptrs a{}, b{};
-  for (int p = 0; p < kI8Slices; p++) {
-    a.p[p] = A.q + p * A.slice_stride;
-    b.p[p] = B.q + p * B.slice_stride;
-    FM_CHECK(((uintptr_t)a.p[p] | (uintptr_t)b.p[p]) % 16 == 0, FM_ERR_INVALID,
-             "sliced operands must be 16-byte aligned");
-  }
-  i8_launch<2>(st, M, N, Kq, a, A.ld, A.scale, b, B.ld, B.scale, e);
+  FM_CHECK(e.ldw % 32 == 0 && e.ldw >= N && ((uintptr_t)e.Wh | (uintptr_t)e.Wl) % 16 == 0 &&
+               (!e.K || (e.ldk % 2 == 0 && e.ldk >= N && (uintptr_t)e.K % 16 == 0)),
+           FM_ERR_INVALID, "integer contraction: k and w must be 16-byte aligned rows");
+  i8_launch(st, M, N, Kq / kI8Pad, A.q, A.scale, B.q, B.scale, e);
   FM_HIP(hipGetLastError());
 }
 

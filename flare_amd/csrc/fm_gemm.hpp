@@ -63,22 +63,37 @@ void gemm_split(hipStream_t st, int ep, int np, int M, int N, int Kpad, const Sp
                 const SplitOperand &B, const EpArgs &e);
 // Exact-integer kernel contraction (the chosen form's S = B X_hat^T with the
 // EP_KERNEL epilogue, k in FP64 and w as BF16 pairs): both operands as
-// kI8Slices int8 slices of their rows, slice p at q + p slice_stride, rows of
-// ld bytes (a multiple of kI8Pad, zero padded), with a power-of-two scale per
-// row. The slices come from slice_rows.
+// kI8Slices int8 slices of their rows with a power-of-two scale per row, in
+// the tiled slice layout below. The slices come from slice_rows (or the
+// chosen form's descriptor pass).
 constexpr int kI8Slices = 5;
-constexpr int kI8Pad = 64;
-constexpr int kI8TileN = 64;  // column tile: the energy partials are per column tile
+constexpr int kI8Pad = 64;     // k step; the reduction length is padded to it with zeros
+constexpr int kI8TileA = 128;  // row tile of the descriptor operand
+constexpr int kI8TileN = 64;   // column tile (sparse set): the energy partials are per column tile
 inline long long i8_ld(long long k) { return (k + kI8Pad - 1) / kI8Pad * kI8Pad; }
+// Tiled slice layout of n rows with reduction length Kq = nk kI8Pad: rows in
+// tiles of tr, and tile (row tile rt, k step ks) is kI8Slices consecutive
+// blocks of tr x kI8Pad bytes (slice, then row, then k), so one k step of a
+// row tile is a single contiguous run of kI8Slices tr kI8Pad bytes. Rows past
+// n in the last tile are never read into a kept result.
+__host__ __device__ inline long long i8_tile_offset(int tr, long long nk, long long row, int k,
+                                                    int p) {
+  return (((row / tr) * nk + (k >> 6)) * kI8Slices + p) * (long long)tr * kI8Pad +
+         (row % tr) * kI8Pad + (k & (kI8Pad - 1));
+}
+inline size_t i8_tiled_bytes(long long n, int tr, long long Kq) {
+  return (size_t)((n + tr - 1) / tr) * tr * Kq * kI8Slices;
+}
 struct SliceOperand {
-  const int8_t *q = nullptr;
-  long long ld = 0, slice_stride = 0;
+  const int8_t *q = nullptr;  // the operand's first tile (row 0 starts a tile)
+  long long Kq = 0;           // padded reduction length, a multiple of kI8Pad
+  int tr = 0;                 // rows per tile
   const double *scale = nullptr;
 };
 void slice_rows(hipStream_t st, const float *X, long long ldx, int n, int nd, int8_t *Q,
-                long long ldq, long long slice_stride, double *scale);
+                long long Kq, int tr, double *scale);
 void slice_rows(hipStream_t st, const double *X, long long ldx, int n, int nd, int8_t *Q,
-                long long ldq, long long slice_stride, double *scale);
+                long long Kq, int tr, double *scale);
 void gemm_i8_kernel(hipStream_t st, int M, int N, int Kq, const SliceOperand &A,
                     const SliceOperand &B, const EpArgs &e);
 // Host-side split of FP32 values into np BF16 pieces, rounding to nearest

@@ -225,10 +225,16 @@ struct fm_structure_s {
   // Density coefficients c (atom-major, n x nrad H, descriptor precision),
   // kept for the adjoint.
   fm::DevBuf C;
-  // Chosen form, B2: the FP64 descriptor rows as int8 slices (kI8Slices x n x
-  // ldq, species-sorted rows) with a power-of-two scale per row, written by
-  // the descriptor pass; the kernel contraction's operand.
-  fm::DevBuf Bq, Bq_scale;
+  // Chosen form, B2: the FP64 descriptor rows as int8 slices with a
+  // power-of-two scale per species-sorted row, written by the descriptor
+  // pass; the kernel contraction's operand. The slices are in the tiled
+  // layout (fm_gemm.hpp) with ldq = the padded reduction length and each
+  // species starting a row tile: species t's rows are slice rows
+  // q_tile_row[t] + (row - type_off[t]) (q_shift[t] = q_tile_row[t] -
+  // type_off[t] on the device).
+  fm::DevBuf Bq, Bq_scale, q_shift;
+  std::vector<long long> q_tile_row;
+  std::vector<int> q_shift_host;
   long long ldq = 0;
   bool has_q = false;
   // Dense descriptor derivatives (FP64, single structure): bonds in the

@@ -247,11 +247,10 @@ static void read_outputs(fm_structure s, const double *o, size_t n_out, const do
 
 
 // Rows per tile of one species in the chosen form: at most T and, when T
-// allows, a whole number of rounds of the kernel contraction's 128 x 128
-// split tiles over the GPU's workgroup slots (three such workgroups fit a
-// CU), so the last round of each launch is not left nearly empty (74 row
-// tiles x 11 column tiles on 768 slots would run a second round of 46).
-// Rows of a tile do not change any result.
+// allows, a whole number of rounds of the kernel contraction's 128 x 64
+// tiles over the GPU's workgroup slots (one such workgroup per CU: its two
+// LDS buffers take 122 KB), so the last round of each launch is not left
+// nearly empty. Rows of a tile do not change any result.
 static int species_tile_rows(fm_context ctx, int T, int Ms, int form) {
   if (form != FM_FORM_CHOSEN || Ms == 0) return T;
   int cus = 0;
@@ -259,10 +258,9 @@ static int species_tile_rows(fm_context ctx, int T, int Ms, int form) {
           hipSuccess ||
       cus <= 0)
     return T;
-  // The integer contraction: 128 x 64 tiles, two workgroups per CU.
-  int per_round = 2 * cus / ((Ms + kI8TileN - 1) / kI8TileN);
-  int rounds = per_round > 0 ? (T / 128) / per_round : 0;
-  return rounds > 0 ? rounds * per_round * 128 : T;
+  int per_round = cus / ((Ms + kI8TileN - 1) / kI8TileN);
+  int rounds = per_round > 0 ? (T / kI8TileA) / per_round : 0;
+  return rounds > 0 ? rounds * per_round * kI8TileA : T;
 }
 
 // The model's sparse rows as int8 slices for the chosen form's kernel
@@ -277,28 +275,28 @@ static void model_slices(fm_model m, hipStream_t st) {
   for (int t = 0; t < m->nos; t++) {
     int Ms = m->M_by_type[t];
     if (Ms == 0) continue;
-    m->Xq[t].alloc(pool, (size_t)kI8Slices * Ms * ldq);
+    m->Xq[t].alloc(pool, i8_tiled_bytes(Ms, kI8TileN, ldq));
     m->Xq_scale[t].alloc(pool, (size_t)Ms * sizeof(double));
-    slice_rows(st, m->X[t].as<double>(), nd, Ms, nd, m->Xq[t].as<int8_t>(), ldq,
-               (long long)Ms * ldq, m->Xq_scale[t].as<double>());
+    slice_rows(st, m->X[t].as<double>(), nd, Ms, nd, m->Xq[t].as<int8_t>(), ldq, kI8TileN,
+               m->Xq_scale[t].as<double>());
   }
   m->xq_ready = true;
 }
 
 // Chosen form: `cnt` FP32 descriptor rows (leading dimension nd) as int8
-// slices in Aq / Aq_scale (for rows without slices of their FP64 values).
+// slices in Aq / Aq_scale (for rows without slices of their FP64 values;
+// Aq holds at least i8_tiled_bytes(cnt, kI8TileA, i8_ld(nd)) bytes).
 static SliceOperand slice_fp32_rows(fm_context ctx, hipStream_t st, const float *rows, int cnt,
                                     int nd, DevBuf &Aq, DevBuf &Aq_scale) {
   long long ldq = i8_ld(nd);
   {
     KScope timer(ctx, KT_SLICE, st);
-    slice_rows(st, rows, nd, cnt, nd, Aq.as<int8_t>(), ldq, (long long)cnt * ldq,
-               Aq_scale.as<double>());
+    slice_rows(st, rows, nd, cnt, nd, Aq.as<int8_t>(), ldq, kI8TileA, Aq_scale.as<double>());
   }
   SliceOperand A;
   A.q = Aq.as<int8_t>();
-  A.ld = ldq;
-  A.slice_stride = (long long)cnt * ldq;
+  A.Kq = ldq;
+  A.tr = kI8TileA;
   A.scale = Aq_scale.as<double>();
   return A;
 }
@@ -311,8 +309,8 @@ static void kernel_contraction_i8(fm_model m, hipStream_t st, int t, const Slice
   int Ms = m->M_by_type[t];
   SliceOperand Bx;
   Bx.q = m->Xq[t].as<int8_t>();
-  Bx.ld = ldq;
-  Bx.slice_stride = (long long)Ms * ldq;
+  Bx.Kq = ldq;
+  Bx.tr = kI8TileN;
   Bx.scale = m->Xq_scale[t].as<double>();
   KScope timer(m->ctx, KT_SGEMM, st);
   gemm_i8_kernel(st, cnt, Ms, (int)ldq, A, Bx, ek);
@@ -356,9 +354,9 @@ static void predict_tiled(fm_model m, fm_structure s, int flags, int form, doubl
   DevBuf U(pool, std::max<size_t>(1, (size_t)n * nc * esz));
   DevBuf fbond(pool, std::max<size_t>(1, (size_t)3 * s->n_bonds * esz));
   DevBuf vir(pool, std::max<size_t>(1, (size_t)6 * n * sizeof(double)));
-  // Tile buffers. k is FP64 in both forms (rows padded to 16 bytes).
+  // Tile buffers. k is FP64 in both forms (rows padded to 128 bytes).
   long long ldm = split_ld(std::max(Mmax, 1));
-  size_t kld = ((size_t)Mmax + 1) / 2 * 2;
+  size_t kld = ((size_t)Mmax + 15) / 16 * 16;  // rows of whole 128-byte lines
   DevBuf Bt(pool, std::max<size_t>(1, (size_t)T * nd * esz));
   DevBuf Gt(pool, std::max<size_t>(1, (size_t)T * nd * esz));
   DevBuf Tq, Tq_scale, Wt, Wh, Wl, Kt;
@@ -366,7 +364,7 @@ static void predict_tiled(fm_model m, fm_structure s, int flags, int form, doubl
     Wt.alloc(pool, std::max<size_t>(1, (size_t)T * Mmax * sizeof(double)));
   } else {
     model_slices(m, st);
-    Tq.alloc(pool, std::max<size_t>(1, (size_t)kI8Slices * T * i8_ld(nd)));
+    Tq.alloc(pool, std::max<size_t>(1, i8_tiled_bytes(T, kI8TileA, i8_ld(nd))));
     Tq_scale.alloc(pool, std::max<size_t>(1, (size_t)T * sizeof(double)));
     Wh.alloc(pool, std::max<size_t>(1, (size_t)T * ldm * sizeof(uint16_t)));
     Wl.alloc(pool, std::max<size_t>(1, (size_t)T * ldm * sizeof(uint16_t)));
@@ -495,13 +493,13 @@ void predict(fm_model m, fm_structure s, int flags, int form, double *mean_efs,
   // Per-species slices of the atom-major w and k matrices (n_s x M_s each)
   // and of the energy / variance partials.
   // The chosen form keeps w as BF16 pairs with padded rows (ws = split_ld).
-  // k is FP64 in both forms, rows padded to 2 values (16 bytes).
+  // k is FP64 in both forms, rows padded to 16 values (whole 128-byte lines).
   std::vector<size_t> wo(nos + 1, 0), eo(nos + 1, 0), qo(nos + 1, 0), so(nos + 1, 0),
       ko(nos + 1, 0), kld(nos, 0);
   for (int t = 0; t < nos; t++) {
     size_t ns = s->n_by_type[t], Ms = m->M_by_type[t];
     wo[t + 1] = wo[t] + ns * Ms;
-    kld[t] = (Ms + 1) / 2 * 2;
+    kld[t] = (Ms + 15) / 16 * 16;  // rows of whole 128-byte lines
     ko[t + 1] = ko[t] + ns * kld[t];
     so[t + 1] = so[t] + ns * (size_t)split_ld((long long)Ms);
     eo[t + 1] = eo[t] + ns * (size_t)std::max(1, gemm_col_tiles(form, (int)Ms));
@@ -527,7 +525,7 @@ void predict(fm_model m, fm_structure s, int flags, int form, double *mean_efs,
     if (!s->has_q) {
       int nmax = 0;
       for (int t = 0; t < nos; t++) nmax = std::max(nmax, s->n_by_type[t]);
-      Aq.alloc(pool, std::max<size_t>(1, (size_t)kI8Slices * nmax * i8_ld(nd)));
+      Aq.alloc(pool, std::max<size_t>(1, i8_tiled_bytes(nmax, kI8TileA, i8_ld(nd))));
       Aq_scale.alloc(pool, std::max<size_t>(1, (size_t)nmax * sizeof(double)));
     }
   }
@@ -575,9 +573,10 @@ void predict(fm_model m, fm_structure s, int flags, int form, double *mean_efs,
     } else {
       SliceOperand A;
       if (s->has_q) {
-        A.q = s->Bq.as<int8_t>() + (size_t)ro * s->ldq;
-        A.ld = s->ldq;
-        A.slice_stride = (long long)n * s->ldq;
+        // Species t's rows start a tile of the structure's slices.
+        A.q = s->Bq.as<int8_t>() + i8_tiled_bytes(s->q_tile_row[t], kI8TileA, s->ldq);
+        A.Kq = s->ldq;
+        A.tr = kI8TileA;
         A.scale = s->Bq_scale.as<double>() + ro;
       } else {
         A = slice_fp32_rows(ctx, st, s->B.as<float>() + (size_t)ro * nd, ns, nd, Aq, Aq_scale);
