@@ -20,6 +20,8 @@ CUTOFF_KIND = {"quadratic": 0, "cosine": 1}
 
 def lib(fast=False):
     name = "libref_cpu_fast.so" if fast else "libref_cpu.so"
+    if not fast and os.environ.get("ORACLE_SANITIZED") == "1":
+        name = os.path.join("_asan", "libref_cpu.so")
     if name in _LIBS:
         return _LIBS[name]
     path = os.path.join(_HERE, name)
