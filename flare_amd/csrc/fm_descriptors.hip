@@ -850,6 +850,15 @@ static int atoms_per_block(size_t lds_per_atom) {
   return best;
 }
 
+// Atoms resident per CU with lds_per_atom bytes each (the residency
+// atoms_per_block maximises).
+static size_t resident_atoms(size_t lds_per_atom) {
+  const size_t cap = 160 * 1024;
+  if (lds_per_atom > cap) return 0;
+  int apb = atoms_per_block(lds_per_atom);
+  return std::min<size_t>(cap / (apb * lds_per_atom), 8) * apb;
+}
+
 // Calls f(std::integral_constant<int, L>) for the l_max values with an
 // unrolled harmonic sweep (those of the common B2 / B3 settings), else with
 // L = -1 (the sweep loops over the run-time l_max).
@@ -935,7 +944,12 @@ void launch_b2(fm_structure s, bool want_b2) {
       w = std::max(w, (size_t)a.nrad * a.H + (size_t)a.nd);
     return w;
   };
-  bool wide = 3 * work_of(64) <= 4 * work_of(32);
+  // 64-bond chunks take half the chunk passes of 32-bond ones; they are used
+  // when they cost at most a third more LDS and keep as many atoms resident
+  // per CU (the mixed pass's staged row makes the two sizes close in LDS,
+  // where only the residency tells them apart).
+  bool wide = 3 * work_of(64) <= 4 * work_of(32) &&
+              resident_atoms(work_of(64) * csz) >= resident_atoms(work_of(32) * csz);
   size_t work = work_of(wide ? 64 : 32);
   size_t per = work * csz;
   int apb = atoms_per_block(per);
