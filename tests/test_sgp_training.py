@@ -228,3 +228,29 @@ def test_json_layout_roundtrip(C, oracle, strucs, tmp_path):
     assert j["Kuu"] is None and j["alpha"] is None
     e2 = C.SparseGP.from_json(path)
     assert e2.Kuu.shape == (0, 0) and e2.alpha.size == 0
+
+
+def test_kuu_and_update_L_inverse(C, oracle, strucs):
+    """Kuu from envs_envs against the kernel's closed form on the oracle's
+    descriptors, sigma^2 (x_i . x_j / |x_i| |x_j|)^p within a species and 0
+    across species (normalized_dot_product.cpp:28-89), and update_L_inverse
+    (sparse_gp.cpp:610-618): L_inv lower triangular with
+    L_inv (Kuu + jitter I) L_inv^T = I and Kuu^-1 = L_inv^T L_inv."""
+    sgp = C.SparseGP([oracle_kernel(C, oracle, 1.7, 2.0)], 0.1, 0.2, 0.3)
+    sgp.add_specific_environments(strucs[0], [0, 1, 3, 4])
+    sgp.add_specific_environments(strucs[1], [2, 3, 5])
+    sgp.update_L_inverse()
+    M = sgp.n_sparse
+    cd = sgp.sparse_descriptors[0]
+    rows, spec = [], []
+    for t in range(len(cd.descriptors)):
+        X = np.asarray(cd.descriptors[t])
+        rows.extend(X / np.linalg.norm(X, axis=1)[:, None])
+        spec.extend([t] * len(X))
+    Xh, spec = np.array(rows), np.array(spec)
+    want = 1.7 ** 2 * (Xh @ Xh.T) ** 2 * (spec[:, None] == spec[None, :])
+    np.testing.assert_allclose(sgp.Kuu, want, rtol=1e-13, atol=1e-14)
+    A = sgp.Kuu + sgp.Kuu_jitter * np.eye(M)
+    assert np.allclose(np.triu(sgp.L_inv, 1), 0)
+    np.testing.assert_allclose(sgp.L_inv @ A @ sgp.L_inv.T, np.eye(M), atol=1e-8)
+    np.testing.assert_allclose(sgp.Kuu_inverse, sgp.L_inv.T @ sgp.L_inv, rtol=1e-14)
