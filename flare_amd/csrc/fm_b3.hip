@@ -268,8 +268,7 @@ struct RowRange {
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_b3(int n, B3Dev t, const T *Cg, const int *row_of_atom,
-                                             RowRange rr, T *B, double *norm, double *nnv,
-                                             __bf16 *B0, __bf16 *B1, int ldp) {
+                                             RowRange rr, T *B, double *norm, double *nnv) {
   extern __shared__ __align__(16) unsigned char smem[];
   int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   int k = blockIdx.x * (blockDim.x >> 6) + w;
@@ -300,20 +299,9 @@ __global__ __launch_bounds__(256) void k_b3(int n, B3Dev t, const T *Cg, const i
       }
       int kd = ls * t.n_tri + tr;
       Brow[kd] = acc;
-      if (B0) {
-        float r = (float)acc;
-        __bf16 h = (__bf16)r;
-        B0[(size_t)brow * ldp + kd] = h;
-        B1[(size_t)brow * ldp + kd] = (__bf16)(r - (float)h);
-      }
       s2 += acc * acc;
     }
   }
-  if (B0)
-    for (int kd = t.nd + lane; kd < ldp; kd += 64) {
-      B0[(size_t)brow * ldp + kd] = (__bf16)0.0f;
-      B1[(size_t)brow * ldp + kd] = (__bf16)0.0f;
-    }
   s2 = wave_sum(s2);
   if (lane == 0) {
     norm[row] = (double)sqrt(s2);
@@ -493,8 +481,7 @@ __global__ __launch_bounds__(512) void k_b3p(int n, B3Dev t, const int *__restri
                                               const int *__restrict__ rt_pk,
                                               const T *__restrict__ rt_w, const T *Cg,
                                               const int *row_of_atom, RowRange rr, T *B,
-                                              double *norm, double *nnv, __bf16 *B0,
-                                              __bf16 *B1, int ldp) {
+                                              double *norm, double *nnv) {
   extern __shared__ __align__(16) unsigned char smem[];
   int k = blockIdx.x;
   if (k >= n) return;
@@ -554,21 +541,10 @@ __global__ __launch_bounds__(512) void k_b3p(int n, B3Dev t, const int *__restri
       T v = bl[kk];
       size_t kd = (size_t)ls * n_tri + kk;
       Brow[kd] = v;
-      if (B0) {
-        float r = (float)v;
-        __bf16 h = (__bf16)r;
-        B0[(size_t)brow * ldp + kd] = h;
-        B1[(size_t)brow * ldp + kd] = (__bf16)(r - (float)h);
-      }
       s2 += v * v;
     }
     __syncthreads();
   }
-  if (B0)
-    for (int kd = t.nd + tid; kd < ldp; kd += nt) {
-      B0[(size_t)brow * ldp + kd] = (__bf16)0.0f;
-      B1[(size_t)brow * ldp + kd] = (__bf16)0.0f;
-    }
   s2 = block_sum(s2, red);
   if (tid == 0) {
     norm[row] = (double)sqrt(s2);
@@ -950,10 +926,10 @@ static size_t b3_pair_lds(const B3Dev &d, size_t esz, bool adjoint, int &nt) {
 }
 
 void launch_b3(fm_structure s) {
-  launch_b3_rows(s, 0, s->n, false, s->B.as<void>(), nullptr, nullptr);
+  launch_b3_rows(s, 0, s->n, false, s->B.as<void>());
 }
 
-void launch_b3_rows(fm_structure s, int r0, int n, bool tile, void *Bout, void *B0, void *B1) {
+void launch_b3_rows(fm_structure s, int r0, int n, bool tile, void *Bout) {
   hipStream_t st = s->ctx->stream;
   B3State *b = b3_state(s);
   B3Dev d = b->dev();
@@ -966,14 +942,14 @@ void launch_b3_rows(fm_structure s, int r0, int n, bool tile, void *Bout, void *
   if (lds) {
     if (f64)
       hipLaunchKernelGGL(k_b3p<double>, dim3(n), dim3(nt), lds, st, n, d, b->rt_off.as<int>(),
-                         b->rt_pk.as<int>(), b->rt_w<double>(), s->C.as<double>(), s->row_of_atom.as<int>(), rr, (double *)Bout,
-                         s->norm.as<double>(), s->nn.as<double>(), (__bf16 *)nullptr,
-                         (__bf16 *)nullptr, 0);
+                         b->rt_pk.as<int>(), b->rt_w<double>(), s->C.as<double>(),
+                         s->row_of_atom.as<int>(), rr, (double *)Bout, s->norm.as<double>(),
+                         s->nn.as<double>());
     else
       hipLaunchKernelGGL(k_b3p<float>, dim3(n), dim3(nt), lds, st, n, d, b->rt_off.as<int>(),
-                         b->rt_pk.as<int>(), b->rt_w<float>(), s->C.as<float>(), s->row_of_atom.as<int>(), rr, (float *)Bout,
-                         s->norm.as<double>(), s->nn.as<double>(), (__bf16 *)B0, (__bf16 *)B1,
-                         (int)split_ld(s->nd));
+                         b->rt_pk.as<int>(), b->rt_w<float>(), s->C.as<float>(),
+                         s->row_of_atom.as<int>(), rr, (float *)Bout, s->norm.as<double>(),
+                         s->nn.as<double>());
     FM_HIP(hipGetLastError());
     return;
   }
@@ -983,13 +959,11 @@ void launch_b3_rows(fm_structure s, int r0, int n, bool tile, void *Bout, void *
   if (f64)
     hipLaunchKernelGGL(k_b3<double>, dim3(nblk), dim3(64 * apb), apb * per, st, n, d,
                        s->C.as<double>(), s->row_of_atom.as<int>(), rr, (double *)Bout,
-                       s->norm.as<double>(), s->nn.as<double>(), (__bf16 *)nullptr,
-                       (__bf16 *)nullptr, 0);
+                       s->norm.as<double>(), s->nn.as<double>());
   else
     hipLaunchKernelGGL(k_b3<float>, dim3(nblk), dim3(64 * apb), apb * per, st, n, d,
                        s->C.as<float>(), s->row_of_atom.as<int>(), rr, (float *)Bout,
-                       s->norm.as<double>(), s->nn.as<double>(), (__bf16 *)B0, (__bf16 *)B1,
-                       (int)split_ld(s->nd));
+                       s->norm.as<double>(), s->nn.as<double>());
   FM_HIP(hipGetLastError());
 }
 

@@ -313,9 +313,8 @@ int b3_size(int nos, int N, int lmax);
 void launch_b3(fm_structure s);
 // B3 rows of one call: all atoms (tile false: atom order, each written at its
 // species-sorted row of Bout) or the species-sorted rows [r0, r0 + n) (tile
-// true: written as rows 0.. n - 1 of Bout, a row tile). B0 / B1: the BF16
-// pieces (chosen form), same row layout with split_ld(nd).
-void launch_b3_rows(fm_structure s, int r0, int n, bool tile, void *Bout, void *B0, void *B1);
+// true: written as rows 0.. n - 1 of Bout, a row tile).
+void launch_b3_rows(fm_structure s, int r0, int n, bool tile, void *Bout);
 // The model's sparse set in the device order of the structure's B3 rows
 // (permuted once; an error if it is already in another order).
 void b3_order_model(fm_model m, fm_structure s);

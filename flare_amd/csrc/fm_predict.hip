@@ -379,7 +379,7 @@ static void predict_tiled(fm_model m, fm_structure s, int flags, int form, doubl
     int Ts = species_tile_rows(ctx, T, Ms, form);
     for (int c0 = 0; c0 < ns; c0 += Ts) {
       int cnt = std::min(Ts, ns - c0), r0 = ro + c0;
-      launch_b3_rows(s, r0, cnt, true, Bt.as<void>(), nullptr, nullptr);
+      launch_b3_rows(s, r0, cnt, true, Bt.as<void>());
       double *ep = Epart.as<double>() + eo[t] + c0;
       if (Ms == 0) {
         hipLaunchKernelGGL(k_fill, dim3((cnt + 255) / 256), dim3(256), 0, st, ep,
