@@ -277,7 +277,9 @@ struct SplitOut {
   __bf16 *p[2];
 };
 
-// The chosen form's outputs of the FP64 descriptor pass (k_b2 with MIXED):
+// The chosen form's outputs of the FP64 descriptor pass (k_b2 with MIXED
+// set; 1 stages the FP64 row in LDS and slices it after B_l, 2 slices each
+// value as it is formed and stages only the slices):
 // the B row and c in FP32 (for the adjoint) and the FP64 row as kI8Slices
 // int8 slices with a power-of-two scale (the kernel contraction's operand,
 // see k_slice_rows in fm_gemm.hip for the slicing).
@@ -289,7 +291,7 @@ struct MixedOut {
   const int *shift;    // per species: slice row = species-sorted row + shift
 };
 
-template <typename T, int LC, int CH, bool MIXED = false>
+template <typename T, int LC, int CH, int MIXED = 0>
 __global__ __launch_bounds__(64 * kAtomsPerBlock) void k_b2(B2Args a, T *B, double *norm,
                                                               double *nn, SplitOut Bs, int ldp,
                                                               T *Cg, int want_b2, int work,
@@ -310,16 +312,44 @@ __global__ __launch_bounds__(64 * kAtomsPerBlock) void k_b2(B2Args a, T *B, doub
   if (!want_b2) return;
   // B_l = C_l C_l^T on the matrix cores (C_l[n][m] = c[n, l^2 + m]); the
   // upper triangle n2 >= n1 belongs at p(n1, n2) (L + 1) + l of the row. FP64
-  // stores each element there directly (the L2 merges a row's scattered
-  // stores; staging would cost it occupancy); FP32 stages the row in LDS
-  // (over the bond tables, no longer needed) and writes it and its two BF16
-  // pieces with coalesced stores.
+  // and MIXED 2's FP32 row store each element there directly (the L2 merges
+  // a row's scattered stores; staging would cost occupancy); FP32 and MIXED 1
+  // stage the row in LDS (over the bond tables, no longer needed) and write
+  // it with coalesced stores.
   int L1 = a.lmax + 1;
   int r16 = lane & 15, k4 = lane >> 4;
   int row = a.row_of_atom[i];
   T *Brow = B + (size_t)row * a.nd;
   T *bst = tab;
   T s2 = 0;
+  // MIXED 2: the row's power-of-two scale comes before the row itself: |B|'s
+  // largest element is a diagonal one, max_{n,l} sum_m c[n, lm]^2 (Cauchy-
+  // Schwarz within each l), so the values are sliced as they are formed and
+  // only the int8 slices (rows of Kq bytes, zero past nd) are staged in LDS
+  // over the bond tables; the FP32 row goes straight to HBM.
+  double q_inv = 0;
+  long long Kq = 0;
+  int8_t *qst = reinterpret_cast<int8_t *>(tab);
+  if constexpr (MIXED) Kq = mo.nk * kI8Pad;
+  if constexpr (MIXED == 2) {
+    double mx = 0;
+    for (int q = lane; q < nr * L1; q += 64) {
+      int n = q / L1, l = q % L1;
+      const T *cn = c + n * H + l * l;
+      double sq = 0;
+      for (int m = 0; m < 2 * l + 1; m++) sq += (double)cn[m] * (double)cn[m];
+      mx = fmax(mx, sq);
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+    int E = 0;
+    if (mx > 0) frexp(mx, &E);  // mx = m 2^E, m in [0.5, 1)
+    q_inv = ldexp(1.0, -E);
+    if (lane == 0) mo.scale[row] = ldexp(1.0, E);
+    for (int q = a.nd + lane; q < Kq; q += 64)
+#pragma unroll
+      for (int p = 0; p < kI8Slices; p++) qst[p * Kq + q] = 0;
+  }
   for (int l = 0; l <= a.lmax; l++) {
     int nm = 2 * l + 1, lm0 = l * l;
     for (int rt = 0; rt < nr; rt += 16) {
@@ -347,10 +377,20 @@ __global__ __launch_bounds__(64 * kAtomsPerBlock) void k_b2(B2Args a, T *B, doub
           if (n1 < nr && n2 < nr && n2 >= n1) {
             int k = (n1 * nr - (n1 * (n1 - 1)) / 2 + (n2 - n1)) * L1 + l;
             T v = acc[q];
-            if constexpr (sizeof(T) == 4 || MIXED)
+            if constexpr (MIXED == 2) {
+              mo.B[(size_t)row * a.nd + k] = (float)v;
+              double r = (double)v * q_inv;
+#pragma unroll
+              for (int p = 0; p < kI8Slices; p++) {
+                double sl = rint(ldexp(r, 6 + 7 * p));
+                r -= ldexp(sl, -(6 + 7 * p));
+                qst[p * Kq + k] = (int8_t)(int)sl;
+              }
+            } else if constexpr (sizeof(T) == 4 || MIXED == 1) {
               bst[k] = v;
-            else
+            } else {
               Brow[k] = v;
+            }
             s2 += v * v;
           }
         }
@@ -382,39 +422,46 @@ __global__ __launch_bounds__(64 * kAtomsPerBlock) void k_b2(B2Args a, T *B, doub
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    float *B32 = mo.B + (size_t)row * a.nd;
-    double mx = 0;
-    for (int k = lane; k < a.nd; k += 64) {
-      B32[k] = (float)bst[k];
-      mx = fmax(mx, fabs((double)bst[k]));
-    }
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
-    int E = 0;
-    if (mx > 0) frexp(mx, &E);  // mx = m 2^E, m in [0.5, 1)
-    double inv = ldexp(1.0, -E);
-    if (lane == 0) mo.scale[row] = ldexp(1.0, E);
     long long qrow = row + mo.shift[a.species[i]];
-    long long Kq = mo.nk * kI8Pad;
-    for (int k0 = 4 * lane; k0 < Kq; k0 += 256) {
-      int packed[kI8Slices];
-#pragma unroll
-      for (int p = 0; p < kI8Slices; p++) packed[p] = 0;
-#pragma unroll
-      for (int j = 0; j < 4; j++) {
-        int k = k0 + j;
-        double r = k < a.nd ? (double)bst[k] * inv : 0.0;
-#pragma unroll
-        for (int p = 0; p < kI8Slices; p++) {
-          double v = rint(ldexp(r, 6 + 7 * p));
-          r -= ldexp(v, -(6 + 7 * p));
-          packed[p] |= ((int)v & 0xff) << (8 * j);
-        }
+    if constexpr (MIXED == 1) {
+      float *B32 = mo.B + (size_t)row * a.nd;
+      double mx = 0;
+      for (int k = lane; k < a.nd; k += 64) {
+        B32[k] = (float)bst[k];
+        mx = fmax(mx, fabs((double)bst[k]));
       }
 #pragma unroll
-      for (int p = 0; p < kI8Slices; p++)
-        *reinterpret_cast<int *>(mo.q + i8_tile_offset(kI8TileA, mo.nk, qrow, k0, p)) =
-            packed[p];
+      for (int off = 32; off >= 1; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+      int E = 0;
+      if (mx > 0) frexp(mx, &E);  // mx = m 2^E, m in [0.5, 1)
+      double inv = ldexp(1.0, -E);
+      if (lane == 0) mo.scale[row] = ldexp(1.0, E);
+      for (int k0 = 4 * lane; k0 < Kq; k0 += 256) {
+        int packed[kI8Slices];
+#pragma unroll
+        for (int p = 0; p < kI8Slices; p++) packed[p] = 0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          int k = k0 + j;
+          double r = k < a.nd ? (double)bst[k] * inv : 0.0;
+#pragma unroll
+          for (int p = 0; p < kI8Slices; p++) {
+            double v = rint(ldexp(r, 6 + 7 * p));
+            r -= ldexp(v, -(6 + 7 * p));
+            packed[p] |= ((int)v & 0xff) << (8 * j);
+          }
+        }
+#pragma unroll
+        for (int p = 0; p < kI8Slices; p++)
+          *reinterpret_cast<int *>(mo.q + i8_tile_offset(kI8TileA, mo.nk, qrow, k0, p)) =
+              packed[p];
+      }
+    } else {
+      for (int k0 = 4 * lane; k0 < Kq; k0 += 256)
+#pragma unroll
+        for (int p = 0; p < kI8Slices; p++)
+          *reinterpret_cast<int *>(mo.q + i8_tile_offset(kI8TileA, mo.nk, qrow, k0, p)) =
+              *reinterpret_cast<const int *>(&qst[p * Kq + k0]);
     }
   }
   s2 = wave_sum(s2);
@@ -937,20 +984,38 @@ void launch_b2(fm_structure s, bool want_b2) {
   bool mixed = want_b2 && s->form != FM_FORM_FP64;
   size_t esz = s->form == FM_FORM_FP64 ? sizeof(double) : sizeof(float);
   size_t csz = s->form == FM_FORM_FP64 || mixed ? sizeof(double) : sizeof(float);
-  // FP32 and the mixed pass stage the B row after c (over the bond tables).
-  auto work_of = [&](int ch) {
+  // FP32 and mixed mode 1 stage the B row after c (over the bond tables);
+  // mixed mode 2 stages only the row's int8 slices (kI8Slices x Kq bytes).
+  auto work_of = [&](int ch, int mode) {
     size_t w = b2_work(a.nrad * a.H, a.H, a.N, ch);
-    if (want_b2 && s->form != FM_FORM_FP64)
+    if (want_b2 && s->form != FM_FORM_FP64 && mode != 2)
       w = std::max(w, (size_t)a.nrad * a.H + (size_t)a.nd);
+    if (mode == 2)
+      w = std::max(w, (size_t)a.nrad * a.H +
+                          ((size_t)kI8Slices * (size_t)i8_ld(a.nd) + csz - 1) / csz);
     return w;
   };
   // 64-bond chunks take half the chunk passes of 32-bond ones; they are used
   // when they cost at most a third more LDS and keep as many atoms resident
-  // per CU (the mixed pass's staged row makes the two sizes close in LDS,
-  // where only the residency tells them apart).
-  bool wide = 3 * work_of(64) <= 4 * work_of(32) &&
-              resident_atoms(work_of(64) * csz) >= resident_atoms(work_of(32) * csz);
-  size_t work = work_of(wide ? 64 : 32);
+  // per CU (a staged row makes the two sizes close in LDS, where only the
+  // residency tells them apart).
+  auto chunk_of = [&](int mode) {
+    bool wide = 3 * work_of(64, mode) <= 4 * work_of(32, mode) &&
+                resident_atoms(work_of(64, mode) * csz) >= resident_atoms(work_of(32, mode) * csz);
+    return wide ? 64 : 32;
+  };
+  // Mixed mode 2 (slices formed with the values, FP32 row stored directly)
+  // spends more ALU per element than mode 1 (FP64 row staged, sliced after)
+  // and pays off only where its smaller LDS footprint keeps more atoms
+  // resident per CU.
+  int mode = 0;
+  if (mixed) {
+    size_t w1 = work_of(chunk_of(1), 1), w2 = work_of(chunk_of(2), 2);
+    mode = resident_atoms(w2 * csz) > resident_atoms(w1 * csz) ? 2 : 1;
+  }
+  int chunk = chunk_of(mode);
+  bool wide = chunk == 64;
+  size_t work = work_of(chunk, mode);
   size_t per = work * csz;
   int apb = atoms_per_block(per);
   size_t lds = apb * per;
@@ -999,10 +1064,17 @@ void launch_b2(fm_structure s, bool want_b2) {
                            s->B.as<double>(), s->norm.as<double>(), s->nn.as<double>(), none, 0,
                            s->C.as<double>(), (int)want_b2, (int)work, mo);
       });
-    else if (mixed)
+    else if (mode == 1)
       with_lmax(a.lmax, [&](auto lc) {
         constexpr int LC = decltype(lc)::value;
-        hipLaunchKernelGGL((k_b2<double, LC, CH, true>), dim3(nblk), dim3(64 * apb), lds, st,
+        hipLaunchKernelGGL((k_b2<double, LC, CH, 1>), dim3(nblk), dim3(64 * apb), lds, st,
+                           a, (double *)nullptr, s->norm.as<double>(), s->nn.as<double>(), none,
+                           0, (double *)nullptr, (int)want_b2, (int)work, mo);
+      });
+    else if (mode == 2)
+      with_lmax(a.lmax, [&](auto lc) {
+        constexpr int LC = decltype(lc)::value;
+        hipLaunchKernelGGL((k_b2<double, LC, CH, 2>), dim3(nblk), dim3(64 * apb), lds, st,
                            a, (double *)nullptr, s->norm.as<double>(), s->nn.as<double>(), none,
                            0, (double *)nullptr, (int)want_b2, (int)work, mo);
       });
