@@ -55,14 +55,19 @@ struct TileOut {
 // run of logical tiles. With column index fastest (the default) the
 // workgroups sharing a row tile of A share that XCD's L2; with row index
 // fastest those sharing a column tile of B do.
-__device__ __forceinline__ void xcd_tile(int &tx, int &ty, bool rows_fastest = false) {
+__device__ __forceinline__ int xcd_id() {
   constexpr int kXcd = 8;
   int ntx = gridDim.x, nty = gridDim.y, nblk = ntx * nty;
   int pid = blockIdx.y * ntx + blockIdx.x;
   int per = (nblk + kXcd - 1) / kXcd;
   int tall = nblk % kXcd == 0 ? kXcd : nblk % kXcd;  // XCDs that get `per` ids
   int xcd = pid % kXcd, local = pid / kXcd;
-  int id = xcd < tall ? xcd * per + local : tall * per + (xcd - tall) * (per - 1) + local;
+  return xcd < tall ? xcd * per + local : tall * per + (xcd - tall) * (per - 1) + local;
+}
+
+__device__ __forceinline__ void xcd_tile(int &tx, int &ty, bool rows_fastest = false) {
+  int ntx = gridDim.x, nty = gridDim.y;
+  int id = xcd_id();
   if (rows_fastest) {
     ty = id % nty;
     tx = id / nty;
@@ -70,6 +75,19 @@ __device__ __forceinline__ void xcd_tile(int &tx, int &ty, bool rows_fastest = f
     tx = id % ntx;
     ty = id / ntx;
   }
+}
+
+// As xcd_tile, with the logical tiles of each XCD's run taken in groups of
+// G row tiles, row index fastest within a group: the workgroups an XCD holds
+// at once then cover G row tiles by (resident / G) column tiles instead of
+// (resident / columns) row tiles by every column tile, so fewer A and B tile
+// bytes share that XCD's L2 per k step.
+__device__ __forceinline__ void xcd_tile_grouped(int &tx, int &ty, int G) {
+  int ntx = gridDim.x, nty = gridDim.y;
+  int id = xcd_id();
+  int gw = G * ntx, first = id / gw * G, gs = min(G, nty - first), r = id % gw;
+  tx = r / gs;
+  ty = first + r % gs;
 }
 
 template <typename Acc>
@@ -588,7 +606,7 @@ __global__ __launch_bounds__(512) void k_gemm_i8(int M, int N, int nk, const int
   int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   int wr = wid >> 1, wc = wid & 1;
   int tx, ty;
-  xcd_tile(tx, ty);
+  xcd_tile_grouped(tx, ty, kI8Group);
   int row0 = ty * BM, col0 = tx * BN;
   const int8_t *Ab = Ag + (size_t)ty * nk * A_ST + 16 * tid;
   const int8_t *Bb = Bg + (size_t)tx * nk * B_ST + 16 * tid;
